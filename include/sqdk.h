@@ -1,0 +1,246 @@
+/*
+ * sqdk.h -- C ABI of the MI355X SQD kernel library (librqsqd.so).
+ *
+ * The library implements the hot path of sample-based quantum diagonalization
+ * (rqutils/sqd.py, rqutils/ground_locg.py): the matrix-free Pauli-sum matvec
+ * over a sorted subspace of bitstrings and the single-vector LOBPCG that
+ * drives it. Every entry point takes plain device pointers and sizes, returns
+ * 0 on success or a negative status, and never throws; sqdk_last_error()
+ * gives the message of the last failure on the calling thread. The library
+ * allocates no device memory for data: callers pass buffers (sized by the
+ * *_workspace queries) and a HIP stream (hipStream_t passed as void*).
+ *
+ * Layout conventions
+ *   key     uint64: sum_q s_q 2^q of a bitstring (qubit q = bit q). Ascending
+ *           integer order equals the reference's lexicographic order of
+ *           bit-packed padded rows (rqutils/sqd.py:237, 424-426), so sorted
+ *           order and source indices agree integer for integer.
+ *   bucket  keys sharing key >> bucket_bits. bstart[b] is the index of the
+ *           first key of bucket b; the aligned buckets are the unit of the
+ *           matvec's LDS staging.
+ *   group   Pauli terms sharing one X mask. Term k of group g lives at
+ *           grp_off[g] <= k < grp_off[g+1] with Z mask zmask[k] and phased
+ *           coefficient coef[2k] + i coef[2k+1] = c_k (-i)^{popcount(x&z)}
+ *           (rqutils/paulis/symplectic.py:107-112).
+ *   class   off-diagonal groups sharing x >> bucket_bits (the "high part"):
+ *           for a row bucket a all of them read target bucket a ^ high.
+ *   vectors complex, interleaved (re, im): double for complex128, float for
+ *           complex64.
+ */
+#ifndef SQDK_H
+#define SQDK_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQDK_ABI_VERSION 2
+
+enum sqdk_status {
+  SQDK_OK = 0,
+  SQDK_EINVAL = -1,   /* bad argument (maps to ValueError) */
+  SQDK_ERANGE = -2,   /* size beyond a supported limit */
+  SQDK_EHIP = -3,     /* HIP runtime error (maps to RuntimeError) */
+  SQDK_ENOSPC = -4,   /* workspace or output buffer too small */
+  SQDK_ECALLBACK = -5 /* an external matvec callback reported failure */
+};
+
+enum sqdk_precision { SQDK_C128 = 0, SQDK_C64 = 1 };
+
+/* Sorted subspace S with its bucket directories (device pointers). Replaces
+ * the uniquified state array states_u of rqutils/sqd.py:316 (rows past the
+ * unique count are not stored: n is the unique count). */
+typedef struct sqdk_space {
+  const uint64_t* keys;    /* [n] strictly ascending */
+  const uint32_t* bstart;  /* [2^(num_qubits - bucket_bits) + 1] */
+  const uint16_t* fdir;    /* [2^(num_qubits - bucket_bits) << fine_bits]:
+                              keys of bucket b below sub-bucket e */
+  const uint32_t* items;   /* [n_items] first row of each row chunk */
+  const uint64_t* pc_mask; /* [2]: bit w set iff some key has popcount w */
+  int64_t n;               /* 1 <= n < 2^31 */
+  int32_t num_qubits;      /* key width, 1..64 */
+  int32_t bucket_bits;     /* m: bucket = key >> m; num_qubits - m <= 26 */
+  int32_t fine_bits;       /* d <= min(m, 14) */
+  int32_t chunk_rows;      /* rows per item; items never cross a bucket */
+  int32_t n_items;
+  int32_t max_bucket;      /* keys in the largest bucket (<= 16384) */
+} sqdk_space;
+
+/* Pauli sum grouped by X mask and ordered by class (device pointers).
+ * Replaces the packed arrays of PauliSumXZ (rqutils/paulis/symplectic.py:
+ * 42-49): x u8[J,Bp] -> xmask, z u8[J,Kmax,Bp] -> zmask + grp_off (ragged
+ * instead of zero-padded), c [J,Kmax] -> coef / coef32. */
+typedef struct sqdk_ham {
+  const uint64_t* xmask;    /* [J] off-diagonal groups in class order */
+  const int32_t* grp_off;   /* [J+1] term ranges of those groups */
+  const uint64_t* zmask;    /* [T] */
+  const double* coef;       /* [2T] complex128 phased coefficients */
+  const float* coef32;      /* [2T] the same rounded to complex64 (or NULL) */
+  const uint32_t* class_hi; /* [C] x >> bucket_bits of each class */
+  const int32_t* class_off; /* [C+1] group ranges of the classes */
+  int32_t n_groups;         /* J (off-diagonal groups only) */
+  int32_t n_terms;          /* T (all terms, diagonal ones included) */
+  int32_t n_classes;        /* C */
+  int32_t diag_t0, diag_t1; /* term range of the X = 0 group (empty if none) */
+  int32_t num_qubits;
+  int32_t bucket_bits;      /* must equal the space's */
+  int32_t reserved;
+} sqdk_ham;
+
+/* Launch tuning of the bucket-join matvec (0 = library default). */
+typedef struct sqdk_tune {
+  int32_t rows_per_thread; /* 4, 8 or 16; must match chunk_rows / 256 */
+  int32_t use_pc_filter;   /* skip targets whose popcount no key has */
+  int32_t xcd_remap;       /* map consecutive items onto one XCD */
+  int32_t reserved;
+} sqdk_tune;
+
+/* ---- library ----------------------------------------------------------- */
+
+/* No reference counterpart: ABI handshake for the Python loader. */
+int sqdk_abi_version(void);
+
+/* Thread-local message of the last failing call (empty string if none). */
+const char* sqdk_last_error(void);
+
+/* ---- subspace ---------------------------------------------------------- */
+
+/* Check keys and summarise them: err_out[0] (device int32) receives the
+ * number of out-of-order (not strictly ascending) or too-wide keys, pc_mask
+ * (device, 2 words) the popcounts present. No reference counterpart: the
+ * reference's inputs are uniquified by sorting (rqutils/sqd.py:414-445). */
+int sqdk_space_scan(const uint64_t* keys, int64_t n, int32_t num_qubits,
+                    uint64_t* pc_mask, int32_t* err_out, void* stream);
+
+/* dir[p] = number of keys < (p << shift) for 0 <= p <= 2^(num_qubits-shift)
+ * (uint32). Used for the bucket directory and to choose bucket_bits. */
+int sqdk_space_dir(const uint64_t* keys, int64_t n, int32_t num_qubits,
+                   int32_t shift, uint32_t* dir, void* stream);
+
+/* Fine directory: fdir[(b << fine_bits) + e] = number of keys of bucket b
+ * whose sub-bucket ((key >> (bucket_bits - fine_bits)) & (2^fine_bits - 1))
+ * is below e. Buckets must hold fewer than 65536 keys. */
+int sqdk_space_fine(const uint64_t* keys, int64_t n, int32_t num_qubits,
+                    int32_t bucket_bits, int32_t fine_bits,
+                    const uint32_t* bstart, uint16_t* fdir, void* stream);
+
+/* Row chunks of at most chunk_rows rows that never cross a bucket boundary:
+ * items[k] = first row of chunk k; the count goes to n_items_out (device
+ * int32). Fails with SQDK_ENOSPC (count still written) if max_items is short. */
+int sqdk_space_items(const uint32_t* bstart, int64_t n_buckets,
+                     int32_t chunk_rows, uint32_t* items, int64_t max_items,
+                     int32_t* n_items_out, void* stream);
+
+/* xsource[i] = j with keys[j] == keys[i] ^ xmask, else -1 (int32).
+ * Replaces get_xsource (rqutils/sqd.py:448-503); bit-exact. */
+int sqdk_xsource(const sqdk_space* space, uint64_t xmask, int32_t* out,
+                 void* stream);
+
+/* Coefficient vector of one X group: out[i] = sum_k coef_k
+ * (-1)^{popcount(zmask_k & key_i)} over terms t0 <= k < t1, complex128
+ * (2n doubles). Replaces get_diagonal (rqutils/sqd.py:554-575); zero
+ * coefficients are skipped, not a stop (the reference stops at them). */
+int sqdk_diagonal(const uint64_t* zmask, const double* coef, int32_t t0,
+                  int32_t t1, const uint64_t* keys, int64_t n, double* out,
+                  void* stream);
+
+/* ---- matrix-free matvec ------------------------------------------------ */
+
+/* out = H v over the subspace: per row, the X = 0 group's coefficient times
+ * v[row] plus, per off-diagonal group, one XOR and one search of the target
+ * in the LDS-staged target bucket. Replaces apply_h / apply_h_x_cached and
+ * siblings (rqutils/sqd.py:590-684). precision SQDK_C128: double storage and
+ * FP64 sums; SQDK_C64: float storage, FP32 two-sum accumulation. */
+int sqdk_matvec(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
+                const void* v, void* out, const sqdk_tune* tune, void* stream);
+
+/* ---- vector kernels (LOBPCG building blocks) --------------------------- */
+
+/* out[k] = sum_i conj(a_k[i]) b_k[i] for k < npairs (<= 16), accumulated in
+ * double in a fixed order; results (complex double, 2*npairs) go to device
+ * memory. Replaces the jnp.sum(conj(a)*b) reductions of
+ * rqutils/ground_locg.py:214-421. work: sqdk_dots_workspace bytes. */
+int sqdk_dots(int64_t n, int32_t precision, int32_t npairs,
+              const void* const* a, const void* const* b, double* out,
+              void* work, size_t work_bytes, void* stream);
+int sqdk_dots_workspace(int64_t n, int32_t npairs, size_t* bytes);
+
+/* Convert between complex128 and complex64 vectors (n elements). */
+int sqdk_convert(int64_t n, int32_t from_precision, const void* in,
+                 int32_t to_precision, void* out, void* stream);
+
+/* ---- single-vector LOBPCG ---------------------------------------------- */
+
+/* External matvec: write A*in into out (both n complex elements of the
+ * solver's precision) on the given stream; return 0 on success. */
+typedef int (*sqdk_matvec_fn)(void* ctx, const void* in, void* out, void* stream);
+
+enum sqdk_locg_mode {
+  SQDK_LOCG_REFERENCE = 0, /* 4 matvecs/iteration, structure of ground_locg.py:288-341 */
+  SQDK_LOCG_FUSED = 1      /* 1 matvec/iteration, implicit products, Gram-based RR */
+};
+
+enum sqdk_locg_criterion {
+  SQDK_CRIT_REFERENCE = 0, /* |r| < tol * 10 * n_for_tol * (|Ax| - theta), ground_locg.py:324-331 */
+  SQDK_CRIT_RELATIVE = 1   /* |A x - theta x| <= tol * |theta| */
+};
+
+typedef struct sqdk_locg_opts {
+  int32_t mode;            /* sqdk_locg_mode */
+  int32_t precision;       /* SQDK_C128 or SQDK_C64 */
+  int32_t criterion;       /* sqdk_locg_criterion */
+  int32_t maxiter;
+  double tol;
+  int64_t n_for_tol;       /* vector length used by SQDK_CRIT_REFERENCE */
+  int32_t check_every;     /* host convergence poll period in iterations (>=1) */
+  int32_t refresh_every;   /* FUSED: recompute A x and A p explicitly every k iterations */
+  int32_t use_graph;       /* capture check_every iterations as one HIP graph */
+  int32_t reserved;
+} sqdk_locg_opts;
+
+typedef struct sqdk_locg_result {
+  double eigval;           /* Rayleigh-Ritz value theta of the returned vector */
+  double resid_norm;       /* |r| of the last iterate as tracked by the solver */
+  double ax_norm;          /* |A x| of the last iterate */
+  int32_t niter;           /* loop iterations, counted as ground_locg.py:366-372 */
+  int32_t converged;
+  int32_t matvecs;
+  int32_t status;
+} sqdk_locg_result;
+
+/* The operator: the built-in Pauli-sum matvec (ham + space + tune) when fn is
+ * NULL, else the external callback. */
+typedef struct sqdk_op {
+  const sqdk_ham* ham;
+  const sqdk_space* space;
+  const sqdk_tune* tune;
+  sqdk_matvec_fn fn;
+  void* ctx;
+} sqdk_op;
+
+/* Device workspace (bytes) for a solve of n elements. */
+int sqdk_locg_workspace(int64_t n, const sqdk_locg_opts* opts, size_t* bytes);
+
+/* Single-vector LOBPCG for the lowest eigenpair of a Hermitian operator.
+ * Replaces ground_locg / _ground_locg_callable (rqutils/ground_locg.py:128-377).
+ * x (n complex, solver precision, device) holds the start vector on entry and
+ * the normalised eigenvector on return. */
+int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x,
+                    const sqdk_locg_opts* opts, void* work, size_t work_bytes,
+                    sqdk_locg_result* result, void* stream);
+
+/* Lowest eigenpair of one Hermitian 3x3 (Kopp 2008, as ground_locg.py:438-471)
+ * or 2x2 (ground_locg.py:424-435) matrix, evaluated by the device code path:
+ * mat (device) is 2*dim*dim doubles (row-major complex); out (device) receives
+ * the eigenvalue then 2*dim doubles of eigenvector. dim = 2 or 3. */
+int sqdk_small_eigenpair(int32_t dim, const double* mat, double* out,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SQDK_H */

@@ -1,0 +1,17 @@
+"""CPU oracle for the SQD hot path: test infrastructure, not product code.
+
+This package restates, in NumPy FP64 / complex128, the algorithms of the
+reference's sample-based quantum diagonalization path:
+
+- ``sqd_oracle``: Pauli-sum packing and grouping (rqutils/paulis/symplectic.py),
+  state packing, uniquification, the sort-join source-index search, the
+  per-group diagonal and the cached-xsource matvec (rqutils/sqd.py);
+- ``locg_oracle``: the single-vector LOBPCG with its projection and the
+  closed-form 2x2 / 3x3 Hermitian eigenpairs (rqutils/ground_locg.py);
+- ``dense``: an independent dense Kronecker-product construction of Pauli
+  sums, used to pin the restatement itself.
+
+Only the test suite, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg
+of ``bench.py`` may import this package, and only as the checker or as the
+timed CPU baseline. The product (``rqutils_amd``) never imports it.
+"""

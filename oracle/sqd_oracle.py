@@ -1,0 +1,220 @@
+"""NumPy FP64 restatement of the reference's SQD operator path (test oracle).
+
+Test infrastructure only: the product never imports this module. Every
+function restates one reference function in the reference's own data layout
+(bit-packed ``uint8`` rows with a leading pad bit) and cites the lines it
+follows. Where the reference has a known defect, the default here is the
+mathematically intended behaviour and the reference's behaviour is available
+behind a flag (documented per function).
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+
+__all__ = [
+    "from_paulisum", "pack_states", "uniquify_states", "get_xsource",
+    "xsource_by_lookup", "get_diagonal", "apply_xgrp", "apply_h_x_cached",
+    "apply_h", "vinit_from_min_diag", "hproj_coo", "packed_to_keys",
+    "PHASES",
+]
+
+# (-i)^k for k = popcount(x & z) mod 4 -- rqutils/paulis/symplectic.py:110-111
+PHASES = np.array([1.0, -1.0j, -1.0, 1.0j])
+
+
+def from_paulisum(paulis, coeffs, force_real=False, add_padding=False):
+    """Group Pauli strings by X signature; tuple-input path.
+
+    Follows rqutils/paulis/symplectic.py:58-76 (parse, drop zero inputs, merge
+    duplicates), :93-96 (force_real), :98-115 (group by unique X, phase-fold
+    c * (-i)^{popcount(x&z)}, downcast to real when every phased coefficient
+    is real), :117-124 (pad bit, packbits).
+
+    Returns ``(x u8[J, Bp], z u8[J, Kmax, Bp], c [J, Kmax], num_qubits)``.
+    """
+    if len(paulis) != len(coeffs):
+        raise ValueError("Lengths of Pauli and coeff lists do not match")
+    if len(set(len(p) for p in paulis)) != 1:
+        raise ValueError("Pauli strings have non-uniform lengths")
+    coeffs = np.array(coeffs)
+    chars = np.array([list(p.upper()) for p in paulis])
+    nonzero = np.nonzero(coeffs)
+    coeffs = coeffs[nonzero]
+    chars = chars[nonzero]
+    chars, inverse = np.unique(chars, axis=0, return_inverse=True)
+    inverse = inverse.reshape(-1)
+    merged = np.zeros(chars.shape[0], dtype=np.result_type(coeffs.dtype, np.float64))
+    np.add.at(merged, inverse, coeffs)
+    coeffs = merged
+    xbits = np.logical_or(chars == "X", chars == "Y")
+    zbits = np.logical_or(chars == "Y", chars == "Z")
+    num_qubits = chars.shape[1]
+
+    if force_real:
+        if np.any(np.imag(coeffs) != 0.0):
+            warnings.warn("Found nonzero imaginary part when force_real=True")
+        coeffs = np.real(coeffs)
+
+    xuniq, indices, counts = np.unique(xbits, axis=0, return_inverse=True,
+                                       return_counts=True)
+    indices = indices.reshape(-1)
+    xsig = xuniq.astype(np.uint8)
+    shape = (xsig.shape[0], int(np.max(counts)))
+    zsig = np.zeros(shape + (num_qubits,), dtype=np.uint8)
+    phc = np.zeros(shape, dtype=np.complex128)
+    for g in range(xsig.shape[0]):
+        members = np.nonzero(indices == g)[0]
+        zs = zbits[members].astype(np.uint8)
+        zsig[g, : counts[g]] = zs
+        ph = np.sum(xsig[g] & zs, axis=1) & 3
+        phc[g, : counts[g]] = coeffs[members] * PHASES[ph]
+    if np.all(phc.imag == 0.0):
+        phc = phc.real
+    if add_padding:
+        xsig = np.pad(xsig, ((0, 0), (1, 0)))
+        zsig = np.pad(zsig, ((0, 0), (0, 0), (1, 0)))
+    return np.packbits(xsig, axis=-1), np.packbits(zsig, axis=-1), phc, num_qubits
+
+
+def pack_states(states):
+    """Bit-pack binary rows behind a zero pad bit -- rqutils/sqd.py:237."""
+    states = np.asarray(states).astype(np.uint8)
+    return np.packbits(np.pad(states, ((0, 0), (1, 0))), axis=1)
+
+
+def uniquify_states(states_p, states_size):
+    """Sorted unique rows, filled with 0xFF to ``states_size`` rows.
+
+    Restates rqutils/sqd.py:414-445 (multi-key sort, adjacent-unique,
+    compaction, fill value 255). Returns ``(states_u, n_unique)``.
+    """
+    uniq = np.unique(states_p, axis=0)
+    out = np.full((states_size, states_p.shape[1]), 255, dtype=np.uint8)
+    out[: uniq.shape[0]] = uniq
+    return out, uniq.shape[0]
+
+
+def get_xsource(xsig, states):
+    """Index of ``S[i] ^ x`` in ``S`` or -1, by the reference's sort-join.
+
+    Restates rqutils/sqd.py:476-500 step by step: stack ``[S; S^x]``, stable
+    multi-key sort carrying the stack index, equal-neighbour test, then the
+    ``cumsum(bincount(cumsum))`` compaction back to row order. This keeps the
+    reference's behaviour on repeated fill rows as well.
+    """
+    size = states.shape[0]
+    mapped = np.bitwise_xor(states, xsig[None, :])
+    joined = np.concatenate([states, mapped], axis=0)
+    # np.lexsort is stable; the last key is primary, so reverse the columns.
+    order = np.lexsort(joined.T[::-1]).astype(np.int64)
+    jsorted = joined[order]
+    equal = np.all(jsorted[:-1] == jsorted[1:], axis=1)
+    source_idx = np.where(equal, order[1:] - size, -1)
+    is_state = (order < size).astype(np.int64)
+    tposition = np.cumsum(np.bincount(np.cumsum(is_state), minlength=size)[:size])
+    xsource = np.full(size, -1, dtype=np.int64)
+    ok = tposition < source_idx.shape[0]
+    xsource[ok] = source_idx[tposition[ok]]
+    # Negative values other than -1 are out-of-range gathers in the reference
+    # (mode='fill', wrap_negative_indices=False): they read as "no source".
+    xsource[xsource < 0] = -1
+    return xsource.astype(np.int32)
+
+
+def xsource_by_lookup(xsig, states):
+    """Brute-force dictionary lookup of ``S[i] ^ x``; pins ``get_xsource``."""
+    table = {row.tobytes(): i for i, row in enumerate(states)}
+    mapped = np.bitwise_xor(states, xsig[None, :])
+    return np.array([table.get(row.tobytes(), -1) for row in mapped], dtype=np.int32)
+
+
+_POPCOUNT8 = np.array([bin(i).count("1") for i in range(256)], dtype=np.uint8)
+
+
+def get_diagonal(zsigs, coeffs, states, stop_at_zero=False):
+    """Composed diagonal of one X group -- rqutils/sqd.py:554-575.
+
+    ``C[i] = sum_k c_k (-1)^{popcount(S[i] & z_k)}``. The reference stops at
+    the first zero coefficient (sqd.py:562-564); with ``stop_at_zero=False``
+    (default) zero terms are skipped instead, which is the intended result.
+    """
+    diag = np.zeros(states.shape[0], dtype=np.result_type(coeffs.dtype, np.float64))
+    for k in range(coeffs.shape[0]):
+        if coeffs[k] == 0:
+            if stop_at_zero:
+                break
+            continue
+        parity = np.sum(_POPCOUNT8[states & zsigs[k][None, :]], axis=1, dtype=np.int64) & 1
+        diag = diag + coeffs[k] * (1.0 - 2.0 * parity)
+    return diag
+
+
+def apply_xgrp(xsource, diagonal, vec):
+    """Gather ``v[xsource]`` with fill 0, times the diagonal -- sqd.py:578-587."""
+    gathered = np.where(xsource >= 0, vec[np.maximum(xsource, 0)], 0)
+    return gathered * diagonal
+
+
+def apply_h_x_cached(vec, xsources, zsigs, coeffs, states):
+    """``Hv`` from cached source indices -- rqutils/sqd.py:641-654.
+
+    Accumulates the groups in their stored order (ascending X signature).
+    """
+    out = np.zeros_like(vec, dtype=np.result_type(vec.dtype, coeffs.dtype))
+    for g in range(xsources.shape[0]):
+        out = out + apply_xgrp(xsources[g], get_diagonal(zsigs[g], coeffs[g], states), vec)
+    return out
+
+
+def apply_h(vec, xsigs, zsigs, coeffs, states):
+    """``Hv`` recomputing the source indices per call -- rqutils/sqd.py:590-605."""
+    xs = np.stack([get_xsource(xsigs[g], states) for g in range(xsigs.shape[0])])
+    return apply_h_x_cached(vec, xs, zsigs, coeffs, states)
+
+
+def vinit_from_min_diag(xsigs, zsigs, coeffs, states_u, n_unique):
+    """Default start vector of ``run_sqd`` -- rqutils/sqd.py:373-398.
+
+    One-hot at the argmin of the real group-0 diagonal over non-fill rows when
+    group 0 is the identity X signature; else one-hot at index 0.
+    """
+    size = states_u.shape[0]
+    dtype = np.complex128 if np.iscomplexobj(coeffs) else np.float64
+    v = np.zeros(size, dtype=dtype)
+    if np.all(xsigs[0] == 0):
+        diag = np.real(get_diagonal(zsigs[0], coeffs[0], states_u))
+        diag = np.where(states_u[:, 0] == 255, np.max(diag), diag)
+        v[int(np.argmin(diag))] = 1
+    else:
+        v[0] = 1
+    return v
+
+
+def hproj_coo(xsigs, zsigs, coeffs, states):
+    """Projected matrix as COO triplets, intended semantics of sqd.py:251-296.
+
+    The reference packs the states without the pad bit while using a padded
+    Hamiltonian (sqd.py:277 vs :280); here both use the pad bit, so the result
+    is the projector-sandwiched matrix ``P H P``.
+    """
+    rows, cols, vals = [], [], []
+    for g in range(xsigs.shape[0]):
+        xs = get_xsource(xsigs[g], states)
+        d = get_diagonal(zsigs[g], coeffs[g], states)
+        ok = xs >= 0
+        rows.append(np.nonzero(ok)[0])
+        cols.append(xs[ok])
+        vals.append(d[ok])
+    return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+
+
+def packed_to_keys(states_p, num_qubits):
+    """Integer key ``sum_q s_q 2^q`` of each padded packed row (pad bit must be 0).
+
+    Used by tests to relate the reference layout to the uint64-key layout.
+    """
+    bits = np.unpackbits(states_p, axis=1)[:, 1 : 1 + num_qubits].astype(np.uint64)
+    weights = np.uint64(1) << np.arange(num_qubits - 1, -1, -1, dtype=np.uint64)
+    return (bits * weights[None, :]).sum(axis=1, dtype=np.uint64)

@@ -1,0 +1,54 @@
+// Internal launch helpers shared by the matvec, vector and LOBPCG sources.
+// Every helper that takes `skip` turns into a no-op when *skip != 0 at kernel
+// start: the LOBPCG enqueues several iterations ahead and stops work on the
+// device once its convergence flag is set.
+#pragma once
+
+#include "common.hpp"
+
+namespace sqdk {
+
+int check_space_public(const sqdk_space* sp);
+
+// Matvec with an optional device skip flag: when *skip != 0 at kernel start
+// the launch does nothing (used for iterations after convergence inside a
+// captured chunk).
+int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
+                const void* v, void* out, const sqdk_tune* tune, hipStream_t s,
+                const int32_t* skip = nullptr);
+
+constexpr int kMaxPairs = 16;
+constexpr int kDotBlocks = 512;   // partial sums per pair (fixed: deterministic)
+
+struct PairList {
+  const void* a[kMaxPairs];
+  const void* b[kMaxPairs];
+};
+
+// out[k] (complex double) = sum_i conj(a_k[i]) b_k[i]; partials needs
+// kDotBlocks * npairs * 2 doubles.
+int dots_impl(int64_t n, int precision, int npairs, const PairList& pl, double* out,
+              double* partials, hipStream_t s, const int32_t* skip = nullptr);
+
+// out = sum_k S[cidx[k]] * in_k (complex coefficients read from device slots).
+int lincomb_impl(int64_t n, int precision, void* out, int k, const void* const* in,
+                 const int* cidx, const double* slots, hipStream_t s,
+                 const int32_t* skip = nullptr);
+
+// Gram and projected-operator sums of the fused LOBPCG over
+// V = (X, R, P), AV = (AX, AR, AP): 13 complex sums written to out.
+int gram_impl(int64_t n, int precision, const void* const* v, const void* const* av,
+              double* out, double* partials, hipStream_t s, const int32_t* skip = nullptr);
+
+// Fused LOBPCG update; coefficients at slots[c_base ...]:
+//   X' = c0 X + c1 R + c2 P, AX' = c0 AX + c1 AR + c2 AP,
+//   P' = d1 R + d2 P, AP' = d1 AR + d2 AP, R' = AX' - theta X'.
+int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, const void* AR,
+                      void* P, void* AP, const double* slots, int c_base, hipStream_t s,
+                      const int32_t* skip = nullptr);
+
+int convert_impl(int64_t n, int from_p, const void* in, int to_p, void* out, hipStream_t s);
+
+inline size_t elem_bytes(int precision) { return precision == SQDK_C128 ? 16 : 8; }
+
+}  // namespace sqdk

@@ -1,0 +1,762 @@
+// Single-vector LOBPCG on the device (rqutils/ground_locg.py:128-471).
+//
+// Every vector stays in HBM and every scalar (inner products, Rayleigh-Ritz
+// matrices, eigenpairs, update coefficients, the convergence flag) stays in a
+// small device slot area; the host only enqueues kernels and polls the flag
+// every check_every iterations. Two modes:
+//
+//  * REFERENCE follows ground_locg.py step by step: body_iter0 (:264-272),
+//    body_iter1 with the closed-form 2x2 (:274-286), then the loop body
+//    (:288-341) with _project_out (:380-421), compute_sas with three matvecs
+//    (:235-248), the Kopp closed-form 3x3 (:438-471) and a fourth matvec for
+//    the new residual; convergence |r| < tol * 10 * N * (|Ax| - theta).
+//  * FUSED keeps X, AX, R, AR, P, AP and needs one matvec (A R) per
+//    iteration: one pass forms the 3x3 Gram and projected matrices of
+//    (X, R, P) in FP64, one device thread solves the generalized 3x3
+//    Rayleigh-Ritz problem (Cholesky + complex Jacobi), and one pass updates
+//    all vectors. A X and A P are recomputed every refresh_every iterations
+//    and once more to verify the residual at convergence.
+#include "internal.hpp"
+
+#include <cmath>
+#include <vector>
+
+namespace sqdk {
+
+namespace {
+
+// ---- slot layout (complex double units) -----------------------------------
+enum Slot : int {
+  kOne = 0,
+  kZero = 1,
+  kD = 2,        // dot results D0.. (16 pairs)
+  kGram = 20,    // 13 fused Gram sums
+  kC = 40,       // lincomb coefficients C0..C7
+  kUpd = 50,     // fused update coefficients (6)
+  kTheta = 60,   // .x theta
+  kKappa = 61,   // kappa[0..2]
+  kInfo = 64,    // .x |r|, .y |Ax|
+  kInfo2 = 65,   // .x reltol / tol*|theta|, .y last criterion value
+  kNumSlots = 80
+};
+enum Flag : int { kConv = 0, kNiter = 1, kPValid = 2, kStatus = 3, kNumFlags = 16 };
+
+constexpr size_t kSlotBytes = kNumSlots * sizeof(double2);
+constexpr size_t kFlagBytes = kNumFlags * sizeof(int32_t);
+constexpr size_t kPartialBytes = sizeof(double) * 2 * kMaxPairs * kDotBlocks;
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// ---- small closed forms, as ground_locg.py computes them -------------------
+
+// Lowest eigenpair of a 2x2 Hermitian matrix (ground_locg.py:424-435).
+__device__ void eig2_reference(const double2 m[2][2], double& lam, double2 vec[2]) {
+  const double d0 = m[0][0].x, d1 = m[1][1].x;
+  const double det = d0 * d1 - cabs2(m[1][0]);
+  const double tr = d0 + d1;
+  lam = (tr - sqrt(tr * tr - 4.0 * det)) * 0.5;
+  const double2 num = make_double2(d1 - lam + m[1][0].x, -m[1][0].y);
+  const double2 den = make_double2(d0 - lam + m[1][0].x, m[1][0].y);
+  const double dd = cabs2(den);
+  const double2 e0 = make_double2((num.x * den.x + num.y * den.y) / dd,
+                                  (num.y * den.x - num.x * den.y) / dd);
+  const double nrm = sqrt(e0.x * e0.x + e0.y * e0.y + 1.0);
+  vec[0] = cscale(e0, 1.0 / nrm);
+  vec[1] = make_double2(-1.0 / nrm, 0.0);
+}
+
+// Lowest eigenpair of a 3x3 Hermitian matrix by Cardano's method
+// (J. Kopp, Int. J. Mod. Phys. C 19, 523 (2008); ground_locg.py:438-471):
+// characteristic coefficients, the trigonometric root, and the eigenvector as
+// the conjugated cross product of columns 1 and 2 of (A - lambda I).
+__device__ void eig3_reference(const double2 m[3][3], double& lam, double2 vec[3]) {
+  const double d0 = m[0][0].x, d1 = m[1][1].x, d2 = m[2][2].x;
+  const double o10 = cabs2(m[1][0]), o20 = cabs2(m[2][0]), o21 = cabs2(m[2][1]);
+  const double c2 = -(d0 + d1 + d2);
+  const double c1 = (d0 * d2 + d1 * d0 + d2 * d1) - (o10 + o20 + o21);
+  double c0 = d0 * o21 + d1 * o20 + d2 * o10;
+  c0 -= d0 * d1 * d2;
+  c0 -= 2.0 * cmul(cmul(m[0][2], m[1][0]), m[2][1]).x;
+  const double p = c2 * c2 - 3.0 * c1;
+  const double q = -13.5 * c0 - c2 * c2 * c2 + 4.5 * c2 * c1;
+  const double phi =
+      atan2(sqrt(27.0 * (0.25 * c1 * c1 * (p - c1) + c0 * (q + 6.75 * c0))), q) / 3.0;
+  const double cphi = cos(phi), sphi = sin(phi);
+  const double s3 = sqrt(3.0);
+  const double xmin = fmin(fmin(2.0 * cphi, -cphi - s3 * sphi), -cphi + s3 * sphi);
+  lam = sqrt(p) / 3.0 * xmin - c2 / 3.0;
+  double2 v0[3] = {m[0][1], m[1][1], m[2][1]};
+  double2 v1[3] = {m[0][2], m[1][2], m[2][2]};
+  v0[1].x -= lam;
+  v1[2].x -= lam;
+  double2 e[3] = {csub(cmul(v0[1], v1[2]), cmul(v0[2], v1[1])),
+                  csub(cmul(v0[2], v1[0]), cmul(v0[0], v1[2])),
+                  csub(cmul(v0[0], v1[1]), cmul(v0[1], v1[0]))};
+  const double nrm = sqrt(cabs2(e[0]) + cabs2(e[1]) + cabs2(e[2]));
+  for (int i = 0; i < 3; ++i) vec[i] = cscale(cconj(e[i]), 1.0 / nrm);
+}
+
+// ---- robust small eigensolver for the fused Rayleigh-Ritz ------------------
+
+// Cyclic complex Jacobi on an n x n (n <= 3) Hermitian matrix: each rotation
+// first removes the phase of a_pq, then applies the real Jacobi rotation.
+// Returns the lowest eigenvalue and its unit eigenvector.
+__device__ void jacobi_lowest(int n, double2 a[3][3], double& lam, double2 vec[3]) {
+  double2 v[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) v[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
+  for (int sweep = 0; sweep < 30; ++sweep) {
+    double off = 0.0, diag = 0.0;
+    for (int i = 0; i < n; ++i) {
+      diag += a[i][i].x * a[i][i].x;
+      for (int j = i + 1; j < n; ++j) off += cabs2(a[i][j]);
+    }
+    if (off <= 1e-32 * diag || off == 0.0) break;
+    for (int p = 0; p < n; ++p) {
+      for (int q = p + 1; q < n; ++q) {
+        const double b = sqrt(cabs2(a[p][q]));
+        if (b == 0.0) continue;
+        // phase: D = diag(.., e^{-i phi} at q) with a_pq = b e^{i phi}
+        const double2 ph = make_double2(a[p][q].x / b, -a[p][q].y / b);  // e^{-i phi}
+        for (int k = 0; k < n; ++k) {
+          a[k][q] = cmul(a[k][q], ph);          // column q times e^{-i phi}
+          v[k][q] = cmul(v[k][q], ph);
+        }
+        for (int k = 0; k < n; ++k) a[q][k] = cmul(a[q][k], cconj(ph));  // row q times e^{i phi}
+        // now a_pq = b is real; real rotation zeroing it
+        const double app = a[p][p].x, aqq = a[q][q].x;
+        const double tau = (aqq - app) / (2.0 * b);
+        const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+        const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+        for (int k = 0; k < n; ++k) {
+          const double2 akp = a[k][p], akq = a[k][q];
+          a[k][p] = csub(cscale(akp, c), cscale(akq, s));
+          a[k][q] = cadd(cscale(akp, s), cscale(akq, c));
+          const double2 vkp = v[k][p], vkq = v[k][q];
+          v[k][p] = csub(cscale(vkp, c), cscale(vkq, s));
+          v[k][q] = cadd(cscale(vkp, s), cscale(vkq, c));
+        }
+        for (int k = 0; k < n; ++k) {
+          const double2 apk = a[p][k], aqk = a[q][k];
+          a[p][k] = csub(cscale(apk, c), cscale(aqk, s));
+          a[q][k] = cadd(cscale(apk, s), cscale(aqk, c));
+        }
+        a[p][q] = make_double2(0.0, 0.0);
+        a[q][p] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+  int imin = 0;
+  for (int i = 1; i < n; ++i)
+    if (a[i][i].x < a[imin][imin].x) imin = i;
+  lam = a[imin][imin].x;
+  for (int i = 0; i < n; ++i) vec[i] = v[i][imin];
+  for (int i = n; i < 3; ++i) vec[i] = make_double2(0.0, 0.0);
+}
+
+// Lower Cholesky factor of an n x n Hermitian matrix; false if a pivot^2 is
+// not above `floor` (the matrix has unit diagonal, so pivots are sines).
+__device__ bool cholesky(int n, const double2 g[3][3], double2 l[3][3], double floor) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) l[i][j] = make_double2(0.0, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double s = g[j][j].x;
+    for (int k = 0; k < j; ++k) s -= cabs2(l[j][k]);
+    if (!(s > floor)) return false;
+    const double ljj = sqrt(s);
+    l[j][j] = make_double2(ljj, 0.0);
+    for (int i = j + 1; i < n; ++i) {
+      double2 t = g[i][j];
+      for (int k = 0; k < j; ++k) t = csub(t, cmul(l[i][k], cconj(l[j][k])));
+      l[i][j] = cscale(t, 1.0 / ljj);
+    }
+  }
+  return true;
+}
+
+struct ScalarParams {
+  double tol;
+  double nlen;     // vector length in the reference criterion
+  int criterion;
+  int precision;
+};
+
+enum Step : int {
+  kStepInitNorm = 0,   // C0 = 1 / sqrt(D0)
+  kStepRho,            // theta = Re D0, C0 = -theta
+  kStepInvNormGuard,   // C0 = 1 / (|D0| == 0 ? 1 : |D0|)
+  kStepEig2,           // 2x2 Rayleigh-Ritz from D0..D2
+  kStepInvNorm2,       // C0 = 1/sqrt(D0), C1 = 1/sqrt(D1)
+  kStepNegTheta,       // C0 = -theta
+  kStepProjNeg,        // C0 = -D0, C1 = -D1
+  kStepThresh,         // C0 = sqrt(D0) >= 0.99 ? 1 : 0
+  kStepEig3,           // 3x3 Rayleigh-Ritz from D0..D5; C0 = k1, C1 = k2
+  kStepTU,             // from D0 = |s|^2: C0 = k0/ns, C1 = -ns, C2 = k0, C3 = 1
+  kStepConv,           // D0 = |Ax|^2, D1 = |r|^2: reference criterion
+  kStepFusedRR,        // fused: criterion + generalized 3x3 Rayleigh-Ritz
+  kStepRefreshRho,     // fused refresh: theta = Re D1 / D0, C0 = -theta
+  kStepVerify,         // D0 = |Ax|^2, D1 = |r|^2 after refresh: info only
+};
+
+__device__ bool criterion_met(const ScalarParams& p, double rnorm, double axnorm, double theta,
+                              double2* S) {
+  bool conv;
+  double thr;
+  if (p.criterion == SQDK_CRIT_REFERENCE) {
+    thr = (axnorm - theta) * p.nlen * 10.0 * p.tol;
+    conv = rnorm < thr;
+  } else {
+    thr = p.tol * fabs(theta);
+    conv = rnorm <= thr;
+  }
+  S[kInfo] = make_double2(rnorm, axnorm);
+  S[kInfo2].x = thr;
+  return conv;
+}
+
+__global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __restrict__ F,
+                              ScalarParams prm, int use_skip) {
+  if (threadIdx.x != 0) return;
+  if (use_skip && F[kConv] != 0) return;
+  double2* D = S + kD;
+  double2* C = S + kC;
+  switch (step) {
+    case kStepInitNorm:
+      C[0] = make_double2(1.0 / sqrt(D[0].x), 0.0);
+      break;
+    case kStepRho:
+      S[kTheta] = make_double2(D[0].x, 0.0);
+      C[0] = make_double2(-D[0].x, 0.0);
+      break;
+    case kStepInvNormGuard: {
+      const double nr = sqrt(D[0].x);
+      C[0] = make_double2(1.0 / (nr == 0.0 ? 1.0 : nr), 0.0);
+      break;
+    }
+    case kStepEig2: {
+      // compute_sas(x, p): D0 = (p, Ax), D1 = (x, Ax), D2 = (p, Ap)
+      double2 m[2][2];
+      m[1][0] = D[0];
+      m[0][1] = cconj(D[0]);
+      m[0][0] = D[1];
+      m[1][1] = D[2];
+      double lam;
+      double2 k[2];
+      eig2_reference(m, lam, k);
+      S[kTheta] = make_double2(lam, 0.0);
+      S[kKappa] = k[0];
+      S[kKappa + 1] = k[1];
+      // t = p k0 - x k1 ; u = x k0 + p k1
+      C[0] = k[0];
+      C[1] = make_double2(-k[1].x, -k[1].y);
+      C[2] = k[0];
+      C[3] = k[1];
+      break;
+    }
+    case kStepInvNorm2:
+      C[0] = make_double2(1.0 / sqrt(D[0].x), 0.0);
+      C[1] = make_double2(1.0 / sqrt(D[1].x), 0.0);
+      break;
+    case kStepNegTheta:
+      C[0] = make_double2(-S[kTheta].x, 0.0);
+      break;
+    case kStepProjNeg:
+      C[0] = make_double2(-D[0].x, -D[0].y);
+      C[1] = make_double2(-D[1].x, -D[1].y);
+      break;
+    case kStepThresh:
+      C[0] = make_double2(sqrt(D[0].x) >= 0.99 ? 1.0 : 0.0, 0.0);
+      break;
+    case kStepEig3: {
+      // compute_sas(x, y, p): D0 = (y,Ax) D1 = (p,Ax) D2 = (p,Ay) D3 = (x,Ax) D4 = (y,Ay) D5 = (p,Ap)
+      double2 m[3][3];
+      m[1][0] = D[0];
+      m[2][0] = D[1];
+      m[2][1] = D[2];
+      m[0][1] = cconj(D[0]);
+      m[0][2] = cconj(D[1]);
+      m[1][2] = cconj(D[2]);
+      m[0][0] = D[3];
+      m[1][1] = D[4];
+      m[2][2] = D[5];
+      double lam;
+      double2 k[3];
+      eig3_reference(m, lam, k);
+      S[kTheta] = make_double2(lam, 0.0);
+      for (int i = 0; i < 3; ++i) S[kKappa + i] = k[i];
+      C[0] = k[1];
+      C[1] = k[2];
+      break;
+    }
+    case kStepTU: {
+      const double ns = sqrt(D[0].x);
+      const double2 k0 = S[kKappa];
+      C[0] = cscale(k0, 1.0 / ns);
+      C[1] = make_double2(-ns, 0.0);
+      C[2] = k0;
+      C[3] = make_double2(1.0, 0.0);
+      break;
+    }
+    case kStepConv: {
+      const double axn = sqrt(D[0].x), rn = sqrt(D[1].x);
+      const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S);
+      F[kNiter] += 1;
+      if (conv) F[kConv] = 1;
+      break;
+    }
+    case kStepFusedRR: {
+      const double2* g = S + kGram;
+      const double theta = S[kTheta].x;
+      const double rn = sqrt(fmax(g[1].x, 0.0));
+      const double axn = sqrt(fmax(g[12].x, 0.0));
+      double2* U = S + kUpd;
+      if (criterion_met(prm, rn, axn, theta, S)) {
+        F[kConv] = 1;
+        break;
+      }
+      // G_ij = (S_i, S_j), M_ij = (S_i, A S_j) over S = (X, R, P)
+      double2 G[3][3], M[3][3];
+      G[0][0] = g[0]; G[1][1] = g[1]; G[2][2] = g[2];
+      G[0][1] = g[3]; G[0][2] = g[4]; G[1][2] = g[5];
+      M[0][0] = g[6]; M[1][1] = g[7]; M[2][2] = g[8];
+      M[0][1] = g[9]; M[0][2] = g[10]; M[1][2] = g[11];
+      for (int i = 0; i < 3; ++i) {
+        G[i][i].y = 0.0;
+        M[i][i].y = 0.0;
+        for (int j = 0; j < i; ++j) {
+          G[i][j] = cconj(G[j][i]);
+          M[i][j] = cconj(M[j][i]);
+        }
+      }
+      int use = (F[kPValid] != 0 && G[2][2].x > 0.0) ? 3 : 2;
+      double dsc[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < use; ++i) dsc[i] = 1.0 / sqrt(G[i][i].x);
+      const double floor = prm.precision == SQDK_C128 ? 1e-14 : 1e-8;
+      double2 Gs[3][3], Ms[3][3], L[3][3];
+      bool ok = false;
+      while (use >= 2 && !ok) {
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            const bool in = i < use && j < use;
+            Gs[i][j] = in ? cscale(G[i][j], dsc[i] * dsc[j]) : make_double2(0.0, 0.0);
+            Ms[i][j] = in ? cscale(M[i][j], dsc[i] * dsc[j]) : make_double2(0.0, 0.0);
+          }
+        ok = cholesky(use, Gs, L, floor);
+        if (!ok) --use;
+      }
+      if (!ok) {
+        // R is numerically parallel to X: nothing left to improve.
+        F[kStatus] = 2;
+        F[kConv] = 1;
+        break;
+      }
+      // Linv = L^{-1} (lower triangular)
+      double2 Li[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Li[i][j] = make_double2(0.0, 0.0);
+      for (int j = 0; j < use; ++j) {
+        Li[j][j] = make_double2(1.0 / L[j][j].x, 0.0);
+        for (int i = j + 1; i < use; ++i) {
+          double2 t = make_double2(0.0, 0.0);
+          for (int k = j; k < i; ++k) t = cadd(t, cmul(L[i][k], Li[k][j]));
+          Li[i][j] = cscale(make_double2(-t.x, -t.y), 1.0 / L[i][i].x);
+        }
+      }
+      // Cm = Linv Ms Linv^H
+      double2 T[3][3], Cm[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double2 t = make_double2(0.0, 0.0);
+          for (int k = 0; k < use; ++k) t = cadd(t, cmul(Li[i][k], Ms[k][j]));
+          T[i][j] = t;
+        }
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          double2 t = make_double2(0.0, 0.0);
+          for (int k = 0; k < use; ++k) t = cadd(t, cmul(T[i][k], cconj(Li[j][k])));
+          Cm[i][j] = t;
+        }
+      for (int i = 0; i < use; ++i) {
+        Cm[i][i].y = 0.0;
+        for (int j = 0; j < i; ++j) {
+          const double2 h = cscale(cadd(Cm[i][j], cconj(Cm[j][i])), 0.5);
+          Cm[i][j] = h;
+          Cm[j][i] = cconj(h);
+        }
+      }
+      double lam;
+      double2 y[3];
+      jacobi_lowest(use, Cm, lam, y);
+      // c = Linv^H y, then undo the diagonal scaling
+      double2 c[3];
+      for (int i = 0; i < 3; ++i) {
+        double2 t = make_double2(0.0, 0.0);
+        for (int k = 0; k < use; ++k) t = cadd(t, cmul(cconj(Li[k][i]), y[k]));
+        c[i] = i < use ? cscale(t, dsc[i]) : make_double2(0.0, 0.0);
+      }
+      // fix the phase so the X coefficient is real and non-negative
+      const double a0 = sqrt(cabs2(c[0]));
+      if (a0 > 0.0) {
+        const double2 ph = make_double2(c[0].x / a0, -c[0].y / a0);
+        for (int i = 0; i < 3; ++i) c[i] = cmul(c[i], ph);
+      }
+      // P' = c1 R + c2 P, normalised with the Gram matrix
+      double pn2 = 0.0;
+      for (int i = 1; i < 3; ++i)
+        for (int j = 1; j < 3; ++j) pn2 += cmul(cconj(c[i]), cmul(G[i][j], c[j])).x;
+      const double pscale = pn2 > 0.0 ? 1.0 / sqrt(pn2) : 0.0;
+      U[0] = c[0];
+      U[1] = c[1];
+      U[2] = c[2];
+      U[3] = cscale(c[1], pscale);
+      U[4] = cscale(c[2], pscale);
+      U[5] = make_double2(lam, 0.0);
+      S[kTheta] = make_double2(lam, 0.0);
+      F[kPValid] = pscale > 0.0 ? 1 : 0;
+      F[kNiter] += 1;
+      break;
+    }
+    case kStepRefreshRho: {
+      const double th = D[1].x / D[0].x;
+      S[kTheta] = make_double2(th, 0.0);
+      C[0] = make_double2(-th, 0.0);
+      break;
+    }
+    case kStepVerify: {
+      const double axn = sqrt(D[0].x), rn = sqrt(D[1].x);
+      const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S);
+      F[kConv] = conv ? 1 : 0;
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+__global__ void init_slots_kernel(double2* S, int32_t* F) {
+  const int i = threadIdx.x;
+  if (i < kNumSlots) S[i] = make_double2(i == kOne ? 1.0 : 0.0, 0.0);
+  if (i < kNumFlags) F[i] = 0;
+}
+
+__global__ void small_eig_kernel(int dim, const double2* __restrict__ mat, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double lam;
+  if (dim == 2) {
+    double2 m[2][2] = {{mat[0], mat[1]}, {mat[2], mat[3]}};
+    double2 v[2];
+    eig2_reference(m, lam, v);
+    out[0] = lam;
+    for (int i = 0; i < 2; ++i) { out[1 + 2 * i] = v[i].x; out[2 + 2 * i] = v[i].y; }
+  } else {
+    double2 m[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) m[i][j] = mat[3 * i + j];
+    double2 v[3];
+    eig3_reference(m, lam, v);
+    out[0] = lam;
+    for (int i = 0; i < 3; ++i) { out[1 + 2 * i] = v[i].x; out[2 + 2 * i] = v[i].y; }
+  }
+}
+
+// ---- host driver -------------------------------------------------------------
+
+struct Solver {
+  const sqdk_op* op;
+  int64_t n;
+  int prec;
+  hipStream_t s;
+  double2* S;
+  int32_t* F;
+  double* partials;
+  char* vec[7];
+  ScalarParams prm;
+  int matvecs = 0;
+
+  int apply(const void* in, void* out, bool skip) {
+    ++matvecs;
+    if (op->fn) {
+      const int rc = op->fn(op->ctx, in, out, s);
+      if (rc != 0) return fail(SQDK_ECALLBACK, "matvec callback returned %d", rc);
+      return SQDK_OK;
+    }
+    return matvec_impl(op->ham, op->space, prec, in, out, op->tune, s, skip ? F + kConv : nullptr);
+  }
+  int dots(std::initializer_list<std::pair<const void*, const void*>> pairs, bool skip,
+           int slot = kD) {
+    PairList pl{};
+    int k = 0;
+    for (auto& p : pairs) {
+      pl.a[k] = p.first;
+      pl.b[k] = p.second;
+      ++k;
+    }
+    return dots_impl(n, prec, k, pl, reinterpret_cast<double*>(S + slot), partials, s,
+                     skip ? F + kConv : nullptr);
+  }
+  int lin(void* out, std::initializer_list<const void*> in, std::initializer_list<int> cidx,
+          bool skip) {
+    const void* ins[4];
+    int ci[4];
+    int k = 0;
+    for (auto p : in) ins[k++] = p;
+    k = 0;
+    for (auto c : cidx) ci[k++] = c;
+    return lincomb_impl(n, prec, out, k, ins, ci, reinterpret_cast<const double*>(S), s,
+                        skip ? F + kConv : nullptr);
+  }
+  int scalar(int step, bool skip) {
+    scalar_kernel<<<1, 64, 0, s>>>(step, S, F, prm, skip ? 1 : 0);
+    SQDK_LAUNCHED("scalar_kernel");
+    return SQDK_OK;
+  }
+  int read_flags(int32_t* host) {
+    SQDK_HIP(hipMemcpyAsync(host, F, kFlagBytes, hipMemcpyDeviceToHost, s));
+    SQDK_HIP(hipStreamSynchronize(s));
+    return SQDK_OK;
+  }
+  int read_slots(double2* host) {
+    SQDK_HIP(hipMemcpyAsync(host, S, kSlotBytes, hipMemcpyDeviceToHost, s));
+    SQDK_HIP(hipStreamSynchronize(s));
+    return SQDK_OK;
+  }
+};
+
+constexpr int C0 = kC, C1 = kC + 1, C2 = kC + 2, C3 = kC + 3;
+
+int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host) {
+  char* y = sv.vec[0];
+  char* r = sv.vec[1];
+  char* p = sv.vec[2];
+  char* ax = sv.vec[3];
+  char* ay = sv.vec[4];
+  char* ap = sv.vec[5];
+  // xinit /= |xinit|  (ground_locg.py:346)
+  SQDK_TRY(sv.dots({{x, x}}, false));
+  SQDK_TRY(sv.scalar(kStepInitNorm, false));
+  SQDK_TRY(sv.lin(x, {x}, {C0}, false));
+  // body_iter0 (:264-272): r = Ax - rho x
+  SQDK_TRY(sv.apply(x, ax, false));
+  SQDK_TRY(sv.dots({{x, ax}}, false));
+  SQDK_TRY(sv.scalar(kStepRho, false));
+  SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, false));
+  // body_iter1 (:274-286)
+  SQDK_TRY(sv.dots({{r, r}}, false));
+  SQDK_TRY(sv.scalar(kStepInvNormGuard, false));
+  SQDK_TRY(sv.lin(p, {r}, {C0}, false));
+  SQDK_TRY(sv.apply(x, ax, false));
+  SQDK_TRY(sv.apply(p, ap, false));
+  SQDK_TRY(sv.dots({{p, ax}, {x, ax}, {p, ap}}, false));
+  SQDK_TRY(sv.scalar(kStepEig2, false));
+  SQDK_TRY(sv.lin(y, {p, x}, {C0, C1}, false));   // t
+  SQDK_TRY(sv.lin(ay, {x, p}, {C2, C3}, false));  // u
+  SQDK_TRY(sv.dots({{ay, ay}, {y, y}}, false));
+  SQDK_TRY(sv.scalar(kStepInvNorm2, false));
+  SQDK_TRY(sv.lin(x, {ay}, {C0}, false));
+  SQDK_TRY(sv.lin(y, {y}, {C1}, false));
+  SQDK_TRY(sv.apply(x, ax, false));
+  SQDK_TRY(sv.scalar(kStepNegTheta, false));
+  SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, false));
+
+  int done = 0;
+  while (done < o.maxiter) {
+    const int chunk = std::min(o.check_every, o.maxiter - done);
+    for (int it = 0; it < chunk; ++it) {
+      // _project_out((x, y), r) (:380-421)
+      const void* src = r;
+      for (int rep = 0; rep < 2; ++rep) {
+        SQDK_TRY(sv.dots({{x, src}, {y, src}}, true));
+        SQDK_TRY(sv.scalar(kStepProjNeg, true));
+        SQDK_TRY(sv.lin(p, {src, x, y}, {kOne, C0, C1}, true));
+        SQDK_TRY(sv.dots({{p, p}}, true));
+        SQDK_TRY(sv.scalar(kStepInvNormGuard, true));
+        SQDK_TRY(sv.lin(p, {p}, {C0}, true));
+        src = p;
+      }
+      for (int rep = 0; rep < 2; ++rep) {
+        SQDK_TRY(sv.dots({{x, p}, {y, p}}, true));
+        SQDK_TRY(sv.scalar(kStepProjNeg, true));
+        SQDK_TRY(sv.lin(p, {p, x, y}, {kOne, C0, C1}, true));
+      }
+      SQDK_TRY(sv.dots({{p, p}}, true));
+      SQDK_TRY(sv.scalar(kStepThresh, true));
+      SQDK_TRY(sv.lin(p, {p}, {C0}, true));
+      // rayleigh_ritz(x, y, p) (:235-255)
+      SQDK_TRY(sv.apply(x, ax, true));
+      SQDK_TRY(sv.apply(y, ay, true));
+      SQDK_TRY(sv.apply(p, ap, true));
+      SQDK_TRY(sv.dots({{y, ax}, {p, ax}, {p, ay}, {x, ax}, {y, ay}, {p, ap}}, true));
+      SQDK_TRY(sv.scalar(kStepEig3, true));
+      // s = y k1 + p k2 -> ay ; t = s k0/ns - x ns -> ap ; u = x k0 + s -> ax
+      SQDK_TRY(sv.lin(ay, {y, p}, {C0, C1}, true));
+      SQDK_TRY(sv.dots({{ay, ay}}, true));
+      SQDK_TRY(sv.scalar(kStepTU, true));
+      SQDK_TRY(sv.lin(ap, {ay, x}, {C0, C1}, true));
+      SQDK_TRY(sv.lin(ax, {x, ay}, {C2, C3}, true));
+      SQDK_TRY(sv.dots({{ax, ax}, {ap, ap}}, true));
+      SQDK_TRY(sv.scalar(kStepInvNorm2, true));
+      SQDK_TRY(sv.lin(x, {ax}, {C0}, true));
+      SQDK_TRY(sv.lin(y, {ap}, {C1}, true));
+      // r' = A x' - theta x' and the convergence test (:306-334)
+      SQDK_TRY(sv.apply(x, ax, true));
+      SQDK_TRY(sv.scalar(kStepNegTheta, true));
+      SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, true));
+      SQDK_TRY(sv.dots({{ax, ax}, {r, r}}, true));
+      SQDK_TRY(sv.scalar(kStepConv, true));
+      if (sv.op->fn && it + 1 < chunk) break;  // callbacks cannot be skipped on device
+    }
+    SQDK_TRY(sv.read_flags(flags_host));
+    done = flags_host[kNiter];
+    if (flags_host[kConv]) break;
+  }
+  return SQDK_OK;
+}
+
+int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host) {
+  char* X = static_cast<char*>(x);
+  char* R = sv.vec[0];
+  char* P = sv.vec[1];
+  char* AX = sv.vec[2];
+  char* AR = sv.vec[3];
+  char* AP = sv.vec[4];
+  const size_t vb = (size_t)sv.n * elem_bytes(sv.prec);
+  SQDK_HIP(hipMemsetAsync(P, 0, vb, sv.s));
+  SQDK_HIP(hipMemsetAsync(AP, 0, vb, sv.s));
+  SQDK_TRY(sv.dots({{X, X}}, false));
+  SQDK_TRY(sv.scalar(kStepInitNorm, false));
+  SQDK_TRY(sv.lin(X, {X}, {C0}, false));
+  SQDK_TRY(sv.apply(X, AX, false));
+  SQDK_TRY(sv.dots({{X, AX}}, false));
+  SQDK_TRY(sv.scalar(kStepRho, false));
+  SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, false));
+
+  const void* V[3] = {X, R, P};
+  const void* AV[3] = {AX, AR, AP};
+  int done = 0;
+  int since_refresh = 0;
+  while (true) {
+    const int chunk = std::min(o.check_every, o.maxiter - done);
+    for (int it = 0; it < chunk; ++it) {
+      SQDK_TRY(sv.apply(R, AR, true));
+      SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, reinterpret_cast<double*>(sv.S + kGram),
+                         sv.partials, sv.s, sv.F + kConv));
+      SQDK_TRY(sv.scalar(kStepFusedRR, true));
+      SQDK_TRY(fused_update_impl(sv.n, sv.prec, X, AX, R, AR, P, AP,
+                                 reinterpret_cast<const double*>(sv.S), kUpd, sv.s,
+                                 sv.F + kConv));
+      if (o.refresh_every > 0 && ++since_refresh >= o.refresh_every) {
+        since_refresh = 0;
+        SQDK_TRY(sv.apply(X, AX, true));
+        SQDK_TRY(sv.apply(P, AP, true));
+        SQDK_TRY(sv.dots({{X, X}, {X, AX}}, true));
+        SQDK_TRY(sv.scalar(kStepRefreshRho, true));
+        SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, true));
+      }
+      if (sv.op->fn && it + 1 < chunk) break;
+    }
+    SQDK_TRY(sv.read_flags(flags_host));
+    done = flags_host[kNiter];
+    const bool conv = flags_host[kConv] != 0;
+    if (conv || done >= o.maxiter) {
+      // Verify with an explicit product: A X, theta = <X, AX> / <X, X>, r = AX - theta X.
+      SQDK_TRY(sv.apply(X, AX, false));
+      SQDK_TRY(sv.dots({{X, X}, {X, AX}}, false));
+      SQDK_TRY(sv.scalar(kStepRefreshRho, false));
+      SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, false));
+      SQDK_TRY(sv.dots({{AX, AX}, {R, R}}, false));
+      SQDK_TRY(sv.scalar(kStepVerify, false));
+      SQDK_TRY(sv.read_flags(flags_host));
+      if (flags_host[kConv] || done >= o.maxiter || flags_host[kStatus] == 2) break;
+      // Not converged after all: A P is refreshed too and the loop goes on.
+      SQDK_TRY(sv.apply(P, AP, false));
+      since_refresh = 0;
+    }
+  }
+  return SQDK_OK;
+}
+
+}  // namespace
+
+}  // namespace sqdk
+
+using namespace sqdk;
+
+extern "C" {
+
+int sqdk_locg_workspace(int64_t n, const sqdk_locg_opts* opts, size_t* bytes) {
+  SQDK_REQUIRE(opts && bytes, "NULL pointer argument");
+  SQDK_REQUIRE(n >= 1, "empty problem");
+  SQDK_REQUIRE(opts->precision == SQDK_C128 || opts->precision == SQDK_C64,
+               "unknown precision %d", opts->precision);
+  const size_t vb = align256((size_t)n * elem_bytes(opts->precision));
+  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) + 7 * vb;
+  return SQDK_OK;
+}
+
+int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts* opts,
+                    void* work, size_t work_bytes, sqdk_locg_result* result, void* stream) {
+  SQDK_REQUIRE(op && x && opts && work && result, "NULL pointer argument");
+  SQDK_REQUIRE(op->fn || (op->ham && op->space), "operator has neither a callback nor ham+space");
+  SQDK_REQUIRE(opts->mode == SQDK_LOCG_REFERENCE || opts->mode == SQDK_LOCG_FUSED,
+               "unknown mode %d", opts->mode);
+  SQDK_REQUIRE(opts->maxiter >= 0, "maxiter %d < 0", opts->maxiter);
+  SQDK_REQUIRE(opts->criterion == SQDK_CRIT_REFERENCE || opts->criterion == SQDK_CRIT_RELATIVE,
+               "unknown criterion %d", opts->criterion);
+  if (!op->fn) {
+    SQDK_REQUIRE(op->space->n == n, "operator size %lld != vector size %lld",
+                 (long long)op->space->n, (long long)n);
+  }
+  size_t need = 0;
+  SQDK_TRY(sqdk_locg_workspace(n, opts, &need));
+  SQDK_REQUIRE(work_bytes >= need, "workspace of %zu bytes, %zu needed", work_bytes, need);
+  sqdk_locg_opts o = *opts;
+  if (o.check_every < 1) o.check_every = 1;
+  if (op->fn) o.check_every = 1;
+
+  Solver sv{};
+  sv.op = op;
+  sv.n = n;
+  sv.prec = o.precision;
+  sv.s = as_stream(stream);
+  char* w = static_cast<char*>(work);
+  sv.S = reinterpret_cast<double2*>(w);
+  w += align256(kSlotBytes);
+  sv.F = reinterpret_cast<int32_t*>(w);
+  w += align256(kFlagBytes);
+  sv.partials = reinterpret_cast<double*>(w);
+  w += align256(kPartialBytes);
+  const size_t vb = align256((size_t)n * elem_bytes(o.precision));
+  for (int i = 0; i < 7; ++i) sv.vec[i] = w + i * vb;
+  sv.prm.tol = o.tol;
+  sv.prm.nlen = (double)(o.n_for_tol > 0 ? o.n_for_tol : n);
+  sv.prm.criterion = o.criterion;
+  sv.prm.precision = o.precision;
+
+  init_slots_kernel<<<1, 128, 0, sv.s>>>(sv.S, sv.F);
+  SQDK_LAUNCHED("init_slots_kernel");
+  int32_t flags[kNumFlags] = {0};
+  int rc = o.mode == SQDK_LOCG_REFERENCE ? run_reference(sv, x, o, flags)
+                                         : run_fused(sv, x, o, flags);
+  if (rc != SQDK_OK) return rc;
+  double2 slots[kNumSlots];
+  SQDK_TRY(sv.read_slots(slots));
+  SQDK_TRY(sv.read_flags(flags));
+  result->eigval = slots[kTheta].x;
+  result->resid_norm = slots[kInfo].x;
+  result->ax_norm = slots[kInfo].y;
+  result->niter = flags[kNiter];
+  result->converged = flags[kConv];
+  result->matvecs = sv.matvecs;
+  result->status = flags[kStatus];
+  return SQDK_OK;
+}
+
+int sqdk_small_eigenpair(int32_t dim, const double* mat, double* out, void* stream) {
+  SQDK_REQUIRE(mat && out, "NULL pointer argument");
+  SQDK_REQUIRE(dim == 2 || dim == 3, "dim %d not 2 or 3", dim);
+  small_eig_kernel<<<1, 64, 0, as_stream(stream)>>>(dim, reinterpret_cast<const double2*>(mat), out);
+  SQDK_LAUNCHED("small_eig_kernel");
+  return SQDK_OK;
+}
+
+}  // extern "C"

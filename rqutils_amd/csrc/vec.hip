@@ -1,0 +1,281 @@
+// Vector kernels of the LOBPCG: multi-pair inner products with FP64
+// accumulation (the jnp.sum(conj(a) * b) reductions of
+// rqutils/ground_locg.py:214-421), the fused Gram sums, linear combinations
+// with device-resident coefficients, the fused subspace update and
+// precision conversion. All reductions use a fixed grid and a fixed
+// combination order, so results are bitwise reproducible run to run.
+#include "internal.hpp"
+
+namespace sqdk {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+dots_partial_kernel(int64_t n, int npairs, PairList pl, double* __restrict__ partials,
+                    const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  __shared__ double red[2 * kMaxPairs * (kBlock / kWave)];
+  double acc[2 * kMaxPairs];
+#pragma unroll
+  for (int k = 0; k < 2 * kMaxPairs; ++k) acc[k] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+#pragma unroll
+    for (int k = 0; k < kMaxPairs; ++k) {
+      if (k < npairs) {
+        const double2 x = Vec<VT>::load(reinterpret_cast<const VT*>(pl.a[k]), i);
+        const double2 y = Vec<VT>::load(reinterpret_cast<const VT*>(pl.b[k]), i);
+        const double2 p = cmulc(x, y);
+        acc[2 * k] += p.x;
+        acc[2 * k + 1] += p.y;
+      }
+    }
+  }
+  block_sum<2 * kMaxPairs>(acc, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 2 * npairs; ++k) partials[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
+  }
+}
+
+// One block per output value: sum of the per-block partials in a fixed order.
+__global__ void __launch_bounds__(kBlock)
+dots_final_kernel(int nblocks, const double* __restrict__ partials, double* __restrict__ out,
+                  const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  __shared__ double red[kBlock / kWave];
+  double v[1] = {0.0};
+  for (int i = threadIdx.x; i < nblocks; i += kBlock) v[0] += partials[(int64_t)blockIdx.x * nblocks + i];
+  block_sum<1>(v, red);
+  if (threadIdx.x == 0) out[blockIdx.x] = v[0];
+}
+
+constexpr int kGramSums = 13;
+
+// Sums over V = (X, R, P), AV = (AX, AR, AP):
+//  0 XX 1 RR 2 PP 3 XR 4 XP 5 RP          (Gram, conj(first) * second)
+//  6 X.AX 7 R.AR 8 P.AP 9 X.AR 10 X.AP 11 R.AP   (projected operator)
+// 12 AX.AX
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+gram_partial_kernel(int64_t n, const VT* __restrict__ X, const VT* __restrict__ R,
+                    const VT* __restrict__ P, const VT* __restrict__ AX,
+                    const VT* __restrict__ AR, const VT* __restrict__ AP,
+                    double* __restrict__ partials, const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  __shared__ double red[2 * kGramSums * (kBlock / kWave)];
+  double acc[2 * kGramSums];
+#pragma unroll
+  for (int k = 0; k < 2 * kGramSums; ++k) acc[k] = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double2 x = Vec<VT>::load(X, i), r = Vec<VT>::load(R, i), p = Vec<VT>::load(P, i);
+    const double2 ax = Vec<VT>::load(AX, i), ar = Vec<VT>::load(AR, i), ap = Vec<VT>::load(AP, i);
+    const double2 prods[kGramSums] = {cmulc(x, x),  cmulc(r, r),  cmulc(p, p),   cmulc(x, r),  cmulc(x, p),
+                                      cmulc(r, p),  cmulc(x, ax), cmulc(r, ar),  cmulc(p, ap), cmulc(x, ar),
+                                      cmulc(x, ap), cmulc(r, ap), cmulc(ax, ax)};
+#pragma unroll
+    for (int k = 0; k < kGramSums; ++k) {
+      acc[2 * k] += prods[k].x;
+      acc[2 * k + 1] += prods[k].y;
+    }
+  }
+  block_sum<2 * kGramSums>(acc, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 2 * kGramSums; ++k) partials[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
+  }
+}
+
+struct LinIn {
+  const void* in[4];
+  int cidx[4];
+};
+
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+lincomb_kernel(int64_t n, int k, LinIn li, const double2* __restrict__ slots, VT* __restrict__ out,
+               const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  double2 c[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) c[j] = j < k ? slots[li.cidx[j]] : make_double2(0.0, 0.0);
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (j < k) acc = cadd(acc, cmul(c[j], Vec<VT>::load(reinterpret_cast<const VT*>(li.in[j]), i)));
+    }
+    Vec<VT>::store(out, i, acc);
+  }
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* __restrict__ R,
+                    const VT* __restrict__ AR, VT* __restrict__ P, VT* __restrict__ AP,
+                    const double2* __restrict__ c, const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  // c[0..2] = c0, c1, c2; c[3..4] = d1, d2; c[5].x = theta
+  const double2 c0 = c[0], c1 = c[1], c2 = c[2], d1 = c[3], d2 = c[4];
+  const double theta = c[5].x;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double2 x = Vec<VT>::load(X, i), r = Vec<VT>::load(R, i), p = Vec<VT>::load(P, i);
+    const double2 ax = Vec<VT>::load(AX, i), ar = Vec<VT>::load(AR, i), ap = Vec<VT>::load(AP, i);
+    const double2 xn = cadd(cadd(cmul(c0, x), cmul(c1, r)), cmul(c2, p));
+    const double2 axn = cadd(cadd(cmul(c0, ax), cmul(c1, ar)), cmul(c2, ap));
+    const double2 pn = cadd(cmul(d1, r), cmul(d2, p));
+    const double2 apn = cadd(cmul(d1, ar), cmul(d2, ap));
+    const double2 rn = csub(axn, cscale(xn, theta));
+    Vec<VT>::store(X, i, xn);
+    Vec<VT>::store(AX, i, axn);
+    Vec<VT>::store(P, i, pn);
+    Vec<VT>::store(AP, i, apn);
+    Vec<VT>::store(R, i, rn);
+  }
+}
+
+template <typename VI, typename VO>
+__global__ void __launch_bounds__(kBlock)
+convert_kernel(int64_t n, const VI* __restrict__ in, VO* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    Vec<VO>::store(out, i, Vec<VI>::load(in, i));
+}
+
+inline unsigned stream_grid(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock), 8192));
+}
+
+inline unsigned dot_grid(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock), kDotBlocks));
+}
+
+}  // namespace
+
+int dots_impl(int64_t n, int precision, int npairs, const PairList& pl, double* out,
+              double* partials, hipStream_t s, const int32_t* skip) {
+  SQDK_REQUIRE(npairs >= 1 && npairs <= kMaxPairs, "npairs %d out of range", npairs);
+  SQDK_REQUIRE(n >= 1, "empty vectors");
+  const unsigned g = dot_grid(n);
+  if (precision == SQDK_C128)
+    dots_partial_kernel<double2><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip);
+  else if (precision == SQDK_C64)
+    dots_partial_kernel<float2><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip);
+  else
+    return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  SQDK_LAUNCHED("dots_partial_kernel");
+  dots_final_kernel<<<2 * npairs, kBlock, 0, s>>>((int)g, partials, out, skip);
+  SQDK_LAUNCHED("dots_final_kernel");
+  return SQDK_OK;
+}
+
+int gram_impl(int64_t n, int precision, const void* const* v, const void* const* av,
+              double* out, double* partials, hipStream_t s, const int32_t* skip) {
+  const unsigned g = dot_grid(n);
+  if (precision == SQDK_C128)
+    gram_partial_kernel<double2><<<g, kBlock, 0, s>>>(
+        n, (const double2*)v[0], (const double2*)v[1], (const double2*)v[2],
+        (const double2*)av[0], (const double2*)av[1], (const double2*)av[2], partials, skip);
+  else
+    gram_partial_kernel<float2><<<g, kBlock, 0, s>>>(
+        n, (const float2*)v[0], (const float2*)v[1], (const float2*)v[2],
+        (const float2*)av[0], (const float2*)av[1], (const float2*)av[2], partials, skip);
+  SQDK_LAUNCHED("gram_partial_kernel");
+  dots_final_kernel<<<2 * kGramSums, kBlock, 0, s>>>((int)g, partials, out, skip);
+  SQDK_LAUNCHED("dots_final_kernel");
+  return SQDK_OK;
+}
+
+int lincomb_impl(int64_t n, int precision, void* out, int k, const void* const* in,
+                 const int* cidx, const double* slots, hipStream_t s, const int32_t* skip) {
+  SQDK_REQUIRE(k >= 1 && k <= 4, "lincomb of %d inputs", k);
+  LinIn li{};
+  for (int j = 0; j < k; ++j) {
+    li.in[j] = in[j];
+    li.cidx[j] = cidx[j];
+  }
+  const unsigned g = stream_grid(n);
+  const double2* sl = reinterpret_cast<const double2*>(slots);
+  if (precision == SQDK_C128)
+    lincomb_kernel<double2><<<g, kBlock, 0, s>>>(n, k, li, sl, (double2*)out, skip);
+  else
+    lincomb_kernel<float2><<<g, kBlock, 0, s>>>(n, k, li, sl, (float2*)out, skip);
+  SQDK_LAUNCHED("lincomb_kernel");
+  return SQDK_OK;
+}
+
+int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, const void* AR,
+                      void* P, void* AP, const double* slots, int c_base, hipStream_t s,
+                      const int32_t* skip) {
+  const unsigned g = stream_grid(n);
+  const double2* c = reinterpret_cast<const double2*>(slots) + c_base;
+  if (precision == SQDK_C128)
+    fused_update_kernel<double2><<<g, kBlock, 0, s>>>(n, (double2*)X, (double2*)AX, (double2*)R,
+                                                      (const double2*)AR, (double2*)P,
+                                                      (double2*)AP, c, skip);
+  else
+    fused_update_kernel<float2><<<g, kBlock, 0, s>>>(n, (float2*)X, (float2*)AX, (float2*)R,
+                                                     (const float2*)AR, (float2*)P,
+                                                     (float2*)AP, c, skip);
+  SQDK_LAUNCHED("fused_update_kernel");
+  return SQDK_OK;
+}
+
+int convert_impl(int64_t n, int from_p, const void* in, int to_p, void* out, hipStream_t s) {
+  SQDK_REQUIRE(in && out, "NULL vector");
+  SQDK_REQUIRE(n >= 1, "empty vector");
+  const unsigned g = stream_grid(n);
+  if (from_p == SQDK_C128 && to_p == SQDK_C64)
+    convert_kernel<double2, float2><<<g, kBlock, 0, s>>>(n, (const double2*)in, (float2*)out);
+  else if (from_p == SQDK_C64 && to_p == SQDK_C128)
+    convert_kernel<float2, double2><<<g, kBlock, 0, s>>>(n, (const float2*)in, (double2*)out);
+  else if (from_p == SQDK_C128 && to_p == SQDK_C128)
+    convert_kernel<double2, double2><<<g, kBlock, 0, s>>>(n, (const double2*)in, (double2*)out);
+  else if (from_p == SQDK_C64 && to_p == SQDK_C64)
+    convert_kernel<float2, float2><<<g, kBlock, 0, s>>>(n, (const float2*)in, (float2*)out);
+  else
+    return fail(SQDK_EINVAL, "unknown precision pair %d -> %d", from_p, to_p);
+  SQDK_LAUNCHED("convert_kernel");
+  return SQDK_OK;
+}
+
+}  // namespace sqdk
+
+using namespace sqdk;
+
+extern "C" {
+
+int sqdk_dots_workspace(int64_t n, int32_t npairs, size_t* bytes) {
+  SQDK_REQUIRE(bytes, "NULL output");
+  SQDK_REQUIRE(npairs >= 1 && npairs <= kMaxPairs, "npairs %d out of range", npairs);
+  (void)n;
+  *bytes = sizeof(double) * 2 * (size_t)npairs * kDotBlocks;
+  return SQDK_OK;
+}
+
+int sqdk_dots(int64_t n, int32_t precision, int32_t npairs, const void* const* a,
+              const void* const* b, double* out, void* work, size_t work_bytes, void* stream) {
+  SQDK_REQUIRE(a && b && out && work, "NULL pointer argument");
+  SQDK_REQUIRE(npairs >= 1 && npairs <= kMaxPairs, "npairs %d out of range", npairs);
+  SQDK_REQUIRE(work_bytes >= sizeof(double) * 2 * (size_t)npairs * kDotBlocks,
+               "dots workspace of %zu bytes too small", work_bytes);
+  PairList pl{};
+  for (int k = 0; k < npairs; ++k) {
+    SQDK_REQUIRE(a[k] && b[k], "NULL vector in pair %d", k);
+    pl.a[k] = a[k];
+    pl.b[k] = b[k];
+  }
+  return dots_impl(n, precision, npairs, pl, out, reinterpret_cast<double*>(work),
+                   as_stream(stream));
+}
+
+int sqdk_convert(int64_t n, int32_t from_precision, const void* in, int32_t to_precision,
+                 void* out, void* stream) {
+  return convert_impl(n, from_precision, in, to_precision, out, as_stream(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,370 @@
+"""Device-resident subspace and Pauli-sum operator for the SQD matvec.
+
+``Subspace`` holds the sorted uint64 keys of the sampled bitstrings in HBM with
+the directories the bucket-join matvec needs: the bucket directory (first key
+of every aligned bucket ``key >> m``), the fine directory inside each bucket
+and the row chunks one workgroup processes. ``PauliOperator`` holds a Pauli
+sum grouped by X mask, the groups ordered by class (``x >> m``) so that one
+staged target bucket serves every group of a class. Both are plain torch
+tensors handed to the C-ABI library by pointer.
+
+These replace the reference's packed state array and cached source indices
+(rqutils/sqd.py:316-351): nothing of size J x N is ever stored.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .paulis.symplectic import PauliSumXZ, _bits_to_u64
+
+__all__ = ["Subspace", "PauliOperator", "states_to_keys", "keys_to_states",
+           "full_space_matmul", "DEFAULT_BUCKET_CAP", "DEFAULT_ROWS_PER_THREAD"]
+
+DEFAULT_BUCKET_CAP = 8192       # keys per bucket staged in LDS (32 KiB as uint32)
+DEFAULT_ROWS_PER_THREAD = 8     # rows per thread; a chunk is 256 x this
+MAX_DIR_BITS = 26               # at most 2^26 buckets
+
+
+def states_to_keys(states: np.ndarray) -> np.ndarray:
+    """uint64 key ``sum_q s_q 2^q`` of each binary row (column 0 = qubit n-1)."""
+    states = np.asarray(states)
+    if states.ndim != 2:
+        raise ValueError("states must be a 2-d binary array")
+    return _bits_to_u64(states.astype(np.uint8))
+
+
+def keys_to_states(keys: np.ndarray, num_qubits: int) -> np.ndarray:
+    """Binary rows (column 0 = qubit n-1) of uint64 keys."""
+    keys = np.asarray(keys, dtype=np.uint64)
+    shifts = np.arange(num_qubits - 1, -1, -1, dtype=np.uint64)
+    return ((keys[:, None] >> shifts[None, :]) & np.uint64(1)).astype(np.uint8)
+
+
+def _as_i64(t: torch.Tensor) -> torch.Tensor:
+    return t if t.dtype == torch.int64 else t.view(torch.int64)
+
+
+def _device(device) -> torch.device:
+    dev = torch.device(device if device is not None else "cuda")
+    if dev.type != "cuda":
+        raise ValueError(f"the SQD kernels run on a GPU device, not {dev}")
+    if dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    return dev
+
+
+class Subspace:
+    """Sorted, distinct uint64 keys in HBM plus the bucket-join directories.
+
+    ``keys``: numpy uint64 array or a torch int64/uint64 tensor, strictly
+    ascending, each below ``2**num_qubits``. ``bucket_bits`` defaults to the
+    largest m whose biggest bucket holds at most ``bucket_cap`` keys.
+    """
+
+    def __init__(self, keys, num_qubits: int, *, device=None, bucket_bits: int | None = None,
+                 bucket_cap: int = DEFAULT_BUCKET_CAP,
+                 rows_per_thread: int = DEFAULT_ROWS_PER_THREAD, fine_bits: int | None = None):
+        L = _lib.lib()
+        self.device = _device(device)
+        if not 1 <= num_qubits <= 64:
+            raise ValueError(f"num_qubits={num_qubits} outside 1..64")
+        self.num_qubits = int(num_qubits)
+        if isinstance(keys, torch.Tensor):
+            kt = _as_i64(keys.to(self.device)).contiguous()
+        else:
+            arr = np.ascontiguousarray(np.asarray(keys, dtype=np.uint64))
+            kt = torch.from_numpy(arr.view(np.int64)).to(self.device)
+        n = kt.numel()
+        if n < 1 or n >= 2 ** 31:
+            raise ValueError(f"subspace size {n} outside 1..2^31-1")
+        self.keys = kt
+        self.n = int(n)
+        stream = _lib.stream_ptr(self.device)
+        with torch.cuda.device(self.device):
+            self.pc_mask = torch.zeros(2, dtype=torch.int64, device=self.device)
+            err = torch.zeros(1, dtype=torch.int32, device=self.device)
+            _lib.check(L.sqdk_space_scan(kt.data_ptr(), n, self.num_qubits,
+                                         self.pc_mask.data_ptr(), err.data_ptr(), stream),
+                       "sqdk_space_scan")
+            if int(err.item()) != 0:
+                raise ValueError(f"keys must be strictly ascending and below 2^{num_qubits} "
+                                 f"({int(err.item())} violations)")
+            self.rows_per_thread = int(rows_per_thread)
+            self.chunk_rows = 256 * self.rows_per_thread
+            self._plan(bucket_bits, bucket_cap, fine_bits)
+
+    # -- planning --------------------------------------------------------------
+
+    def _dir(self, m: int) -> torch.Tensor:
+        L = _lib.lib()
+        nb = 1 << (self.num_qubits - m)
+        d = torch.empty(nb + 1, dtype=torch.int32, device=self.device)
+        _lib.check(L.sqdk_space_dir(self.keys.data_ptr(), self.n, self.num_qubits, m,
+                                    d.data_ptr(), _lib.stream_ptr(self.device)),
+                   "sqdk_space_dir")
+        return d
+
+    @staticmethod
+    def _max_bucket(d: torch.Tensor) -> int:
+        return int((d[1:] - d[:-1]).max().item())
+
+    def _plan(self, bucket_bits, cap, fine_bits):
+        L = _lib.lib()
+        nq = self.num_qubits
+        lo_m = max(0, nq - MAX_DIR_BITS)
+        hi_m = min(nq, 63)
+        if bucket_bits is None:
+            # Largest m with max bucket <= cap (the max bucket shrinks with m).
+            if self._max_bucket(self._dir(lo_m)) > cap:
+                raise ValueError(f"keys too concentrated: a bucket at the finest level 2^{lo_m} "
+                                 f"holds more than {cap} keys")
+            a, b = lo_m, hi_m  # invariant: max_bucket(a) <= cap
+            while a < b:
+                mid = (a + b + 1) // 2
+                if self._max_bucket(self._dir(mid)) <= cap:
+                    a = mid
+                else:
+                    b = mid - 1
+            bucket_bits = a
+        if not lo_m <= bucket_bits <= hi_m:
+            raise ValueError(f"bucket_bits={bucket_bits} outside {lo_m}..{hi_m}")
+        self.bucket_bits = int(bucket_bits)
+        self.bstart = self._dir(self.bucket_bits)
+        self.n_buckets = 1 << (nq - self.bucket_bits)
+        self.max_bucket = self._max_bucket(self.bstart)
+        if self.max_bucket >= 65536:
+            raise ValueError(f"largest bucket holds {self.max_bucket} keys (limit 65535)")
+        if fine_bits is None:
+            fine_bits = min(self.bucket_bits, 13, max(0, int(np.ceil(np.log2(self.max_bucket)))))
+        self.fine_bits = int(fine_bits)
+        self.fdir = torch.empty(self.n_buckets << self.fine_bits, dtype=torch.int16,
+                                device=self.device)
+        stream = _lib.stream_ptr(self.device)
+        _lib.check(L.sqdk_space_fine(self.keys.data_ptr(), self.n, nq, self.bucket_bits,
+                                     self.fine_bits, self.bstart.data_ptr(),
+                                     self.fdir.data_ptr(), stream), "sqdk_space_fine")
+        max_items = self.n_buckets + -(-self.n // self.chunk_rows)
+        self.items = torch.empty(max_items, dtype=torch.int32, device=self.device)
+        n_items = torch.zeros(1, dtype=torch.int32, device=self.device)
+        _lib.check(L.sqdk_space_items(self.bstart.data_ptr(), self.n_buckets, self.chunk_rows,
+                                      self.items.data_ptr(), max_items, n_items.data_ptr(),
+                                      stream), "sqdk_space_items")
+        self.n_items = int(n_items.item())
+        popcounts = int(self.pc_mask[0].item()) & ((1 << 64) - 1)
+        popcounts |= (int(self.pc_mask[1].item()) & 1) << 64
+        self.popcounts = popcounts
+
+    # -- C view ----------------------------------------------------------------
+
+    def c_struct(self) -> _lib.SqdkSpace:
+        s = _lib.SqdkSpace()
+        s.keys = self.keys.data_ptr()
+        s.bstart = self.bstart.data_ptr()
+        s.fdir = self.fdir.data_ptr()
+        s.items = self.items.data_ptr()
+        s.pc_mask = self.pc_mask.data_ptr()
+        s.n = self.n
+        s.num_qubits = self.num_qubits
+        s.bucket_bits = self.bucket_bits
+        s.fine_bits = self.fine_bits
+        s.chunk_rows = self.chunk_rows
+        s.n_items = self.n_items
+        s.max_bucket = self.max_bucket
+        return s
+
+    def keys_numpy(self) -> np.ndarray:
+        return self.keys.cpu().numpy().view(np.uint64)
+
+    def single_parity(self) -> bool:
+        """True if every key has the same popcount parity."""
+        even = sum((self.popcounts >> w) & 1 for w in range(0, 65, 2))
+        odd = sum((self.popcounts >> w) & 1 for w in range(1, 65, 2))
+        return even == 0 or odd == 0
+
+    def xsource(self, xmask: int) -> torch.Tensor:
+        """Source index of ``key ^ xmask`` for every key, or -1 (int32, on device)."""
+        out = torch.empty(self.n, dtype=torch.int32, device=self.device)
+        sp = self.c_struct()
+        _lib.check(_lib.lib().sqdk_xsource(C.byref(sp), int(xmask) & ((1 << 64) - 1),
+                                           out.data_ptr(), _lib.stream_ptr(self.device)),
+                   "sqdk_xsource")
+        return out
+
+
+@dataclass
+class _HostTables:
+    xmask: np.ndarray
+    grp_off: np.ndarray
+    zmask: np.ndarray
+    coef: np.ndarray
+    class_hi: np.ndarray
+    class_off: np.ndarray
+    diag_terms: int
+    pruned_groups: int
+
+
+class PauliOperator:
+    """A Pauli sum bound to a subspace: ``H`` restricted to ``S``.
+
+    ``ham``: a ``PauliSumXZ`` (padded or not) or ``(labels, coeffs)``. Groups
+    that can never hit a key of ``S`` (odd X weight when every key has the
+    same popcount parity) are dropped: they contribute nothing.
+    """
+
+    def __init__(self, ham, space: Subspace, *, prune: bool = True):
+        if not isinstance(ham, PauliSumXZ):
+            ham = PauliSumXZ.from_paulisum(ham, force_real=True, add_padding=True)
+        if ham.num_qubits != space.num_qubits:
+            raise ValueError(f"Hamiltonian has {ham.num_qubits} qubits, subspace "
+                             f"{space.num_qubits}")
+        self.ham = ham
+        self.space = space
+        self.num_qubits = ham.num_qubits
+        self.is_complex = bool(np.iscomplexobj(ham.c))
+        tables = self._tables(ham, space, prune)
+        self.host = tables
+        dev = space.device
+        as_t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a)).to(dev).view(dt)  # noqa: E731
+        self.xmask = as_t(tables.xmask.view(np.int64), torch.int64)
+        self.grp_off = as_t(tables.grp_off.astype(np.int32), torch.int32)
+        self.zmask = as_t(tables.zmask.view(np.int64), torch.int64)
+        c = np.ascontiguousarray(tables.coef.astype(np.complex128))
+        self.coef = torch.from_numpy(c.view(np.float64)).to(dev)
+        self.coef32 = torch.from_numpy(c.astype(np.complex64).view(np.float32)).to(dev)
+        self.class_hi = as_t(tables.class_hi.astype(np.uint32).view(np.int32), torch.int32)
+        self.class_off = as_t(tables.class_off.astype(np.int32), torch.int32)
+        self.n_groups = int(tables.xmask.shape[0])
+        self.n_terms = int(tables.zmask.shape[0])
+        self.n_classes = int(tables.class_hi.shape[0])
+        self.diag_terms = tables.diag_terms
+        self.tune = _lib.SqdkTune(space.rows_per_thread, 1, 1, 0)
+        self._cstruct = None
+
+    @staticmethod
+    def _tables(ham: PauliSumXZ, space: Subspace, prune: bool) -> _HostTables:
+        xm, zm, coef, count = ham.masks()
+        m = space.bucket_bits
+        is_diag = xm == 0
+        diag_idx = np.nonzero(is_diag)[0]
+        off_idx = np.nonzero(~is_diag)[0]
+        pruned = 0
+        if prune and space.single_parity() and off_idx.size:
+            weights = np.array([bin(int(x)).count("1") for x in xm[off_idx]])
+            keep = (weights & 1) == 0
+            pruned = int(np.count_nonzero(~keep))
+            off_idx = off_idx[keep]
+        hi = (xm[off_idx] >> np.uint64(m)) if m < 64 else np.zeros(off_idx.size, np.uint64)
+        order = np.lexsort((xm[off_idx], hi))
+        off_idx = off_idx[order]
+        hi = hi[order]
+        zs, cs, offs = [], [], [0]
+        diag_terms = 0
+        for g in diag_idx:
+            k = count[g]
+            nz = coef[g, :k] != 0
+            zs.append(zm[g, :k][nz])
+            cs.append(coef[g, :k][nz])
+            diag_terms += int(np.count_nonzero(nz))
+        base = diag_terms
+        for g in off_idx:
+            k = count[g]
+            nz = coef[g, :k] != 0
+            zs.append(zm[g, :k][nz])
+            cs.append(coef[g, :k][nz])
+            base += int(np.count_nonzero(nz))
+            offs.append(base)
+        zmask = np.concatenate(zs) if zs else np.zeros(0, np.uint64)
+        coefs = np.concatenate(cs) if cs else np.zeros(0, np.complex128)
+        if hi.size:
+            starts = np.concatenate([[0], np.nonzero(hi[1:] != hi[:-1])[0] + 1])
+            class_hi = hi[starts]
+            class_off = np.concatenate([starts, [hi.size]])
+        else:
+            class_hi = np.zeros(0, np.uint64)
+            class_off = np.zeros(1, np.int64)
+        return _HostTables(xm[off_idx].astype(np.uint64), np.asarray(offs, np.int64),
+                           zmask.astype(np.uint64), coefs, class_hi, class_off, diag_terms,
+                           pruned)
+
+    def c_struct(self) -> _lib.SqdkHam:
+        h = _lib.SqdkHam()
+        h.xmask = self.xmask.data_ptr()
+        h.grp_off = self.grp_off.data_ptr()
+        h.zmask = self.zmask.data_ptr() if self.n_terms else 0
+        h.coef = self.coef.data_ptr() if self.n_terms else 0
+        h.coef32 = self.coef32.data_ptr() if self.n_terms else 0
+        h.class_hi = self.class_hi.data_ptr() if self.n_classes else 0
+        h.class_off = self.class_off.data_ptr()
+        h.n_groups = self.n_groups
+        h.n_terms = self.n_terms
+        h.n_classes = self.n_classes
+        h.diag_t0 = 0
+        h.diag_t1 = self.diag_terms
+        h.num_qubits = self.num_qubits
+        h.bucket_bits = self.space.bucket_bits
+        return h
+
+    def structs(self):
+        """Cached (ham, space, tune) ctypes structs (keep the operator alive while used)."""
+        if self._cstruct is None:
+            self._cstruct = (self.c_struct(), self.space.c_struct(), self.tune)
+        return self._cstruct
+
+    def matvec(self, v: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """``H v`` for a complex128 or complex64 device vector of length ``n``."""
+        if v.dtype == torch.complex128:
+            prec = _lib.C128
+        elif v.dtype == torch.complex64:
+            prec = _lib.C64
+        else:
+            raise ValueError(f"vector dtype {v.dtype} not complex128/complex64")
+        if v.numel() != self.space.n or v.device != self.space.device:
+            raise ValueError("vector does not match the subspace size/device")
+        v = v.contiguous()
+        if out is None:
+            out = torch.empty_like(v)
+        h, s, t = self.structs()
+        _lib.check(_lib.lib().sqdk_matvec(C.byref(h), C.byref(s), prec, v.data_ptr(),
+                                          out.data_ptr(), C.byref(t),
+                                          _lib.stream_ptr(self.space.device)), "sqdk_matvec")
+        return out
+
+    def diagonal_of_group0(self) -> torch.Tensor:
+        """Coefficient vector of the X = 0 group (complex128; zeros if absent)."""
+        out = torch.zeros(self.space.n, dtype=torch.complex128, device=self.space.device)
+        if self.diag_terms:
+            _lib.check(_lib.lib().sqdk_diagonal(
+                self.zmask.data_ptr(), self.coef.data_ptr(), 0, self.diag_terms,
+                self.space.keys.data_ptr(), self.space.n, out.data_ptr(),
+                _lib.stream_ptr(self.space.device)), "sqdk_diagonal")
+        return out
+
+
+def full_space_matmul(ham: PauliSumXZ, rhs):
+    """``H @ rhs`` over the whole ``2**n`` space (rqutils/paulis/symplectic.py:131-150)."""
+    n = ham.num_qubits
+    is_numpy = not isinstance(rhs, torch.Tensor)
+    vec = torch.as_tensor(np.asarray(rhs)) if is_numpy else rhs
+    if vec.shape[0] != 2 ** n:
+        raise ValueError(f"RHS axis 0 size {vec.shape[0]} is incompatible with num_qubits={n}")
+    if n > 31:
+        raise ValueError("full-space products are limited to 31 qubits")
+    dev = _device(None if is_numpy or vec.device.type != "cuda" else vec.device)
+    space = Subspace(torch.arange(2 ** n, dtype=torch.int64, device=dev), n, device=dev)
+    op = PauliOperator(ham, space, prune=False)
+    cols = vec.reshape(vec.shape[0], -1)
+    out = torch.empty(cols.shape, dtype=torch.complex128, device=dev)
+    for j in range(cols.shape[1]):
+        col = cols[:, j].to(dev, torch.complex128).contiguous()
+        out[:, j] = op.matvec(col)
+    out = out.reshape(vec.shape)
+    if not (np.iscomplexobj(ham.c) or torch.is_complex(vec)):
+        out = out.real
+    if is_numpy:
+        return out.cpu().numpy()
+    return out.to(vec.device)

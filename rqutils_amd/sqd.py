@@ -1,0 +1,227 @@
+r"""Sample-based quantum diagonalization on the GPU (drop-in for ``rqutils.sqd``).
+
+``sqd(hamiltonian, states, states_size=None, return_eigvec=True,
+cache_level=(1, 0))`` keeps the reference's signature and results
+(rqutils/sqd.py:181-248): it projects a Pauli-sum Hamiltonian onto the sorted
+set of distinct sampled bitstrings and returns the lowest eigenvalue, the
+eigenvector over that set and the set itself.
+
+The engine differs from the reference's. States become uint64 keys
+(``sum_q s_q 2^q``; ascending key order is the reference's lexicographic order
+of padded packed rows, rqutils/sqd.py:237, 424-426), the keys live in HBM with
+bucket directories, and the matvec searches every (row, X group) on the fly
+in LDS-staged key buckets, so no J x N source-index cache exists and every
+``cache_level`` gives the same result. The solver is the device LOBPCG of
+:mod:`rqutils_amd.ground_locg`.
+"""
+from __future__ import annotations
+
+import logging
+import time
+from typing import Optional
+
+import numpy as np
+import torch
+from scipy.sparse import coo_array, csr_array
+
+from . import _lib
+from .ground_locg import solve
+from .operator import PauliOperator, Subspace, keys_to_states, states_to_keys
+from .paulis.symplectic import PauliSumXZ, _bits_to_u64, unpack_rows
+
+__all__ = ["sqd", "hproj", "get_xsource", "get_diagonal", "SQDProblem"]
+
+LOG = logging.getLogger(__name__)
+
+_CACHE_LEVELS = {(0, 0), (0, 1), (0, 2), (1, 0), (1, 1), (1, 2)}
+
+
+class SQDProblem:
+    """A Hamiltonian projected on a subspace, resident on one GPU.
+
+    ``keys``: sorted distinct uint64 keys (numpy or torch); ``hamiltonian``:
+    ``PauliSumXZ`` or ``(labels, coeffs)``. Exposes ``matvec`` and ``solve``.
+    """
+
+    def __init__(self, hamiltonian, keys, num_qubits: Optional[int] = None, *, device=None,
+                 **space_kw):
+        if not isinstance(hamiltonian, PauliSumXZ):
+            hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True,
+                                                   add_padding=True)
+        self.hamiltonian = hamiltonian
+        nq = hamiltonian.num_qubits if num_qubits is None else num_qubits
+        self.space = Subspace(keys, nq, device=device, **space_kw)
+        self.op = PauliOperator(hamiltonian, self.space)
+
+    @property
+    def n(self) -> int:
+        return self.space.n
+
+    def matvec(self, v: torch.Tensor, out=None) -> torch.Tensor:
+        return self.op.matvec(v, out)
+
+    def vinit_min_diag(self, dtype=torch.complex128) -> torch.Tensor:
+        """One-hot start at the smallest X=0 diagonal entry, else at row 0
+        (rqutils/sqd.py:373-398)."""
+        v = torch.zeros(self.n, dtype=dtype, device=self.space.device)
+        if np.all(self.hamiltonian.x[0] == 0):
+            diag = self.op.diagonal_of_group0().real
+            v[int(torch.argmin(diag).item())] = 1
+        else:
+            v[0] = 1
+        return v
+
+    def solve(self, x0: torch.Tensor, **kw):
+        return solve(self.op, x0, **kw)
+
+
+def _check_cache_level(cache_level):
+    if tuple(cache_level) not in _CACHE_LEVELS:
+        raise ValueError(f"cache_level must be one of {sorted(_CACHE_LEVELS)}")
+
+
+def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: bool = True,
+        cache_level: tuple[int, int] = (1, 0), *, device=None, mode: str = "reference",
+        precision: str = "c128", maxiter: int = 1000, tol: Optional[float] = None):
+    r"""Sample-based quantum diagonalization (rqutils/sqd.py:181-248).
+
+    Args:
+        hamiltonian: ``(list[str], list[Number])``, a qiskit ``SparsePauliOp``
+            or a ``PauliSumXZ`` built with ``add_padding=True``.
+        states: Binary array ``[N_in, n]`` (column 0 = qubit n-1); duplicates
+            allowed.
+        states_size: Padded problem size; only used, as in the reference, for
+            the length term of the convergence threshold. Must be at least
+            ``len(states)``.
+        return_eigvec: Return the eigenvector and the basis states as well.
+        cache_level: Accepted for compatibility; the device matvec caches no
+            source indices or diagonals, so every level gives the same result.
+        mode / precision / maxiter / tol: solver controls (defaults reproduce
+            the reference: the reference iteration in FP64 with ``tol`` = eps).
+
+    Returns:
+        ``eigval``, or ``(eigval, eigvec, basis_states)`` with ``eigvec`` over
+        the sorted distinct states and ``basis_states`` uint8 ``[dim, n]``.
+    """
+    states = np.asarray(states)
+    if states_size is None:
+        states_size = states.shape[0]
+    if states_size < states.shape[0]:
+        raise ValueError("states_size smaller than the states array length")
+    _check_cache_level(cache_level)
+    if not isinstance(hamiltonian, PauliSumXZ):
+        hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True, add_padding=True)
+    num_qubits = states.shape[1]
+    keys = np.unique(states_to_keys(states))
+    start = time.time()
+    prob = SQDProblem(hamiltonian, keys, num_qubits, device=device)
+    real = not prob.op.is_complex
+    dtype = torch.complex128 if precision == "c128" else torch.complex64
+    x0 = prob.vinit_min_diag(dtype)
+    if tol is None:
+        tol = float(np.finfo(np.float64 if precision == "c128" else np.float32).eps)
+    vec, info = prob.solve(x0, mode=mode, precision=precision, criterion="reference", tol=tol,
+                           maxiter=maxiter, n_for_tol=states_size)
+    LOG.info("Found ground eigenpair in %f seconds.", time.time() - start)
+    eigval = float(info.eigval)
+    if not return_eigvec:
+        return eigval
+    v = vec.cpu().numpy()
+    v = v.real.astype(np.float64) if real else v.astype(np.complex128)
+    return eigval, v, keys_to_states(keys, num_qubits)
+
+
+def hproj(hamiltonian, states, unique_states: bool = False) -> csr_array:
+    """The Hamiltonian projected onto the given states as a scipy CSR matrix.
+
+    Counterpart of rqutils/sqd.py:251-296 with the intended semantics: the
+    packed states and the Hamiltonian use the same bit alignment (the
+    reference packs the states without the pad bit while padding the
+    Hamiltonian). Entry (i, j) is <s_i|H|s_j> over the sorted distinct states.
+    """
+    if not isinstance(hamiltonian, PauliSumXZ):
+        hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, add_padding=True)
+    states = np.asarray(states)
+    keys = states_to_keys(states)
+    if not unique_states:
+        keys = np.unique(keys)
+    n = keys.shape[0]
+    space = Subspace(keys, states.shape[1])
+    op = PauliOperator(hamiltonian, space, prune=False)
+    rows, cols, vals = [], [], []
+    t = op.host
+    row_index = torch.arange(n, device=space.device)
+    if t.diag_terms:
+        d = op.diagonal_of_group0()
+        rows.append(row_index)
+        cols.append(row_index)
+        vals.append(d)
+    coef = op.coef
+    for g in range(op.n_groups):
+        xs = space.xsource(int(t.xmask[g]))
+        dg = torch.empty(n, dtype=torch.complex128, device=space.device)
+        _lib.check(_lib.lib().sqdk_diagonal(
+            op.zmask.data_ptr(), coef.data_ptr(), int(t.grp_off[g]), int(t.grp_off[g + 1]),
+            space.keys.data_ptr(), n, dg.data_ptr(), _lib.stream_ptr(space.device)),
+            "sqdk_diagonal")
+        ok = xs >= 0
+        rows.append(row_index[ok])
+        cols.append(xs[ok].long())
+        vals.append(dg[ok])
+    r = torch.cat(rows).cpu().numpy() if rows else np.zeros(0, np.int64)
+    c = torch.cat(cols).cpu().numpy() if cols else np.zeros(0, np.int64)
+    v = torch.cat(vals).cpu().numpy() if vals else np.zeros(0, np.complex128)
+    if not np.iscomplexobj(hamiltonian.c):
+        v = v.real
+    return csr_array(coo_array((v, (r, c)), shape=(n, n)))
+
+
+def _packed_to_u64(packed: np.ndarray) -> np.ndarray:
+    """Integer of each packed row (all bits, big-endian), for rows of <= 8 bytes."""
+    packed = np.asarray(packed, dtype=np.uint8)
+    nb = packed.shape[-1]
+    if nb > 8:
+        raise ValueError("packed rows wider than 8 bytes are not supported")
+    return _bits_to_u64(np.unpackbits(packed, axis=-1))
+
+
+def get_xsource(xsignature: np.ndarray, states: np.ndarray) -> torch.Tensor:
+    """Source index of ``S[i] ^ x`` in ``S`` or -1 (rqutils/sqd.py:448-503).
+
+    ``states`` are packed, lexicographically sorted distinct rows, optionally
+    followed by fill rows of 0xFF (as ``uniquify_states`` leaves them); fill
+    rows get -1. Bit-exact with the reference on the distinct rows.
+    """
+    states = np.asarray(states, dtype=np.uint8)
+    keys = _packed_to_u64(states)
+    x = int(_packed_to_u64(np.asarray(xsignature, dtype=np.uint8)[None, :])[0])
+    nbits = 8 * states.shape[1]
+    full = (1 << nbits) - 1
+    real = keys != np.uint64(full)
+    nreal = int(np.count_nonzero(real))
+    if nreal and not np.all(real[:nreal]):
+        raise ValueError("fill rows must come after every real row")
+    out = torch.full((states.shape[0],), -1, dtype=torch.int32, device="cuda")
+    if nreal:
+        space = Subspace(keys[:nreal], nbits)
+        out[:nreal] = space.xsource(x)
+    return out
+
+
+def get_diagonal(zsignatures: np.ndarray, coeffs: np.ndarray, states: np.ndarray) -> torch.Tensor:
+    """Composed diagonal ``sum_k c_k (-1)^{popcount(S & z_k)}`` of one X group
+    (rqutils/sqd.py:554-575), as complex128 on the device. Zero coefficients
+    are skipped rather than ending the sum."""
+    states = np.asarray(states, dtype=np.uint8)
+    keys = torch.from_numpy(_packed_to_u64(states).view(np.int64)).cuda()
+    z = _packed_to_u64(np.asarray(zsignatures, dtype=np.uint8))
+    c = np.asarray(coeffs, dtype=np.complex128)
+    keep = c != 0
+    zt = torch.from_numpy(z[keep].view(np.int64)).cuda()
+    ct = torch.from_numpy(np.ascontiguousarray(c[keep]).view(np.float64)).cuda()
+    n = keys.numel()
+    out = torch.empty(n, dtype=torch.complex128, device="cuda")
+    _lib.check(_lib.lib().sqdk_diagonal(zt.data_ptr(), ct.data_ptr(), 0, int(keep.sum()),
+                                        keys.data_ptr(), n, out.data_ptr(), _lib.stream_ptr()),
+               "sqdk_diagonal")
+    return out

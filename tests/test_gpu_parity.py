@@ -1,0 +1,201 @@
+"""GPU parity of the HIP path against the oracle (small, seeded cases).
+
+Integer work (source indices) must match bit for bit; the complex128 matvec
+within 1e-12 relative (different summation order only); complex64 within
+2e-6 relative; eigenvalues within 1e-9 relative of dense eigh.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dense, locg_oracle as lo, sqd_oracle as so
+from tests.helpers import OracleProblem, random_labels, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem_on_gpu(p):
+    from rqutils_amd.sqd import SQDProblem
+    return SQDProblem((p.labels, p.coeffs), p.keys, p.n)
+
+
+def _rand_vec(rng, n):
+    return rng.normal(size=n) + 1j * rng.normal(size=n)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_xsource_bit_exact(gpu, seed):
+    p = small_problem(seed, n=12, terms=50, nstates=700)
+    prob = _problem_on_gpu(p)
+    for g in range(p.x.shape[0]):
+        ref = so.get_xsource(p.x[g], p.packed)
+        xm = int(so.packed_to_keys(p.x[g][None, :], p.n)[0])
+        got = prob.space.xsource(xm).cpu().numpy()
+        assert np.array_equal(got, ref), f"group {g} x={xm:#x}"
+
+
+def test_xsource_full_space_identity(gpu):
+    # S = the whole space: xsource[i] = i ^ x (rqutils/sqd.py:461-463).
+    from rqutils_amd.operator import Subspace
+    n = 14
+    sp = Subspace(np.arange(2 ** n, dtype=np.uint64), n)
+    for x in [0, 1, 0x2A5, 2 ** n - 1]:
+        got = sp.xsource(x).cpu().numpy()
+        assert np.array_equal(got, np.arange(2 ** n) ^ x)
+
+
+@pytest.mark.parametrize("seed", [0, 3])
+def test_matvec_c128_vs_oracle_and_dense(gpu, seed):
+    p = small_problem(seed, n=11, terms=80, nstates=900)
+    prob = _problem_on_gpu(p)
+    rng = np.random.default_rng(100 + seed)
+    v = _rand_vec(rng, p.keys.size)
+    ref = p.matvec(v)
+    got = prob.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+    assert _rel(got, ref) < 1e-12
+    assert _rel(got, p.dense() @ v) < 1e-12
+
+
+def test_matvec_c64_compensated(gpu):
+    p = small_problem(5, n=12, terms=120, nstates=1500)
+    prob = _problem_on_gpu(p)
+    rng = np.random.default_rng(7)
+    v = _rand_vec(rng, p.keys.size).astype(np.complex64)
+    ref = p.matvec(v.astype(np.complex128))
+    got = prob.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+    assert _rel(got.astype(np.complex128), ref) < 2e-6
+
+
+def test_matvec_many_buckets_and_classes(gpu):
+    # A tiny bucket cap forces many buckets, classes and multi-chunk rows.
+    from rqutils_amd.operator import PauliOperator, Subspace
+    p = small_problem(11, n=14, terms=150, nstates=6000)
+    for cap, rows in [(64, 4), (512, 8), (4096, 16)]:
+        sp = Subspace(p.keys, p.n, bucket_cap=cap, rows_per_thread=rows)
+        op = PauliOperator((p.labels, p.coeffs), sp)
+        rng = np.random.default_rng(cap)
+        v = _rand_vec(rng, p.keys.size)
+        got = op.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+        assert _rel(got, p.matvec(v)) < 1e-12, (cap, rows, sp.bucket_bits, sp.n_items)
+
+
+def test_diagonal_group0(gpu):
+    p = small_problem(4, n=10, terms=60, nstates=400)
+    prob = _problem_on_gpu(p)
+    assert np.all(p.x[0] == 0)
+    ref = so.get_diagonal(p.z[0], p.c[0], p.packed)
+    got = prob.op.diagonal_of_group0().cpu().numpy()
+    assert np.allclose(got, ref, rtol=0, atol=1e-12)
+
+
+def test_small_eigenpairs_match_eigh(gpu):
+    from rqutils_amd.ground_locg import eigenpair_2x2, eigenpair_3x3
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        a = a + a.conj().T
+        w, V = np.linalg.eigh(a)
+        lam, vec = eigenpair_3x3(a)
+        assert abs(lam - w[0]) < 1e-12 * max(1, abs(w[0]))
+        assert abs(abs(np.vdot(V[:, 0], vec)) - 1) < 1e-10
+        lam_o, vec_o = lo.eigenpair_3x3(a)
+        assert abs(lam - lam_o) < 1e-13 * max(1, abs(lam_o))
+        b = a[:2, :2]
+        w, V = np.linalg.eigh(b)
+        lam, vec = eigenpair_2x2(b)
+        assert abs(lam - w[0]) < 1e-12 * max(1, abs(w[0]))
+        assert abs(abs(np.vdot(V[:, 0], vec)) - 1) < 1e-10
+
+
+def test_locg_reference_mode_tracks_oracle(gpu):
+    p = small_problem(8, n=10, terms=70, nstates=500)
+    prob = _problem_on_gpu(p)
+    rng = np.random.default_rng(9)
+    x0 = _rand_vec(rng, p.keys.size)
+    th_o, _, it_o = lo.ground_locg(p.matvec, x0)
+    vec, info = prob.solve(torch.from_numpy(x0).cuda(), mode="reference", precision="c128",
+                           criterion="reference", tol=float(np.finfo(np.float64).eps),
+                           maxiter=1000)
+    ev = np.linalg.eigvalsh(p.dense())[0]
+    assert info.converged
+    assert abs(info.eigval - ev) < 1e-9 * abs(ev)
+    assert abs(info.eigval - th_o) < 1e-9 * abs(ev)
+    assert abs(info.niter - it_o) <= max(2, 0.05 * it_o), (info.niter, it_o)
+    assert info.matvecs == 4 * info.niter + 4
+
+
+@pytest.mark.parametrize("precision,tol", [("c128", 1e-10), ("c64", 1e-6)])
+def test_locg_fused_converges(gpu, precision, tol):
+    p = small_problem(12, n=12, terms=120, nstates=2000)
+    prob = _problem_on_gpu(p)
+    rng = np.random.default_rng(13)
+    x0 = _rand_vec(rng, p.keys.size)
+    vec, info = prob.solve(torch.from_numpy(x0).cuda(), mode="fused", precision=precision,
+                           criterion="relative", tol=tol, maxiter=2000)
+    H = p.dense()
+    ev = np.linalg.eigvalsh(H)[0]
+    x = vec.cpu().numpy().astype(np.complex128)
+    x /= np.linalg.norm(x)
+    r = np.linalg.norm(H @ x - info.eigval * x) / abs(info.eigval)
+    assert info.converged
+    assert r <= 1.5 * tol
+    assert abs(info.eigval - ev) / abs(ev) < max(10 * tol * tol, 1e-12) + 1e-7 * (precision == "c64")
+
+
+def test_sqd_end_to_end_vs_dense(gpu):
+    # The reference notebook's check, seeded: 12 qubits, 100 full-weight terms.
+    from rqutils_amd.sqd import sqd
+    rng = np.random.default_rng(2024)
+    n = 12
+    labels = ["".join("IXYZ"[i] for i in row) for row in rng.integers(0, 4, (100, n))]
+    coeffs = rng.uniform(-1, 1, 100)
+    states = rng.integers(0, 2, (3000, n)).astype(np.uint8)
+    eigval, eigvec, basis = sqd((labels, coeffs), states)
+    uniq = np.unique(states, axis=0)
+    assert np.array_equal(basis, uniq)
+    H = dense.dense_hamiltonian(labels, coeffs)
+    idx = dense.states_to_index(uniq)
+    w, V = np.linalg.eigh(dense.projected(H, idx))
+    assert abs(eigval - w[0]) < 1e-9 * abs(w[0])
+    assert abs(abs(np.vdot(V[:, 0], eigvec)) - 1) < 1e-6
+
+
+def test_hproj_matches_dense(gpu):
+    from rqutils_amd.sqd import hproj
+    p = small_problem(21, n=9, terms=40, nstates=200)
+    states = p.states
+    m = hproj((p.labels, p.coeffs), states).toarray()
+    assert np.allclose(m, p.dense(), atol=1e-12)
+
+
+def test_full_space_matmul_vs_dense(gpu):
+    from rqutils_amd.paulis import PauliSumXZ
+    rng = np.random.default_rng(5)
+    n = 8
+    labels = random_labels(rng, n, 30, full=True)
+    coeffs = rng.normal(size=30)
+    h = PauliSumXZ.from_paulisum((labels, coeffs))
+    v = _rand_vec(rng, 2 ** n)
+    got = h.matmul(v)
+    assert np.allclose(got, dense.dense_hamiltonian(labels, coeffs) @ v, atol=1e-12)
+
+
+def test_ground_locg_dense_and_callable(gpu):
+    from rqutils_amd.ground_locg import ground_locg
+    rng = np.random.default_rng(6)
+    a = rng.normal(size=(60, 60))
+    a = a + a.T
+    w = np.linalg.eigvalsh(a)
+    x0 = rng.normal(size=60)
+    ev, vec, it = ground_locg(a, x0)
+    assert abs(ev - w[0]) < 1e-10 * abs(w[0])
+    assert vec.dtype == torch.float64
+    at = torch.from_numpy(a).cuda()
+    ev2, vec2, it2 = ground_locg(lambda v, s: s * torch.mv(at, v), x0, args=(1.0,))
+    assert abs(ev2 - w[0]) < 1e-10 * abs(w[0])
+    ev3, _, _ = ground_locg(a, int(np.argmin(np.diag(a))))
+    assert abs(ev3 - w[0]) < 1e-9 * abs(w[0])
