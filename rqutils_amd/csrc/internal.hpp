@@ -12,10 +12,10 @@ int check_space_public(const sqdk_space* sp);
 
 // Matvec with an optional device skip flag: when *skip != 0 at kernel start
 // the launch does nothing (used for iterations after convergence inside a
-// captured chunk).
+// chunk enqueued ahead). `counter`, if set, counts the launches that ran.
 int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                 const void* v, void* out, const sqdk_tune* tune, hipStream_t s,
-                const int32_t* skip = nullptr);
+                const int32_t* skip = nullptr, int32_t* counter = nullptr);
 
 constexpr int kMaxPairs = 16;
 constexpr int kDotBlocks = 512;   // partial sums per pair (fixed: deterministic)

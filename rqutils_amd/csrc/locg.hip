@@ -39,7 +39,7 @@ enum Slot : int {
   kInfo2 = 65,   // .x reltol / tol*|theta|, .y last criterion value
   kNumSlots = 80
 };
-enum Flag : int { kConv = 0, kNiter = 1, kPValid = 2, kStatus = 3, kNumFlags = 16 };
+enum Flag : int { kConv = 0, kNiter = 1, kPValid = 2, kStatus = 3, kMatvecs = 4, kNumFlags = 16 };
 
 constexpr size_t kSlotBytes = kNumSlots * sizeof(double2);
 constexpr size_t kFlagBytes = kNumFlags * sizeof(int32_t);
@@ -474,13 +474,14 @@ struct Solver {
   int matvecs = 0;
 
   int apply(const void* in, void* out, bool skip) {
-    ++matvecs;
     if (op->fn) {
+      ++matvecs;  // callbacks always run (the host polls every iteration)
       const int rc = op->fn(op->ctx, in, out, s);
       if (rc != 0) return fail(SQDK_ECALLBACK, "matvec callback returned %d", rc);
       return SQDK_OK;
     }
-    return matvec_impl(op->ham, op->space, prec, in, out, op->tune, s, skip ? F + kConv : nullptr);
+    return matvec_impl(op->ham, op->space, prec, in, out, op->tune, s, skip ? F + kConv : nullptr,
+                       F + kMatvecs);
   }
   int dots(std::initializer_list<std::pair<const void*, const void*>> pairs, bool skip,
            int slot = kD) {
@@ -746,7 +747,7 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   result->ax_norm = slots[kInfo].y;
   result->niter = flags[kNiter];
   result->converged = flags[kConv];
-  result->matvecs = sv.matvecs;
+  result->matvecs = sv.matvecs + flags[kMatvecs];
   result->status = flags[kStatus];
   return SQDK_OK;
 }

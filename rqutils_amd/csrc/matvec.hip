@@ -41,6 +41,7 @@ struct MatvecArgs {
   const VT* __restrict__ v;
   VT* __restrict__ out;
   const int32_t* __restrict__ skip;
+  int32_t* __restrict__ counter;
   int32_t n_items;
   int32_t m;          // bucket bits
   int32_t d;          // fine bits
@@ -69,6 +70,7 @@ bucket_join_kernel(MatvecArgs<VT> a) {
   const int w = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   if (w >= a.n_items) return;
   if (a.skip && a.skip[0] != 0) return;
+  if (a.counter && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.counter, 1);
 
   const int m = a.m;
   const int d = a.d;
@@ -175,7 +177,8 @@ int launch_kt(const MatvecArgs<VT>& args, int rows, size_t lds, hipStream_t s) {
 
 template <typename VT>
 int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const void* v,
-              void* out, int rows, int use_pc, int xcd, hipStream_t s, const int32_t* skip) {
+              void* out, int rows, int use_pc, int xcd, hipStream_t s, const int32_t* skip,
+              int32_t* counter) {
   MatvecArgs<VT> args;
   args.keys = sp->keys;
   args.bstart = sp->bstart;
@@ -191,6 +194,7 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.v = reinterpret_cast<const VT*>(v);
   args.out = reinterpret_cast<VT*>(out);
   args.skip = skip;
+  args.counter = counter;
   args.n_items = sp->n_items;
   args.m = sp->bucket_bits;
   args.d = sp->fine_bits;
@@ -214,7 +218,7 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
 
 int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                 const void* v, void* out, const sqdk_tune* tune, hipStream_t s,
-                const int32_t* skip) {
+                const int32_t* skip, int32_t* counter) {
   SQDK_TRY(check_space_public(space));
   SQDK_REQUIRE(ham != nullptr, "ham is NULL");
   SQDK_REQUIRE(v && out, "NULL vector");
@@ -245,11 +249,11 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                space->chunk_rows);
   if (precision == SQDK_C128) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, use_pc, xcd, s, skip);
+    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, use_pc, xcd, s, skip, counter);
   }
   if (precision == SQDK_C64) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, use_pc, xcd, s, skip);
+    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, use_pc, xcd, s, skip, counter);
   }
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }

@@ -140,7 +140,11 @@ class Subspace:
         if self.max_bucket >= 65536:
             raise ValueError(f"largest bucket holds {self.max_bucket} keys (limit 65535)")
         if fine_bits is None:
-            fine_bits = min(self.bucket_bits, 13, max(0, int(np.ceil(np.log2(self.max_bucket)))))
+            # Sized by the mean non-empty bucket: the directory slice staged per
+            # class then costs at most about as many bytes as the keys.
+            sizes = self.bstart[1:] - self.bstart[:-1]
+            mean = float(self.n) / max(1, int(torch.count_nonzero(sizes).item()))
+            fine_bits = min(self.bucket_bits, 13, max(0, int(round(np.log2(max(mean, 1.0))))))
         self.fine_bits = int(fine_bits)
         self.fdir = torch.empty(self.n_buckets << self.fine_bits, dtype=torch.int16,
                                 device=self.device)
@@ -262,7 +266,7 @@ class PauliOperator:
         order = np.lexsort((xm[off_idx], hi))
         off_idx = off_idx[order]
         hi = hi[order]
-        zs, cs, offs = [], [], [0]
+        zs, cs = [], []
         diag_terms = 0
         for g in diag_idx:
             k = count[g]
@@ -270,7 +274,9 @@ class PauliOperator:
             zs.append(zm[g, :k][nz])
             cs.append(coef[g, :k][nz])
             diag_terms += int(np.count_nonzero(nz))
+        # off-diagonal groups follow the X = 0 terms in the term arrays
         base = diag_terms
+        offs = [base]
         for g in off_idx:
             k = count[g]
             nz = coef[g, :k] != 0

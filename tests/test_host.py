@@ -1,0 +1,154 @@
+"""CPU tests of the host side: the C-ABI library loads and exports every
+function include/sqdk.h declares (no compute call without a GPU), the Pauli
+representation, key packing, the class tables and the workload generators."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import sqd_oracle as so
+from tests.helpers import random_labels
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "sqdk.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sqdk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_header_function():
+    import torch  # noqa: F401  (the library binds to torch's HIP runtime)
+    from rqutils_amd import _lib
+    path = _lib.library_path()
+    assert path.exists(), "build the library first: python -m rqutils_amd.build"
+    handle = ctypes.CDLL(str(path))
+    names = _header_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(handle, name), name
+    # the ctypes binding declares exactly the header's functions
+    assert sorted(_lib.SIGNATURES) == names
+
+
+def test_abi_version_and_error_path_without_gpu():
+    from rqutils_amd import _lib
+    L = _lib.lib()
+    assert L.sqdk_abi_version() == _lib.ABI_VERSION
+    # argument validation happens before any HIP call
+    rc = L.sqdk_dots_workspace(10, 99, None)
+    assert rc == _lib.EINVAL
+    with pytest.raises(ValueError):
+        _lib.check(rc, "dots")
+    assert b"NULL" in L.sqdk_last_error() or b"npairs" in L.sqdk_last_error()
+    nbytes = ctypes.c_size_t(0)
+    assert L.sqdk_dots_workspace(100, 4, ctypes.byref(nbytes)) == 0 and nbytes.value > 0
+
+
+def test_paulisumxz_matches_oracle_packing():
+    from rqutils_amd.paulis import PauliSumXZ
+    rng = np.random.default_rng(0)
+    for n, full in [(5, True), (8, False), (13, False), (16, True)]:
+        labels = random_labels(rng, n, 40, 1, 4, full=full)
+        coeffs = rng.normal(size=40)
+        for pad in (False, True):
+            h = PauliSumXZ.from_paulisum((labels, coeffs), force_real=True, add_padding=pad)
+            x, z, c, nq = so.from_paulisum(labels, coeffs, force_real=True, add_padding=pad)
+            assert np.array_equal(h.x, x) and np.array_equal(h.z, z)
+            assert np.allclose(h.c, c, rtol=0, atol=0) and h.c.dtype == c.dtype
+            assert h.num_qubits == nq
+
+
+def test_paulisumxz_errors_and_masks():
+    from rqutils_amd.paulis import PauliSumXZ
+    with pytest.raises(ValueError):
+        PauliSumXZ.from_paulisum((["XX", "Y"], [1.0, 2.0]))
+    with pytest.raises(ValueError):
+        PauliSumXZ.from_paulisum((["XX"], [1.0, 2.0]))
+    with pytest.raises(ValueError):
+        PauliSumXZ.from_paulisum({"XX": 1.0})
+    with pytest.warns(UserWarning):
+        PauliSumXZ.from_paulisum((["XY"], [1.0 + 1.0j]), force_real=True)
+    h = PauliSumXZ.from_paulisum((["XIZ", "YZI", "ZZZ"], [1.0, 2.0, 3.0]), add_padding=True)
+    xm, zm, coef, count = h.masks()
+    assert xm.tolist() == [0, 0b100]          # ZZZ group, then X on qubit 2
+    assert count.tolist() == [1, 2]
+    assert zm[0, 0] == 0b111
+    # YZI: x=100, z=110 -> popcount(x&z)=1 -> coefficient 2 * (-i)
+    assert 2.0 * -1j in coef[1].tolist()
+
+
+def test_keys_round_trip_and_order():
+    from rqutils_amd.operator import keys_to_states, states_to_keys
+    rng = np.random.default_rng(1)
+    for n in (3, 8, 17, 40, 64):
+        states = rng.integers(0, 2, (200, n)).astype(np.uint8)
+        keys = states_to_keys(states)
+        assert np.array_equal(keys_to_states(keys, n), states)
+        if n < 64:
+            assert np.array_equal(keys, so.packed_to_keys(so.pack_states(states), n))
+
+
+class _FakeSpace:
+    def __init__(self, m, parity_single):
+        self.bucket_bits = m
+        self._p = parity_single
+
+    def single_parity(self):
+        return self._p
+
+
+def test_class_tables_order_and_pruning():
+    from rqutils_amd.operator import PauliOperator
+    from rqutils_amd.paulis import PauliSumXZ
+    rng = np.random.default_rng(2)
+    n = 12
+    labels = random_labels(rng, n, 300, 1, 4)
+    coeffs = rng.normal(size=300)
+    h = PauliSumXZ.from_paulisum((labels, coeffs), force_real=True, add_padding=True)
+    xm, zm, coef, count = h.masks()
+    for m in (0, 4, 9, 12):
+        t = PauliOperator._tables(h, _FakeSpace(m, False), prune=True)
+        hi = t.xmask >> np.uint64(m)
+        # classes partition the ordered groups by high part, ascending
+        assert np.all(np.diff(hi.astype(np.int64)) >= 0)
+        for c in range(t.class_hi.size):
+            seg = hi[t.class_off[c]:t.class_off[c + 1]]
+            assert np.all(seg == t.class_hi[c])
+        assert t.class_off[-1] == t.xmask.size
+        # every nonzero term is present exactly once
+        assert t.zmask.size == int(np.count_nonzero(coef))
+        assert t.diag_terms == int(np.count_nonzero(coef[xm == 0]))
+        assert 0 not in t.xmask.tolist()
+        # term ranges: off-diagonal groups start right after the X = 0 terms
+        assert t.grp_off[0] == t.diag_terms and t.grp_off[-1] == t.zmask.size
+        assert np.all(np.diff(t.grp_off) >= 1)
+        for j, x in enumerate(t.xmask):
+            g = int(np.nonzero(xm == x)[0][0])
+            want = sorted(zm[g, :count[g]][coef[g, :count[g]] != 0].tolist())
+            assert sorted(t.zmask[t.grp_off[j]:t.grp_off[j + 1]].tolist()) == want
+    t = PauliOperator._tables(h, _FakeSpace(4, True), prune=True)
+    w = np.array([bin(int(x)).count("1") for x in t.xmask])
+    assert np.all(w % 2 == 0) and t.pruned_groups > 0
+
+
+def test_workload_generators_small():
+    from rqutils_amd import workloads as W
+    w = W.small16()
+    assert w.n == 4096 and np.all(np.diff(w.keys.astype(np.int64)) > 0)
+    assert set(np.bitwise_count(w.keys).tolist()) == {8}
+    assert len(w.labels) == 200 and all(1 <= sum(ch != "I" for ch in l) <= 4 for l in w.labels)
+    k = W.domain_wall_keys(np.random.default_rng(0), 16, 8, 3000)
+    assert k.size == 3000 and set(np.bitwise_count(k).tolist()) == {8}
+    assert np.all(np.diff(k.astype(np.int64)) > 0)
+    # low domain-wall states are over-represented against the uniform sector
+    bits = ((k[:, None] >> np.arange(16, dtype=np.uint64)) & np.uint64(1)).astype(int)
+    walls = np.count_nonzero(np.diff(bits, axis=1), axis=1)
+    u = W.sector_uniform_keys(np.random.default_rng(0), 16, 8, 3000)
+    ub = ((u[:, None] >> np.arange(16, dtype=np.uint64)) & np.uint64(1)).astype(int)
+    assert walls.mean() < np.count_nonzero(np.diff(ub, axis=1), axis=1).mean()
+    b = W.batch_instance(5)
+    assert 20 <= b.num_qubits <= 32 and 2 ** 12 - 1 <= b.n <= 2 ** 20 + 1

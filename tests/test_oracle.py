@@ -1,0 +1,179 @@
+"""Pins the oracle itself (CPU only): brute force, closed forms, invariants and
+the committed golden fixtures. Each check is chosen so that a plausible slip
+in the restatement (a dropped phase, a wrong sign, a transposed operand, an
+off-by-one in the compaction) fails it."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import dense, locg_oracle as lo, sqd_oracle as so
+from tests.helpers import OracleProblem, small_problem
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---- Pauli grouping and phases --------------------------------------------------
+
+def test_single_pauli_phases_match_dense():
+    # Each one-qubit string against its 2x2 matrix: pins the (-i)^{x.z} fold and Y.
+    for lab in ["X", "Y", "Z", "XY", "YZ", "ZY", "YY", "XZY"]:
+        n = len(lab)
+        p = OracleProblem([lab], [1.0], np.arange(2 ** n), n)
+        v = np.random.default_rng(0).normal(size=2 ** n) + 0j
+        assert np.allclose(p.matvec(v), dense.pauli_matrix(lab) @ v), lab
+
+
+def test_grouping_merges_and_drops_zero():
+    x, z, c, n = so.from_paulisum(["XZ", "XZ", "ZZ", "IX", "YI"], [1.0, 2.0, 0.0, 1.5, -1.0])
+    # ZZ dropped (zero), XZ merged (3.0); groups ascending by X: IX (x=01), XZ/YI (x=10)
+    xb = np.unpackbits(x, axis=1)[:, :n]
+    assert xb.tolist() == [[0, 1], [1, 0]]
+    assert c.shape == (2, 2)
+    # XZ: x=10, z=01 -> (-i)^0 -> 3.0 ; YI: x=10, z=10 -> (-i)^1 * (-1) -> +1j
+    assert set(np.round(c[1], 12).tolist()) == {3.0 + 0j, 1j}
+    assert np.round(c[0], 12).tolist() == [1.5 + 0j, 0j]
+
+
+def test_matvec_matches_dense_kronecker():
+    for seed in range(4):
+        p = small_problem(seed, n=9, terms=40, nstates=180, full=(seed % 2 == 1))
+        rng = np.random.default_rng(seed)
+        v = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+        assert np.allclose(p.matvec(v), p.dense() @ v, atol=1e-12)
+
+
+def test_matvec_hermitian():
+    p = small_problem(7, n=10, terms=60, nstates=400)
+    rng = np.random.default_rng(1)
+    u = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    v = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    assert abs(np.vdot(u, p.matvec(v)) - np.vdot(p.matvec(u), v)) < 1e-10
+
+
+# ---- source indices -------------------------------------------------------------
+
+@pytest.mark.parametrize("seed", range(4))
+def test_xsource_sort_join_equals_lookup(seed):
+    p = small_problem(seed, n=11, terms=30, nstates=500)
+    for g in range(p.x.shape[0]):
+        assert np.array_equal(so.get_xsource(p.x[g], p.packed),
+                              so.xsource_by_lookup(p.x[g], p.packed))
+
+
+def test_xsource_full_space_identity():
+    n = 10
+    states = dense_states = np.array([[(k >> (n - 1 - j)) & 1 for j in range(n)]
+                                      for k in range(2 ** n)], dtype=np.uint8)
+    packed = so.pack_states(dense_states)
+    for x in [1, 0x155, 2 ** n - 1]:
+        xb = so.pack_states(np.array([[(x >> (n - 1 - j)) & 1 for j in range(n)]], np.uint8))[0]
+        assert np.array_equal(so.get_xsource(xb, packed), np.arange(2 ** n) ^ x)
+    assert states.shape == (2 ** n, n)
+
+
+def test_key_order_equals_packed_row_order():
+    rng = np.random.default_rng(3)
+    for n in (7, 8, 15, 16, 23, 40):
+        states = rng.integers(0, 2, (300, n)).astype(np.uint8)
+        packed = so.pack_states(states)
+        keys = so.packed_to_keys(packed, n)
+        order_rows = np.lexsort(packed.T[::-1])
+        assert np.array_equal(np.sort(keys), keys[order_rows])
+
+
+def test_uniquify_fill_rows():
+    rng = np.random.default_rng(4)
+    states = rng.integers(0, 2, (50, 6)).astype(np.uint8)
+    packed = so.pack_states(states)
+    su, nu = so.uniquify_states(packed, 80)
+    assert nu == np.unique(states, axis=0).shape[0]
+    assert np.all(su[nu:] == 255)
+    assert np.all(su[:nu, 0] >> 7 == 0)
+
+
+# ---- LOBPCG -----------------------------------------------------------------------
+
+def test_eigenpairs_closed_forms():
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        a = rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3))
+        a = a + a.conj().T
+        w, V = np.linalg.eigh(a)
+        lam, k = lo.eigenpair_3x3(a)
+        assert abs(lam - w[0]) < 1e-12 * max(1, abs(w[0]))
+        assert abs(abs(np.vdot(V[:, 0], k)) - 1) < 1e-9
+        b = a[:2, :2]
+        w2, V2 = np.linalg.eigh(b)
+        lam2, k2 = lo.eigenpair_2x2(b)
+        assert abs(lam2 - w2[0]) < 1e-12 * max(1, abs(w2[0]))
+        assert abs(abs(np.vdot(V2[:, 0], k2)) - 1) < 1e-9
+    # diagonal 3x3: lowest eigenvalue exact; the cross-product vector degenerates (quirk Q5)
+    lam, _ = lo.eigenpair_3x3(np.diag([3.0, -2.0, 1.0]).astype(complex))
+    assert abs(lam + 2.0) < 1e-12
+
+
+def test_project_out_invariants():
+    rng = np.random.default_rng(2)
+    n = 200
+    x = rng.normal(size=n) + 1j * rng.normal(size=n)
+    x /= np.linalg.norm(x)
+    y = rng.normal(size=n) + 1j * rng.normal(size=n)
+    y -= x * np.vdot(x, y)
+    y /= np.linalg.norm(y)
+    r = rng.normal(size=n) + 1j * rng.normal(size=n)
+    p = lo.project_out((x, y), r)
+    assert abs(np.vdot(x, p)) < 1e-14 and abs(np.vdot(y, p)) < 1e-14
+    assert abs(np.linalg.norm(p) - 1) < 1e-12
+    # a residual inside span(x, y): the result is still zero-or-orthonormal
+    # (normalisation amplifies the rounding left after the first subtraction)
+    q = lo.project_out((x, y), 0.3 * x - 2j * y)
+    nq = np.linalg.norm(q)
+    assert nq == 0.0 or (abs(nq - 1) < 1e-12 and abs(np.vdot(x, q)) < 1e-12)
+
+
+def test_ground_locg_oracle_vs_eigh():
+    p = small_problem(5, n=9, terms=50, nstates=300)
+    h = p.dense()
+    rng = np.random.default_rng(5)
+    x0 = rng.normal(size=h.shape[0]) + 1j * rng.normal(size=h.shape[0])
+    th, x, it = lo.ground_locg(lambda v: h @ v, x0)
+    w = np.linalg.eigvalsh(h)
+    assert abs(th - w[0]) < 1e-10 * abs(w[0])
+    assert 0 < it < 1000
+    # maxiter exhaustion returns silently with niter == maxiter
+    _, _, it1 = lo.ground_locg(lambda v: h @ v, x0, maxiter=1)
+    assert it1 == 1
+
+
+# ---- golden fixtures ------------------------------------------------------------
+
+def _golden(name):
+    path = os.path.join(GOLDEN, name)
+    return np.load(path, allow_pickle=False)
+
+
+def test_golden_small16_against_oracle():
+    g = _golden("small16.npz")
+    labels = [str(s) for s in g["labels"]]
+    p = OracleProblem(labels, g["coeffs"], g["keys"], int(g["n"]))
+    # matvec against the fixture's dense product
+    assert np.allclose(p.matvec(g["v"]), g["hv"], rtol=0, atol=1e-11)
+    # source indices of three groups against the fixture's brute-force lookup
+    for k, gi in enumerate(g["xs_groups"]):
+        assert np.array_equal(so.get_xsource(p.x[gi], p.packed), g["xs"][k])
+    # oracle LOBPCG converges to the fixture's dense eigenvalue
+    rng = np.random.default_rng(int(g["seed_x0"]))
+    x0 = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    th, _, _ = lo.ground_locg(p.matvec, x0)
+    assert abs(th - float(g["e0"])) < 1e-10 * abs(float(g["e0"]))
+
+
+def test_golden_phase_table():
+    g = _golden("pauli_phases.npz")
+    for lab, mat in zip(g["labels"], g["mats"]):
+        n = len(str(lab))
+        p = OracleProblem([str(lab)], [1.0], np.arange(2 ** n), n)
+        cols = np.stack([p.matvec(np.eye(2 ** n)[:, j].astype(complex)) for j in range(2 ** n)],
+                        axis=1)
+        assert np.allclose(cols, mat[: 2 ** n, : 2 ** n]), str(lab)
