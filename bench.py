@@ -88,7 +88,11 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solution block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-groups", type=int, default=3)
-    ap.add_argument("--rows-per-thread", type=int, default=8)
+    ap.add_argument("--rows-per-thread", type=int, default=4)
+    ap.add_argument("--stage-v", type=int, default=0, help="stage v slices in LDS (0/1)")
+    ap.add_argument("--bucket-cap", type=int, default=None, help="max keys per bucket")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy first (scale for rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -100,12 +104,24 @@ def main():
     from rqutils_amd.sqd import SQDProblem
 
     dev = torch.device("cuda", 0)
+    if args.calibrate_copy:
+        # A plain copy of known size: 1 GiB read + 1 GiB written per launch.
+        src = torch.ones(1 << 28, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+        log("calibration copy: 3 x 1 GiB device copies done")
     t = time.time()
     w = W.make(args.workload)
     log(f"workload {w.name}: {w.n} states, {len(w.labels)} terms ({time.time() - t:.1f} s)")
     t = time.time()
-    prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev,
-                      rows_per_thread=args.rows_per_thread)
+    space_kw = {"rows_per_thread": args.rows_per_thread}
+    if args.bucket_cap:
+        space_kw["bucket_cap"] = args.bucket_cap
+    prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, **space_kw)
+    prob.op.tune.stage_v = args.stage_v
     torch.cuda.synchronize()
     sp, op = prob.space, prob.op
     log(f"space: bucket_bits={sp.bucket_bits} fine_bits={sp.fine_bits} buckets={sp.n_buckets} "
@@ -172,8 +188,9 @@ def main():
         "config": {"workload": w.name, "num_qubits": w.num_qubits, "states": n,
                    "terms": len(w.labels), "groups_searched": j_s,
                    "groups_pruned_odd": op.host.pruned_groups, "classes": op.n_classes,
-                   "hits": hits, "bucket_bits": sp.bucket_bits, "rows_per_thread":
-                   args.rows_per_thread, "parallelism": "single GPU"},
+                   "hits": hits, "bucket_bits": sp.bucket_bits, "fine_bits": sp.fine_bits,
+                   "rows_per_thread": args.rows_per_thread, "stage_v": args.stage_v,
+                   "parallelism": "single GPU"},
         "matvecs_per_s": round(1e3 / ms, 2),
         "matvec_c128_ms": round(ms128, 4),
         "matvec_c128_gbs": round(b128 / (ms128 * 1e-3) / 1e9, 2),
@@ -182,7 +199,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(os.path.join(
                          ROOT, "profiles", f"bucket_join_kernel_{w.name}_traffic.md")),
-                     "kernel": "bucket_join_kernel<float2,uint32_t,8>",
+                     "kernel": f"bucket_join_kernel<float2,uint32_t,{args.rows_per_thread},{bool(args.stage_v)}>",
                      "bytes_per_launch": b_mv},
         "wall_s_timed": round(wall, 3),
     }

@@ -59,11 +59,11 @@ typedef struct sqdk_space {
   const uint16_t* fdir;    /* [2^(num_qubits - bucket_bits) << fine_bits]:
                               keys of bucket b below sub-bucket e */
   const uint32_t* items;   /* [n_items] first row of each row chunk */
-  const uint64_t* pc_mask; /* [2]: bit w set iff some key has popcount w */
+  const uint64_t* pc_mask; /* [2]: bit w set iff some key has popcount w (host pruning) */
   int64_t n;               /* 1 <= n < 2^31 */
   int32_t num_qubits;      /* key width, 1..64 */
   int32_t bucket_bits;     /* m: bucket = key >> m; num_qubits - m <= 26 */
-  int32_t fine_bits;       /* d <= min(m, 14) */
+  int32_t fine_bits;       /* d <= min(m, 15) */
   int32_t chunk_rows;      /* rows per item; items never cross a bucket */
   int32_t n_items;
   int32_t max_bucket;      /* keys in the largest bucket (<= 16384) */
@@ -93,9 +93,9 @@ typedef struct sqdk_ham {
 /* Launch tuning of the bucket-join matvec (0 = library default). */
 typedef struct sqdk_tune {
   int32_t rows_per_thread; /* 4, 8 or 16; must match chunk_rows / 256 */
-  int32_t use_pc_filter;   /* skip targets whose popcount no key has */
+  int32_t reserved;        /* must be 0 */
   int32_t xcd_remap;       /* map consecutive items onto one XCD */
-  int32_t reserved;
+  int32_t stage_v;         /* stage each target bucket's slice of v in LDS */
 } sqdk_tune;
 
 /* ---- library ----------------------------------------------------------- */

@@ -44,8 +44,8 @@ class SqdkHam(C.Structure):
 
 
 class SqdkTune(C.Structure):
-    _fields_ = [("rows_per_thread", C.c_int32), ("use_pc_filter", C.c_int32),
-                ("xcd_remap", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("rows_per_thread", C.c_int32), ("reserved", C.c_int32),
+                ("xcd_remap", C.c_int32), ("stage_v", C.c_int32)]
 
 
 class SqdkLocgOpts(C.Structure):

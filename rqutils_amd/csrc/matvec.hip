@@ -25,30 +25,34 @@ template <typename VT> struct CoefT;
 template <> struct CoefT<double2> { using type = double2; using real = double; };
 template <> struct CoefT<float2> { using type = float2; using real = float; };
 
-template <typename VT>
-struct MatvecArgs {
-  const uint64_t* __restrict__ keys;
-  const uint32_t* __restrict__ bstart;
-  const uint16_t* __restrict__ fdir;
-  const uint32_t* __restrict__ items;
-  const uint64_t* __restrict__ pc_mask;
-  const uint64_t* __restrict__ xmask;
-  const int32_t* __restrict__ grp_off;
-  const uint64_t* __restrict__ zmask;
-  const typename CoefT<VT>::type* __restrict__ coef;
-  const uint32_t* __restrict__ class_hi;
-  const int32_t* __restrict__ class_off;
-  const VT* __restrict__ v;
-  VT* __restrict__ out;
-  const int32_t* __restrict__ skip;
-  int32_t* __restrict__ counter;
+// Scalar launch parameters; every table is a separate __restrict__ kernel
+// argument so that the compiler can read uniform entries with scalar loads.
+struct MatvecParams {
+  const int32_t* skip;
+  int32_t* counter;
   int32_t n_items;
   int32_t m;          // bucket bits
   int32_t d;          // fine bits
   int32_t n_classes;
   int32_t diag_t0, diag_t1;
-  int32_t use_pc;
   int32_t xcd_remap;
+  int32_t max_bucket;
+};
+
+template <typename VT>
+struct MatvecTables {
+  const uint64_t* keys;
+  const uint32_t* bstart;
+  const uint16_t* fdir;
+  const uint32_t* items;
+  const uint64_t* xmask;
+  const int32_t* grp_off;
+  const uint64_t* zmask;
+  const typename CoefT<VT>::type* coef;
+  const uint32_t* class_hi;
+  const int32_t* class_off;
+  const VT* v;
+  VT* out;
 };
 
 // Workgroup i runs on XCD i % 8; give each XCD a contiguous range of items so
@@ -60,9 +64,30 @@ __device__ inline int remap_xcd(int i, int g) {
   return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
-template <typename VT, typename KT, int ROWS>
+// Byte layout of the dynamic LDS tables of one workgroup.
+struct LdsLayout {
+  size_t pairs, occ, keys, vals, total;
+};
+
+__host__ __device__ inline LdsLayout lds_layout(int fine_bits, int max_bucket, size_t key_bytes, size_t val_bytes) {
+  LdsLayout l;
+  l.pairs = 0;
+  l.occ = ((size_t)4 << fine_bits);
+  l.keys = l.occ + 16 * sizeof(int);
+  l.vals = (l.keys + key_bytes * ((size_t)max_bucket + 1) + 15) & ~(size_t)15;
+  l.total = l.vals + val_bytes * (size_t)max_bucket;
+  return l;
+}
+
+template <typename VT, typename KT, int ROWS, bool STAGE_V>
 __global__ void __launch_bounds__(kBlock)
-bucket_join_kernel(MatvecArgs<VT> a) {
+bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
+                   const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ items,
+                   const uint64_t* __restrict__ xmask,
+                   const int32_t* __restrict__ grp_off, const uint64_t* __restrict__ zmask,
+                   const typename CoefT<VT>::type* __restrict__ coef,
+                   const uint32_t* __restrict__ class_hi, const int32_t* __restrict__ class_off,
+                   const VT* __restrict__ v, VT* __restrict__ out, MatvecParams a) {
   using CT = typename CoefT<VT>::type;
   using RT = typename CoefT<VT>::real;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -76,26 +101,32 @@ bucket_join_kernel(MatvecArgs<VT> a) {
   const int d = a.d;
   const int f = m - d;
   const uint64_t lowmask = (m >= 64) ? ~0ull : ((1ull << m) - 1ull);
-  const uint32_t dmask = (1u << d) - 1u;
   const int ndir = 1 << d;
-  uint16_t* ldir = reinterpret_cast<uint16_t*>(smem);
-  KT* lkeys = reinterpret_cast<KT*>(smem + (((ndir + 1) * 2 + 15) & ~15));
+  const LdsLayout lay = lds_layout(d, a.max_bucket, sizeof(KT), STAGE_V ? sizeof(VT) : 0);
+  uint32_t* lpair = reinterpret_cast<uint32_t*>(smem + lay.pairs);
+  int* s_occ = reinterpret_cast<int*>(smem + lay.occ);
+  KT* lkeys = reinterpret_cast<KT*>(smem + lay.keys);
+  VT* lv = reinterpret_cast<VT*>(smem + lay.vals);
 
-  const uint32_t row0 = a.items[w];
-  const uint64_t bucket = a.keys[row0] >> m;
-  const uint32_t bend = a.bstart[bucket + 1];
+  const uint32_t row0 = items[w];
+  const uint64_t bucket = keys[row0] >> m;
+  const uint64_t bucket_base = m >= 64 ? 0ull : (bucket << m);
+  const uint32_t bend = bstart[bucket + 1];
   const uint32_t row1 = min(bend, row0 + (uint32_t)(ROWS * kBlock));
-  const uint64_t pc[2] = {a.pc_mask[0], a.pc_mask[1]};
 
-  uint64_t s[ROWS];
+  // Each row keeps only its low key bits: the high bits are the bucket's.
+  KT sl[ROWS];
   bool valid[ROWS];
   Acc<VT> acc[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     const uint32_t i = row0 + tid + r * kBlock;
     valid[r] = i < row1;
-    s[r] = valid[r] ? a.keys[i] : 0ull;
+    sl[r] = valid[r] ? (KT)(keys[i] & lowmask) : (KT)0;
   }
+
+  // Waves of a short chunk that hold no row skip the per-group work.
+  const bool wave_active = __any(valid[0]);
 
   // X = 0 group: the diagonal.
   if (a.diag_t1 > a.diag_t0) {
@@ -103,45 +134,85 @@ bucket_join_kernel(MatvecArgs<VT> a) {
     for (int r = 0; r < ROWS; ++r) {
       if (!valid[r]) continue;
       RT cr, ci;
-      group_coef<CT>(a.zmask, a.coef, a.diag_t0, a.diag_t1, s[r], cr, ci);
-      acc[r].fma(cr, ci, a.v[row0 + tid + r * kBlock]);
+      group_coef<CT>(zmask, coef, a.diag_t0, a.diag_t1, bucket_base | (uint64_t)sl[r], cr, ci);
+      acc[r].fma(cr, ci, v[row0 + tid + r * kBlock]);
     }
   }
 
   for (int c = 0; c < a.n_classes; ++c) {
-    const uint64_t q = bucket ^ (uint64_t)a.class_hi[c];
-    const uint32_t qs = a.bstart[q];
-    const uint32_t qe = a.bstart[q + 1];
+    const uint64_t q = bucket ^ (uint64_t)class_hi[c];
+    const uint32_t qs = bstart[q];
+    const uint32_t qe = bstart[q + 1];
     if (qs == qe) continue;  // empty target bucket: no group of the class can hit
     const int qn = (int)(qe - qs);
-    __syncthreads();  // previous class is done with the LDS tables
-    for (int i = tid; i < qn; i += kBlock) lkeys[i] = (KT)(a.keys[qs + i] & lowmask);
-    for (int e = tid; e < ndir; e += kBlock) ldir[e] = a.fdir[(q << d) + e];
-    if (tid == 0) ldir[ndir] = (uint16_t)qn;
+    __syncthreads();  // the previous class is done with the LDS tables
+    for (int i = tid; i < qn; i += kBlock) lkeys[i] = (KT)(keys[qs + i] & lowmask);
+    if (tid == 0) lkeys[qn] = (KT)0;
+    if (STAGE_V) {
+      for (int i = tid; i < qn; i += kBlock) lv[i] = v[qs + i];
+    }
+    // Sub-bucket ranges as (lo | hi << 16) and the largest occupancy.
+    const uint64_t fbase = q << d;
+    int occ = 0;
+    for (int e = tid; e < ndir; e += kBlock) {
+      const uint32_t lo = fdir[fbase + e];
+      const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdir[fbase + e + 1] : (uint32_t)qn;
+      lpair[e] = lo | (hi << 16);
+      occ = max(occ, (int)(hi - lo));
+    }
+#pragma unroll
+    for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
+    if ((tid & (kWave - 1)) == 0) s_occ[tid / kWave] = occ;
     __syncthreads();
-    const VT* __restrict__ vq = a.v + qs;
-    const int g0 = a.class_off[c], g1 = a.class_off[c + 1];
+    int maxocc = 0;
+#pragma unroll
+    for (int i = 0; i < kBlock / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
+    // Steps of the branch-free lower bound: ceil(log2(maxocc)).
+    const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
+    if (!wave_active) continue;  // the barriers above are still reached by every wave
+    const VT* __restrict__ vq = v + qs;
+    const int g0 = class_off[c], g1 = class_off[c + 1];
     for (int g = g0; g < g1; ++g) {
-      const uint64_t x = a.xmask[g];
-      const int t0 = a.grp_off[g], t1 = a.grp_off[g + 1];
+      const KT xl = (KT)(xmask[g] & lowmask);
+      KT tl[ROWS];
+      int base[ROWS], n[ROWS], n0[ROWS];
+      bool ok[ROWS];
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
-        if (!valid[r]) continue;
-        const uint64_t t = s[r] ^ x;
-        if (a.use_pc && !popcount_present(t, pc)) continue;
-        const KT tl = (KT)(t & lowmask);
-        const uint32_t e = (uint32_t)(t >> f) & dmask;
-        int lo = ldir[e];
-        int hi = ldir[e + 1];
-        const int end = hi;
-        while (lo < hi) {
-          const int mid = (lo + hi) >> 1;
-          if (lkeys[mid] < tl) lo = mid + 1; else hi = mid;
+        tl[r] = sl[r] ^ xl;
+        ok[r] = valid[r];
+        const uint32_t pr = lpair[(uint32_t)(tl[r] >> f)];
+        base[r] = (int)(pr & 0xFFFFu);
+        n0[r] = (int)(pr >> 16) - base[r];
+        n[r] = n0[r];
+      }
+      // Branch-free search for the last key <= target: after ceil(log2(n0))
+      // halvings base is that key, so one equality test decides membership.
+      for (int st = 0; st < steps; ++st) {
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          const int half = n[r] >> 1;
+          base[r] = lkeys[base[r] + half] <= tl[r] ? base[r] + half : base[r];
+          n[r] -= half;
         }
-        if (lo < end && lkeys[lo] == tl) {
+      }
+      int idx[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r)
+        idx[r] = (ok[r] && n0[r] > 0 && lkeys[base[r]] == tl[r]) ? base[r] : -1;
+      // Hits: gather first (all rows), then coefficients and accumulation.
+      VT vv[ROWS];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (idx[r] >= 0) vv[r] = STAGE_V ? lv[idx[r]] : vq[idx[r]];
+      }
+      const int t0 = grp_off[g], t1 = grp_off[g + 1];
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        if (idx[r] >= 0) {
           RT cr, ci;
-          group_coef<CT>(a.zmask, a.coef, t0, t1, s[r], cr, ci);
-          acc[r].fma(cr, ci, vq[lo]);
+          group_coef<CT>(zmask, coef, t0, t1, bucket_base | (uint64_t)sl[r], cr, ci);
+          acc[r].fma(cr, ci, vv[r]);
         }
       }
     }
@@ -149,50 +220,53 @@ bucket_join_kernel(MatvecArgs<VT> a) {
 
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    if (valid[r]) a.out[row0 + tid + r * kBlock] = acc[r].result();
+    if (valid[r]) out[row0 + tid + r * kBlock] = acc[r].result();
   }
 }
 
-template <typename VT, typename KT, int ROWS>
-int launch_rows(const MatvecArgs<VT>& args, size_t lds, hipStream_t s) {
-  auto kern = bucket_join_kernel<VT, KT, ROWS>;
+template <typename VT, typename KT, int ROWS, bool SV>
+int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds, hipStream_t s) {
+  auto kern = bucket_join_kernel<VT, KT, ROWS, SV>;
   if (lds > 65536) {
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  kern<<<args.n_items, kBlock, lds, s>>>(args);
+  kern<<<args.n_items, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.items, t.xmask,
+                                         t.grp_off, t.zmask, t.coef, t.class_hi, t.class_off, t.v,
+                                         t.out, args);
   SQDK_LAUNCHED("bucket_join_kernel");
   return SQDK_OK;
 }
 
-template <typename VT, typename KT>
-int launch_kt(const MatvecArgs<VT>& args, int rows, size_t lds, hipStream_t s) {
+template <typename VT, typename KT, bool SV>
+int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, size_t lds,
+              hipStream_t s) {
   switch (rows) {
-    case 4: return launch_rows<VT, KT, 4>(args, lds, s);
-    case 8: return launch_rows<VT, KT, 8>(args, lds, s);
-    case 16: return launch_rows<VT, KT, 16>(args, lds, s);
+    case 4: return launch_rows<VT, KT, 4, SV>(t, args, lds, s);
+    case 8: return launch_rows<VT, KT, 8, SV>(t, args, lds, s);
+    case 16: return launch_rows<VT, KT, 16, SV>(t, args, lds, s);
     default: return fail(SQDK_EINVAL, "rows_per_thread %d unsupported (4, 8 or 16)", rows);
   }
 }
 
 template <typename VT>
 int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const void* v,
-              void* out, int rows, int use_pc, int xcd, hipStream_t s, const int32_t* skip,
-              int32_t* counter) {
-  MatvecArgs<VT> args;
-  args.keys = sp->keys;
-  args.bstart = sp->bstart;
-  args.fdir = sp->fdir;
-  args.items = sp->items;
-  args.pc_mask = sp->pc_mask;
-  args.xmask = h->xmask;
-  args.grp_off = h->grp_off;
-  args.zmask = h->zmask;
-  args.coef = reinterpret_cast<const typename CoefT<VT>::type*>(coef);
-  args.class_hi = h->class_hi;
-  args.class_off = h->class_off;
-  args.v = reinterpret_cast<const VT*>(v);
-  args.out = reinterpret_cast<VT*>(out);
+              void* out, int rows, int xcd, int stage_v, hipStream_t s,
+              const int32_t* skip, int32_t* counter) {
+  MatvecTables<VT> t;
+  t.keys = sp->keys;
+  t.bstart = sp->bstart;
+  t.fdir = sp->fdir;
+  t.items = sp->items;
+  t.xmask = h->xmask;
+  t.grp_off = h->grp_off;
+  t.zmask = h->zmask;
+  t.coef = reinterpret_cast<const typename CoefT<VT>::type*>(coef);
+  t.class_hi = h->class_hi;
+  t.class_off = h->class_off;
+  t.v = reinterpret_cast<const VT*>(v);
+  t.out = reinterpret_cast<VT*>(out);
+  MatvecParams args;
   args.skip = skip;
   args.counter = counter;
   args.n_items = sp->n_items;
@@ -201,17 +275,19 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.n_classes = h->n_classes;
   args.diag_t0 = h->diag_t0;
   args.diag_t1 = h->diag_t1;
-  args.use_pc = use_pc;
   args.xcd_remap = xcd;
+  args.max_bucket = sp->max_bucket;
   const bool narrow = sp->bucket_bits <= 32;
-  const size_t kt = narrow ? 4 : 8;
-  const size_t lds = ((((size_t)(1 << sp->fine_bits) + 1) * 2 + 15) & ~(size_t)15) +
-                     kt * (size_t)sp->max_bucket;
-  if (lds > 160 * 1024)
+  const LdsLayout lay = lds_layout(sp->fine_bits, sp->max_bucket, narrow ? 4 : 8,
+                                   stage_v ? sizeof(VT) : 0);
+  if (lay.total > 160 * 1024)
     return fail(SQDK_ERANGE, "LDS tables of %zu bytes exceed 160 KiB (largest bucket %d keys)",
-                lds, sp->max_bucket);
-  return narrow ? launch_kt<VT, uint32_t>(args, rows, lds, s)
-                : launch_kt<VT, uint64_t>(args, rows, lds, s);
+                lay.total, sp->max_bucket);
+  if (stage_v)
+    return narrow ? launch_kt<VT, uint32_t, true>(t, args, rows, lay.total, s)
+                  : launch_kt<VT, uint64_t, true>(t, args, rows, lay.total, s);
+  return narrow ? launch_kt<VT, uint32_t, false>(t, args, rows, lay.total, s)
+                : launch_kt<VT, uint64_t, false>(t, args, rows, lay.total, s);
 }
 
 }  // namespace
@@ -222,7 +298,7 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
   SQDK_TRY(check_space_public(space));
   SQDK_REQUIRE(ham != nullptr, "ham is NULL");
   SQDK_REQUIRE(v && out, "NULL vector");
-  SQDK_REQUIRE(space->fdir && space->items && space->pc_mask, "space lacks fine directory or items");
+  SQDK_REQUIRE(space->fdir && space->items, "space lacks fine directory or items");
   SQDK_REQUIRE(ham->bucket_bits == space->bucket_bits,
                "ham classes were built for bucket_bits %d, space has %d", ham->bucket_bits,
                space->bucket_bits);
@@ -234,26 +310,26 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
   SQDK_REQUIRE(space->max_bucket >= 1 && space->max_bucket < 65536, "max_bucket %d out of range",
                space->max_bucket);
   SQDK_REQUIRE(space->bucket_bits <= 63, "bucket_bits %d too large", space->bucket_bits);
-  SQDK_REQUIRE(space->fine_bits >= 0 && space->fine_bits <= 14 &&
+  SQDK_REQUIRE(space->fine_bits >= 0 && space->fine_bits <= 15 &&
                    space->fine_bits <= space->bucket_bits,
                "fine_bits %d invalid", space->fine_bits);
   if (space->n_items == 0) return SQDK_OK;
-  int rows = 8, use_pc = 1, xcd = 1;
+  int rows = 4, xcd = 1, stage_v = 0;
   if (tune) {
     if (tune->rows_per_thread) rows = tune->rows_per_thread;
-    use_pc = tune->use_pc_filter;
     xcd = tune->xcd_remap;
+    stage_v = tune->stage_v;
   }
   SQDK_REQUIRE(rows * kBlock == space->chunk_rows,
                "rows_per_thread %d does not match the space's chunk of %d rows", rows,
                space->chunk_rows);
   if (precision == SQDK_C128) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, use_pc, xcd, s, skip, counter);
+    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, xcd, stage_v, s, skip, counter);
   }
   if (precision == SQDK_C64) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, use_pc, xcd, s, skip, counter);
+    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, xcd, stage_v, s, skip, counter);
   }
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }

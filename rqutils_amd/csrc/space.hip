@@ -184,7 +184,7 @@ int sqdk_space_dir(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t 
                    uint32_t* dir, void* stream) {
   SQDK_REQUIRE(keys && dir, "NULL pointer argument");
   SQDK_REQUIRE(n >= 1 && n < (1ll << 31), "n=%lld out of range", (long long)n);
-  SQDK_REQUIRE(shift >= 0 && shift <= num_qubits && num_qubits - shift <= 26,
+  SQDK_REQUIRE(shift >= 0 && shift <= num_qubits && shift <= 63 && num_qubits - shift <= 26,
                "shift %d invalid for %d qubits", shift, num_qubits);
   const int64_t np = 1ll << (num_qubits - shift);
   dir_kernel<<<(unsigned)ceil_div(np + 1, kBlock), kBlock, 0, as_stream(stream)>>>(
@@ -197,9 +197,10 @@ int sqdk_space_fine(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t
                     int32_t fine_bits, const uint32_t* bstart, uint16_t* fdir, void* stream) {
   SQDK_REQUIRE(keys && bstart && fdir, "NULL pointer argument");
   SQDK_REQUIRE(n >= 1 && n < (1ll << 31), "n=%lld out of range", (long long)n);
-  SQDK_REQUIRE(bucket_bits >= 0 && bucket_bits <= num_qubits && num_qubits - bucket_bits <= 26,
+  SQDK_REQUIRE(bucket_bits >= 0 && bucket_bits <= num_qubits && bucket_bits <= 63 &&
+                   num_qubits - bucket_bits <= 26,
                "bucket_bits %d invalid", bucket_bits);
-  SQDK_REQUIRE(fine_bits >= 0 && fine_bits <= bucket_bits && fine_bits <= 14,
+  SQDK_REQUIRE(fine_bits >= 0 && fine_bits <= bucket_bits && fine_bits <= 15,
                "fine_bits %d invalid", fine_bits);
   const int64_t total = (1ll << (num_qubits - bucket_bits)) << fine_bits;
   fine_kernel<<<(unsigned)ceil_div(total, kBlock), kBlock, 0, as_stream(stream)>>>(

@@ -26,7 +26,7 @@ __all__ = ["Subspace", "PauliOperator", "states_to_keys", "keys_to_states",
            "full_space_matmul", "DEFAULT_BUCKET_CAP", "DEFAULT_ROWS_PER_THREAD"]
 
 DEFAULT_BUCKET_CAP = 8192       # keys per bucket staged in LDS (32 KiB as uint32)
-DEFAULT_ROWS_PER_THREAD = 8     # rows per thread; a chunk is 256 x this
+DEFAULT_ROWS_PER_THREAD = 4     # rows per thread; a chunk is 256 x this
 MAX_DIR_BITS = 26               # at most 2^26 buckets
 
 
@@ -246,7 +246,7 @@ class PauliOperator:
         self.n_terms = int(tables.zmask.shape[0])
         self.n_classes = int(tables.class_hi.shape[0])
         self.diag_terms = tables.diag_terms
-        self.tune = _lib.SqdkTune(space.rows_per_thread, 1, 1, 0)
+        self.tune = _lib.SqdkTune(space.rows_per_thread, 0, 1, 0)
         self._cstruct = None
 
     @staticmethod

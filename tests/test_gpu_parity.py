@@ -74,13 +74,38 @@ def test_matvec_many_buckets_and_classes(gpu):
     # A tiny bucket cap forces many buckets, classes and multi-chunk rows.
     from rqutils_amd.operator import PauliOperator, Subspace
     p = small_problem(11, n=14, terms=150, nstates=6000)
-    for cap, rows in [(64, 4), (512, 8), (4096, 16)]:
+    for cap, rows, stage_v in [(64, 4, 0), (512, 8, 1), (4096, 16, 0), (300, 4, 1)]:
         sp = Subspace(p.keys, p.n, bucket_cap=cap, rows_per_thread=rows)
         op = PauliOperator((p.labels, p.coeffs), sp)
+        op.tune.stage_v = stage_v
         rng = np.random.default_rng(cap)
         v = _rand_vec(rng, p.keys.size)
         got = op.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
-        assert _rel(got, p.matvec(v)) < 1e-12, (cap, rows, sp.bucket_bits, sp.n_items)
+        assert _rel(got, p.matvec(v)) < 1e-12, (cap, rows, stage_v, sp.bucket_bits, sp.n_items)
+        v32 = v.astype(np.complex64)
+        got32 = op.matvec(torch.from_numpy(v32).cuda()).cpu().numpy()
+        assert _rel(got32.astype(np.complex128), p.matvec(v32.astype(np.complex128))) < 2e-6
+
+
+def test_matvec_wide_keys_and_fine_bits(gpu):
+    # bucket_bits > 32 (64-bit LDS keys) and explicit fine-directory sizes
+    from rqutils_amd.operator import PauliOperator, Subspace
+    rng = np.random.default_rng(17)
+    n = 48
+    labels = random_labels(rng, n, 200, 1, 3)
+    coeffs = rng.normal(size=200)
+    base = np.sort(rng.choice(2 ** 14, 400, replace=False)).astype(np.uint64)
+    # a structured set closed under some flips so that hits exist
+    keys = np.unique(np.concatenate([base, base ^ np.uint64(0b11), base << np.uint64(30)]))
+    p = OracleProblem(labels, coeffs, keys, n)
+    v = _rand_vec(rng, keys.size)
+    ref = p.matvec(v)
+    for fine in (0, 3, 9):
+        sp = Subspace(keys, n, fine_bits=fine)
+        assert sp.bucket_bits > 32
+        op = PauliOperator((labels, coeffs), sp)
+        got = op.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+        assert _rel(got, ref) < 1e-12, fine
 
 
 def test_diagonal_group0(gpu):
