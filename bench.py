@@ -88,7 +88,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solution block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-groups", type=int, default=3)
-    ap.add_argument("--rows-per-thread", type=int, default=4)
+    ap.add_argument("--rows-per-thread", type=int, default=0,
+                    help="fix rows per thread (0: autotune the launch)")
+    ap.add_argument("--group-split", type=int, default=1)
     ap.add_argument("--stage-v", type=int, default=0, help="stage v slices in LDS (0/1)")
     ap.add_argument("--bucket-cap", type=int, default=None, help="max keys per bucket")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -117,10 +119,17 @@ def main():
     w = W.make(args.workload)
     log(f"workload {w.name}: {w.n} states, {len(w.labels)} terms ({time.time() - t:.1f} s)")
     t = time.time()
-    space_kw = {"rows_per_thread": args.rows_per_thread}
-    if args.bucket_cap:
-        space_kw["bucket_cap"] = args.bucket_cap
-    prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, **space_kw)
+    if args.bucket_cap or args.rows_per_thread:
+        space_kw = {"rows_per_thread": args.rows_per_thread or 4}
+        if args.bucket_cap:
+            space_kw["bucket_cap"] = args.bucket_cap
+        prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev,
+                          group_split=args.group_split, **space_kw)
+    else:
+        prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, tune="auto")
+        log(f"autotune: {[t for t in prob.tuned.table]} -> cap={prob.tuned.bucket_cap} "
+            f"rows={prob.tuned.rows_per_thread} split={prob.tuned.group_split} "
+            f"({time.time() - t:.1f} s incl. tuning)")
     prob.op.tune.stage_v = args.stage_v
     torch.cuda.synchronize()
     sp, op = prob.space, prob.op
@@ -189,7 +198,8 @@ def main():
                    "terms": len(w.labels), "groups_searched": j_s,
                    "groups_pruned_odd": op.host.pruned_groups, "classes": op.n_classes,
                    "hits": hits, "bucket_bits": sp.bucket_bits, "fine_bits": sp.fine_bits,
-                   "rows_per_thread": args.rows_per_thread, "stage_v": args.stage_v,
+                   "rows_per_thread": sp.rows_per_thread, "group_split": op.tune.group_split,
+                   "stage_v": args.stage_v, "autotuned": prob.tuned is not None,
                    "parallelism": "single GPU"},
         "matvecs_per_s": round(1e3 / ms, 2),
         "matvec_c128_ms": round(ms128, 4),
@@ -199,7 +209,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(os.path.join(
                          ROOT, "profiles", f"bucket_join_kernel_{w.name}_traffic.md")),
-                     "kernel": f"bucket_join_kernel<float2,uint32_t,{args.rows_per_thread},{bool(args.stage_v)}>",
+                     "kernel": f"bucket_join_kernel<float2,uint32_t,{sp.rows_per_thread},{bool(args.stage_v)}>",
                      "bytes_per_launch": b_mv},
         "wall_s_timed": round(wall, 3),
     }

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 2
+#define SQDK_ABI_VERSION 3
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -93,7 +93,8 @@ typedef struct sqdk_ham {
 /* Launch tuning of the bucket-join matvec (0 = library default). */
 typedef struct sqdk_tune {
   int32_t rows_per_thread; /* 4, 8 or 16; must match chunk_rows / 256 */
-  int32_t reserved;        /* must be 0 */
+  int32_t group_split;     /* workgroups sharing a row chunk, groups dealt round
+                              robin (<= 1: one); needs sqdk_matvec_workspace bytes */
   int32_t xcd_remap;       /* map consecutive items onto one XCD */
   int32_t stage_v;         /* stage each target bucket's slice of v in LDS */
 } sqdk_tune;
@@ -149,13 +150,18 @@ int sqdk_diagonal(const uint64_t* zmask, const double* coef, int32_t t0,
 
 /* ---- matrix-free matvec ------------------------------------------------ */
 
+/* Workspace bytes of sqdk_matvec for this tuning (0 unless group_split > 1). */
+int sqdk_matvec_workspace(const sqdk_space* space, const sqdk_tune* tune,
+                          int32_t precision, size_t* bytes);
+
 /* out = H v over the subspace: per row, the X = 0 group's coefficient times
  * v[row] plus, per off-diagonal group, one XOR and one search of the target
  * in the LDS-staged target bucket. Replaces apply_h / apply_h_x_cached and
  * siblings (rqutils/sqd.py:590-684). precision SQDK_C128: double storage and
  * FP64 sums; SQDK_C64: float storage, FP32 two-sum accumulation. */
 int sqdk_matvec(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
-                const void* v, void* out, const sqdk_tune* tune, void* stream);
+                const void* v, void* out, const sqdk_tune* tune, void* work,
+                size_t work_bytes, void* stream);
 
 /* ---- vector kernels (LOBPCG building blocks) --------------------------- */
 
@@ -221,8 +227,9 @@ typedef struct sqdk_op {
   void* ctx;
 } sqdk_op;
 
-/* Device workspace (bytes) for a solve of n elements. */
-int sqdk_locg_workspace(int64_t n, const sqdk_locg_opts* opts, size_t* bytes);
+/* Device workspace (bytes) for a solve of n elements with this operator. */
+int sqdk_locg_workspace(const sqdk_op* op, int64_t n, const sqdk_locg_opts* opts,
+                        size_t* bytes);
 
 /* Single-vector LOBPCG for the lowest eigenpair of a Hermitian operator.
  * Replaces ground_locg / _ground_locg_callable (rqutils/ground_locg.py:128-377).

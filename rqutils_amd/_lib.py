@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -44,7 +44,7 @@ class SqdkHam(C.Structure):
 
 
 class SqdkTune(C.Structure):
-    _fields_ = [("rows_per_thread", C.c_int32), ("reserved", C.c_int32),
+    _fields_ = [("rows_per_thread", C.c_int32), ("group_split", C.c_int32),
                 ("xcd_remap", C.c_int32), ("stage_v", C.c_int32)]
 
 
@@ -88,15 +88,18 @@ SIGNATURES = {
     "sqdk_xsource": (C.c_int, [C.POINTER(SqdkSpace), C.c_uint64, C.c_void_p, C.c_void_p]),
     "sqdk_diagonal": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
                                 C.c_int64, C.c_void_p, C.c_void_p]),
+    "sqdk_matvec_workspace": (C.c_int, [C.POINTER(SqdkSpace), C.POINTER(SqdkTune), C.c_int32,
+                                        C.POINTER(C.c_size_t)]),
     "sqdk_matvec": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace), C.c_int32,
-                              C.c_void_p, C.c_void_p, C.POINTER(SqdkTune), C.c_void_p]),
+                              C.c_void_p, C.c_void_p, C.POINTER(SqdkTune), C.c_void_p,
+                              C.c_size_t, C.c_void_p]),
     "sqdk_dots": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
                             C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_size_t,
                             C.c_void_p]),
     "sqdk_dots_workspace": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]),
     "sqdk_convert": (C.c_int, [C.c_int64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                C.c_void_p]),
-    "sqdk_locg_workspace": (C.c_int, [C.c_int64, C.POINTER(SqdkLocgOpts),
+    "sqdk_locg_workspace": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.POINTER(SqdkLocgOpts),
                                       C.POINTER(C.c_size_t)]),
     "sqdk_locg_solve": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.c_void_p,
                                   C.POINTER(SqdkLocgOpts), C.c_void_p, C.c_size_t,

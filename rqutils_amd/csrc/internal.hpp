@@ -14,8 +14,11 @@ int check_space_public(const sqdk_space* sp);
 // the launch does nothing (used for iterations after convergence inside a
 // chunk enqueued ahead). `counter`, if set, counts the launches that ran.
 int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
-                const void* v, void* out, const sqdk_tune* tune, hipStream_t s,
-                const int32_t* skip = nullptr, int32_t* counter = nullptr);
+                const void* v, void* out, const sqdk_tune* tune, void* work, size_t work_bytes,
+                hipStream_t s, const int32_t* skip = nullptr, int32_t* counter = nullptr);
+
+// Workspace bytes the matvec needs for this tuning (partial sums of a group split).
+size_t matvec_work_bytes(const sqdk_space* space, const sqdk_tune* tune, int32_t precision);
 
 constexpr int kMaxPairs = 16;
 constexpr int kDotBlocks = 512;   // partial sums per pair (fixed: deterministic)

@@ -470,6 +470,8 @@ struct Solver {
   int32_t* F;
   double* partials;
   char* vec[7];
+  void* mv_work = nullptr;
+  size_t mv_bytes = 0;
   ScalarParams prm;
   int matvecs = 0;
 
@@ -480,8 +482,8 @@ struct Solver {
       if (rc != 0) return fail(SQDK_ECALLBACK, "matvec callback returned %d", rc);
       return SQDK_OK;
     }
-    return matvec_impl(op->ham, op->space, prec, in, out, op->tune, s, skip ? F + kConv : nullptr,
-                       F + kMatvecs);
+    return matvec_impl(op->ham, op->space, prec, in, out, op->tune, mv_work, mv_bytes, s,
+                       skip ? F + kConv : nullptr, F + kMatvecs);
   }
   int dots(std::initializer_list<std::pair<const void*, const void*>> pairs, bool skip,
            int slot = kD) {
@@ -684,13 +686,16 @@ using namespace sqdk;
 
 extern "C" {
 
-int sqdk_locg_workspace(int64_t n, const sqdk_locg_opts* opts, size_t* bytes) {
+int sqdk_locg_workspace(const sqdk_op* op, int64_t n, const sqdk_locg_opts* opts,
+                        size_t* bytes) {
   SQDK_REQUIRE(opts && bytes, "NULL pointer argument");
   SQDK_REQUIRE(n >= 1, "empty problem");
   SQDK_REQUIRE(opts->precision == SQDK_C128 || opts->precision == SQDK_C64,
                "unknown precision %d", opts->precision);
   const size_t vb = align256((size_t)n * elem_bytes(opts->precision));
-  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) + 7 * vb;
+  size_t mv = 0;
+  if (op && !op->fn && op->space) mv = align256(matvec_work_bytes(op->space, op->tune, opts->precision));
+  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) + 7 * vb + mv;
   return SQDK_OK;
 }
 
@@ -708,7 +713,7 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
                  (long long)op->space->n, (long long)n);
   }
   size_t need = 0;
-  SQDK_TRY(sqdk_locg_workspace(n, opts, &need));
+  SQDK_TRY(sqdk_locg_workspace(op, n, opts, &need));
   SQDK_REQUIRE(work_bytes >= need, "workspace of %zu bytes, %zu needed", work_bytes, need);
   sqdk_locg_opts o = *opts;
   if (o.check_every < 1) o.check_every = 1;
@@ -728,6 +733,10 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   w += align256(kPartialBytes);
   const size_t vb = align256((size_t)n * elem_bytes(o.precision));
   for (int i = 0; i < 7; ++i) sv.vec[i] = w + i * vb;
+  if (!op->fn) {
+    sv.mv_bytes = matvec_work_bytes(op->space, op->tune, o.precision);
+    sv.mv_work = sv.mv_bytes ? w + 7 * vb : nullptr;
+  }
   sv.prm.tol = o.tol;
   sv.prm.nlen = (double)(o.n_for_tol > 0 ? o.n_for_tol : n);
   sv.prm.criterion = o.criterion;

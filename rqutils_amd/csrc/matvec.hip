@@ -37,6 +37,9 @@ struct MatvecParams {
   int32_t diag_t0, diag_t1;
   int32_t xcd_remap;
   int32_t max_bucket;
+  int32_t split;      // workgroups sharing one row chunk (groups dealt round robin)
+  int64_t n;          // rows, for the partial-sum layout
+  void* partial;      // (split - 1) x n partial sums when split > 1
 };
 
 template <typename VT>
@@ -92,7 +95,9 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   using RT = typename CoefT<VT>::real;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
-  const int w = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int logical = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
+  const int w = logical / a.split;   // row chunk
+  const int gs = logical % a.split;  // this workgroup's share of the groups
   if (w >= a.n_items) return;
   if (a.skip && a.skip[0] != 0) return;
   if (a.counter && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.counter, 1);
@@ -128,8 +133,8 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   // Waves of a short chunk that hold no row skip the per-group work.
   const bool wave_active = __any(valid[0]);
 
-  // X = 0 group: the diagonal.
-  if (a.diag_t1 > a.diag_t0) {
+  // X = 0 group: the diagonal (first share only).
+  if (gs == 0 && a.diag_t1 > a.diag_t0) {
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       if (!valid[r]) continue;
@@ -140,6 +145,10 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   }
 
   for (int c = 0; c < a.n_classes; ++c) {
+    const int g0 = class_off[c], g1 = class_off[c + 1];
+    // First group of this class dealt to share gs (groups go round robin).
+    const int gfirst = g0 + ((gs - g0 % a.split) + a.split) % a.split;
+    if (gfirst >= g1) continue;
     const uint64_t q = bucket ^ (uint64_t)class_hi[c];
     const uint32_t qs = bstart[q];
     const uint32_t qe = bstart[q + 1];
@@ -171,8 +180,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
     if (!wave_active) continue;  // the barriers above are still reached by every wave
     const VT* __restrict__ vq = v + qs;
-    const int g0 = class_off[c], g1 = class_off[c + 1];
-    for (int g = g0; g < g1; ++g) {
+    for (int g = gfirst; g < g1; g += a.split) {
       const KT xl = (KT)(xmask[g] & lowmask);
       KT tl[ROWS];
       int base[ROWS], n[ROWS], n0[ROWS];
@@ -218,9 +226,27 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     }
   }
 
+  // Shares other than the first write partial sums for the reduction pass.
+  VT* __restrict__ dst = gs == 0 ? out : reinterpret_cast<VT*>(a.partial) + (int64_t)(gs - 1) * a.n;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    if (valid[r]) out[row0 + tid + r * kBlock] = acc[r].result();
+    if (valid[r]) dst[row0 + tid + r * kBlock] = acc[r].result();
+  }
+}
+
+// out[i] += sum of the partial sums of shares 1..split-1, in share order.
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+split_reduce_kernel(int64_t n, int nparts, const VT* __restrict__ partial, VT* __restrict__ out,
+                    const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    Acc<VT> acc;
+    const VT o = out[i];
+    acc.fma(1, 0, o);
+    for (int k = 0; k < nparts; ++k) acc.fma(1, 0, partial[(int64_t)k * n + i]);
+    out[i] = acc.result();
   }
 }
 
@@ -231,10 +257,19 @@ int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds,
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  kern<<<args.n_items, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.items, t.xmask,
-                                         t.grp_off, t.zmask, t.coef, t.class_hi, t.class_off, t.v,
-                                         t.out, args);
+  const int64_t grid = (int64_t)args.n_items * args.split;
+  if (grid >= (1ll << 31)) return fail(SQDK_ERANGE, "grid of %lld workgroups too large", (long long)grid);
+  kern<<<(unsigned)grid, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.items, t.xmask, t.grp_off,
+                                           t.zmask, t.coef, t.class_hi, t.class_off, t.v, t.out,
+                                           args);
   SQDK_LAUNCHED("bucket_join_kernel");
+  if (args.split > 1) {
+    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(args.n, kBlock), 4096));
+    split_reduce_kernel<VT><<<g, kBlock, 0, s>>>(args.n, args.split - 1,
+                                                 reinterpret_cast<const VT*>(args.partial), t.out,
+                                                 args.skip);
+    SQDK_LAUNCHED("split_reduce_kernel");
+  }
   return SQDK_OK;
 }
 
@@ -251,8 +286,8 @@ int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, siz
 
 template <typename VT>
 int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const void* v,
-              void* out, int rows, int xcd, int stage_v, hipStream_t s,
-              const int32_t* skip, int32_t* counter) {
+              void* out, int rows, int xcd, int stage_v, int split, void* work,
+              hipStream_t s, const int32_t* skip, int32_t* counter) {
   MatvecTables<VT> t;
   t.keys = sp->keys;
   t.bstart = sp->bstart;
@@ -277,6 +312,9 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.diag_t1 = h->diag_t1;
   args.xcd_remap = xcd;
   args.max_bucket = sp->max_bucket;
+  args.split = split;
+  args.n = sp->n;
+  args.partial = work;
   const bool narrow = sp->bucket_bits <= 32;
   const LdsLayout lay = lds_layout(sp->fine_bits, sp->max_bucket, narrow ? 4 : 8,
                                    stage_v ? sizeof(VT) : 0);
@@ -292,9 +330,15 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
 
 }  // namespace
 
+size_t matvec_work_bytes(const sqdk_space* space, const sqdk_tune* tune, int32_t precision) {
+  const int split = (tune && tune->group_split > 1) ? tune->group_split : 1;
+  if (split <= 1 || !space) return 0;
+  return (size_t)(split - 1) * (size_t)space->n * (precision == SQDK_C128 ? 16 : 8);
+}
+
 int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
-                const void* v, void* out, const sqdk_tune* tune, hipStream_t s,
-                const int32_t* skip, int32_t* counter) {
+                const void* v, void* out, const sqdk_tune* tune, void* work, size_t work_bytes,
+                hipStream_t s, const int32_t* skip, int32_t* counter) {
   SQDK_TRY(check_space_public(space));
   SQDK_REQUIRE(ham != nullptr, "ham is NULL");
   SQDK_REQUIRE(v && out, "NULL vector");
@@ -314,29 +358,47 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                    space->fine_bits <= space->bucket_bits,
                "fine_bits %d invalid", space->fine_bits);
   if (space->n_items == 0) return SQDK_OK;
-  int rows = 4, xcd = 1, stage_v = 0;
+  int rows = 4, xcd = 1, stage_v = 0, split = 1;
   if (tune) {
     if (tune->rows_per_thread) rows = tune->rows_per_thread;
+    if (tune->group_split > 1) split = tune->group_split;
     xcd = tune->xcd_remap;
     stage_v = tune->stage_v;
   }
+  SQDK_REQUIRE(split <= 1024, "group_split %d too large", split);
   SQDK_REQUIRE(rows * kBlock == space->chunk_rows,
                "rows_per_thread %d does not match the space's chunk of %d rows", rows,
                space->chunk_rows);
+  const size_t need = matvec_work_bytes(space, tune, precision);
+  SQDK_REQUIRE(need == 0 || (work && work_bytes >= need),
+               "matvec workspace of %zu bytes, %zu needed for group_split %d", work_bytes, need,
+               split);
   if (precision == SQDK_C128) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, xcd, stage_v, s, skip, counter);
+    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, xcd, stage_v, split, work, s,
+                              skip, counter);
   }
   if (precision == SQDK_C64) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, xcd, stage_v, s, skip, counter);
+    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, xcd, stage_v, split, work, s,
+                             skip, counter);
   }
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }
 
 }  // namespace sqdk
 
+extern "C" int sqdk_matvec_workspace(const sqdk_space* space, const sqdk_tune* tune,
+                                     int32_t precision, size_t* bytes) {
+  SQDK_REQUIRE(space && bytes, "NULL pointer argument");
+  SQDK_REQUIRE(precision == SQDK_C128 || precision == SQDK_C64, "unknown precision %d", precision);
+  *bytes = sqdk::matvec_work_bytes(space, tune, precision);
+  return SQDK_OK;
+}
+
 extern "C" int sqdk_matvec(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
-                           const void* v, void* out, const sqdk_tune* tune, void* stream) {
-  return sqdk::matvec_impl(ham, space, precision, v, out, tune, sqdk::as_stream(stream));
+                           const void* v, void* out, const sqdk_tune* tune, void* work,
+                           size_t work_bytes, void* stream) {
+  return sqdk::matvec_impl(ham, space, precision, v, out, tune, work, work_bytes,
+                           sqdk::as_stream(stream));
 }

@@ -131,9 +131,6 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     opts.check_every = int(check_every)
     opts.refresh_every = int(refresh_every)
     opts.use_graph = 0
-    nbytes = C.c_size_t(0)
-    _lib.check(L.sqdk_locg_workspace(n, C.byref(opts), C.byref(nbytes)), "sqdk_locg_workspace")
-    work = torch.empty(nbytes.value, dtype=torch.uint8, device=x.device)
     cop = _lib.SqdkOp()
     keep = []
     cb = None
@@ -145,13 +142,17 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
         cop.ham = C.pointer(h)
         cop.space = C.pointer(s)
         cop.tune = C.pointer(t)
-    elif callable(op):
+    elif not callable(op):
+        raise ValueError("op must be a PauliOperator or a callable")
+    nbytes = C.c_size_t(0)
+    _lib.check(L.sqdk_locg_workspace(C.byref(cop), n, C.byref(opts), C.byref(nbytes)),
+               "sqdk_locg_workspace")
+    work = torch.empty(nbytes.value, dtype=torch.uint8, device=x.device)
+    if not isinstance(op, PauliOperator):
         cb = _Callback(op, n, _cdtype(prec), real,
                        [(work.data_ptr(), work.numel(), work),
                         (x.data_ptr(), x.numel() * x.element_size(), x)])
         cop.fn = cb.cfunc
-    else:
-        raise ValueError("op must be a PauliOperator or a callable")
     res = _lib.SqdkLocgResult()
     with torch.cuda.device(x.device):
         rc = L.sqdk_locg_solve(C.byref(cop), n, x.data_ptr(), C.byref(opts), work.data_ptr(),

@@ -246,8 +246,33 @@ class PauliOperator:
         self.n_terms = int(tables.zmask.shape[0])
         self.n_classes = int(tables.class_hi.shape[0])
         self.diag_terms = tables.diag_terms
-        self.tune = _lib.SqdkTune(space.rows_per_thread, 0, 1, 0)
+        self.tune = _lib.SqdkTune(space.rows_per_thread, 1, 1, 0)
         self._cstruct = None
+        self._work = {}
+
+    def set_tune(self, group_split: int = 1, stage_v: int = 0, xcd_remap: int = 1):
+        """Launch options of the matvec (rows per thread are fixed by the subspace)."""
+        if not 1 <= int(group_split) <= 1024:
+            raise ValueError(f"group_split={group_split} outside 1..1024")
+        self.tune.group_split = int(group_split)
+        self.tune.stage_v = int(stage_v)
+        self.tune.xcd_remap = int(xcd_remap)
+        self._work.clear()
+
+    def work_bytes(self, precision: int) -> int:
+        nbytes = C.c_size_t(0)
+        s = self.space.c_struct()
+        _lib.check(_lib.lib().sqdk_matvec_workspace(C.byref(s), C.byref(self.tune), precision,
+                                                    C.byref(nbytes)), "sqdk_matvec_workspace")
+        return nbytes.value
+
+    def _workspace(self, precision: int):
+        key = (precision, self.tune.group_split)
+        if key not in self._work:
+            nb = self.work_bytes(precision)
+            self._work[key] = torch.empty(nb, dtype=torch.uint8, device=self.space.device) \
+                if nb else None
+        return self._work[key]
 
     @staticmethod
     def _tables(ham: PauliSumXZ, space: Subspace, prune: bool) -> _HostTables:
@@ -335,8 +360,11 @@ class PauliOperator:
         if out is None:
             out = torch.empty_like(v)
         h, s, t = self.structs()
+        work = self._workspace(prec)
         _lib.check(_lib.lib().sqdk_matvec(C.byref(h), C.byref(s), prec, v.data_ptr(),
                                           out.data_ptr(), C.byref(t),
+                                          0 if work is None else work.data_ptr(),
+                                          0 if work is None else work.numel(),
                                           _lib.stream_ptr(self.space.device)), "sqdk_matvec")
         return out
 

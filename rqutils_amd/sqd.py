@@ -44,14 +44,22 @@ class SQDProblem:
     """
 
     def __init__(self, hamiltonian, keys, num_qubits: Optional[int] = None, *, device=None,
-                 **space_kw):
+                 tune=None, group_split: int = 1, **space_kw):
         if not isinstance(hamiltonian, PauliSumXZ):
             hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True,
                                                    add_padding=True)
         self.hamiltonian = hamiltonian
         nq = hamiltonian.num_qubits if num_qubits is None else num_qubits
+        self.tuned = None
+        if tune == "auto" or isinstance(tune, str):
+            from .autotune import autotune
+            self.tuned = autotune(hamiltonian, keys, nq, device=device,
+                                  precision="c64" if tune == "auto" else tune)
+            space_kw = {**space_kw, **self.tuned.space_kw()}
+            group_split = self.tuned.group_split
         self.space = Subspace(keys, nq, device=device, **space_kw)
         self.op = PauliOperator(hamiltonian, self.space)
+        self.op.set_tune(group_split=group_split)
 
     @property
     def n(self) -> int:

@@ -44,9 +44,25 @@ class OracleProblem:
         return so.apply_h_x_cached(v, self.xsources, self.z, self.c, self.packed)
 
     def dense(self):
+        """Projected matrix from dense Kronecker products (independent; n <= 11)."""
+        if self.n > 11:
+            raise ValueError("dense Kronecker construction is limited to 11 qubits here")
         h = dense.dense_hamiltonian(self.labels, np.real(self.coeffs))
         idx = self.keys.astype(np.int64)
         return dense.projected(h, idx)
+
+    def csr(self):
+        """Projected matrix from the oracle's sort-join construction."""
+        import scipy.sparse as sps
+        rows, cols, vals = so.hproj_coo(self.x, self.z, self.c, self.packed)
+        n = self.keys.size
+        return sps.csr_matrix((vals, (rows, cols)), shape=(n, n))
+
+    def ground(self):
+        """Lowest eigenvalue of the projected matrix (sparse Lanczos, FP64)."""
+        import scipy.sparse.linalg as spl
+        h = self.csr().astype(np.complex128)
+        return float(spl.eigsh(h, k=1, which="SA", tol=1e-13)[0][0])
 
 
 def small_problem(seed=0, n=10, terms=60, nstates=300, full=False):
