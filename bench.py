@@ -208,7 +208,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(os.path.join(
-                         ROOT, "profiles", f"bucket_join_kernel_{w.name}_traffic.md")),
+                         ROOT, "profiles", f"bucket_join_kernel_{w.name}.md")),
                      "kernel": f"bucket_join_kernel<float2,uint32_t,{sp.rows_per_thread},{bool(args.stage_v)}>",
                      "bytes_per_launch": b_mv},
         "wall_s_timed": round(wall, 3),
