@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 3
+#define SQDK_ABI_VERSION 4
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -89,6 +89,49 @@ typedef struct sqdk_ham {
   int32_t bucket_bits;      /* must equal the space's */
   int32_t reserved;
 } sqdk_ham;
+
+/* One instance of a batch (see sqdk_batch): where its tables start in the
+ * concatenated arrays. All row, group and term indices stored in the tables
+ * are global (they already include the instance's offsets). */
+typedef struct sqdk_inst {
+  int64_t bstart_base;      /* first entry of its bucket directory in bstart */
+  int64_t fdir_base;        /* first entry of its fine directory in fdir */
+  int32_t class_base;       /* first of its classes in class_hi / class_off */
+  int32_t n_classes;
+  int32_t diag_t0, diag_t1; /* term range of its X = 0 group */
+  int32_t bucket_bits;
+  int32_t fine_bits;
+} sqdk_inst;
+
+/* Many independent instances (each its own Hamiltonian and subspace) laid
+ * end to end, for one persistent matvec launch and a batched LOBPCG. Rows of
+ * instance i are row_base[i] <= r < row_base[i+1]; vectors are the
+ * concatenation of the instances' vectors. No counterpart in the reference:
+ * it solves one instance per call. */
+typedef struct sqdk_batch {
+  const uint64_t* keys;      /* [n] keys, strictly ascending within each instance */
+  const uint32_t* bstart;    /* concatenated bucket directories (global rows) */
+  const uint16_t* fdir;      /* concatenated fine directories */
+  const uint32_t* items;     /* [n_items] first (global) row of each chunk */
+  const uint16_t* item_inst; /* [n_items] instance of each chunk */
+  const int32_t* item_base;  /* [n_inst+1] chunks of instance i: item_base[i].. */
+  const int64_t* row_base;   /* [n_inst+1] rows of instance i: row_base[i].. */
+  const uint64_t* xmask;     /* concatenated group tables (global indices) */
+  const int32_t* grp_off;
+  const uint64_t* zmask;
+  const double* coef;
+  const float* coef32;
+  const uint32_t* class_hi;
+  const int32_t* class_off;
+  const sqdk_inst* inst;     /* [n_inst] descriptors (device memory) */
+  int64_t n;                 /* total rows */
+  int32_t n_inst;
+  int32_t n_items;
+  int32_t chunk_rows;        /* rows per chunk: 256 x rows_per_thread */
+  int32_t max_bucket;        /* largest bucket over the batch */
+  int32_t max_fine_bits;
+  int32_t max_bucket_bits;   /* <= 32 selects 32-bit LDS keys */
+} sqdk_batch;
 
 /* Launch tuning of the bucket-join matvec (0 = library default). */
 typedef struct sqdk_tune {
@@ -162,6 +205,15 @@ int sqdk_matvec_workspace(const sqdk_space* space, const sqdk_tune* tune,
 int sqdk_matvec(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                 const void* v, void* out, const sqdk_tune* tune, void* work,
                 size_t work_bytes, void* stream);
+
+/* Workspace bytes of sqdk_batch_matvec for this tuning. */
+int sqdk_batch_matvec_workspace(const sqdk_batch* batch, const sqdk_tune* tune,
+                                int32_t precision, size_t* bytes);
+
+/* out = H_i v_i for every instance i of the batch, in one launch. */
+int sqdk_batch_matvec(const sqdk_batch* batch, int32_t precision, const void* v,
+                      void* out, const sqdk_tune* tune, void* work, size_t work_bytes,
+                      void* stream);
 
 /* ---- vector kernels (LOBPCG building blocks) --------------------------- */
 
@@ -238,6 +290,21 @@ int sqdk_locg_workspace(const sqdk_op* op, int64_t n, const sqdk_locg_opts* opts
 int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x,
                     const sqdk_locg_opts* opts, void* work, size_t work_bytes,
                     sqdk_locg_result* result, void* stream);
+
+/* Device workspace (bytes) of a batched solve. */
+int sqdk_batch_locg_workspace(const sqdk_batch* batch, const sqdk_tune* tune,
+                              const sqdk_locg_opts* opts, size_t* bytes);
+
+/* Fused single-vector LOBPCG on every instance of the batch at once (mode
+ * must be SQDK_LOCG_FUSED). x holds the concatenated start vectors on entry
+ * and the normalised eigenvectors on return; results[i] (host array of
+ * n_inst entries) receives instance i's eigenvalue, iterations and residual.
+ * opts->use_graph captures refresh_every iterations as one HIP graph (on a
+ * private stream, after synchronising the given one). No counterpart in the
+ * reference, which solves one instance per ground_locg call. */
+int sqdk_batch_locg_solve(const sqdk_batch* batch, const sqdk_tune* tune, void* x,
+                          const sqdk_locg_opts* opts, void* work, size_t work_bytes,
+                          sqdk_locg_result* results, void* stream);
 
 /* Lowest eigenpair of one Hermitian 3x3 (Kopp 2008, as ground_locg.py:438-471)
  * or 2x2 (ground_locg.py:424-435) matrix, evaluated by the device code path:

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -40,6 +40,25 @@ class SqdkHam(C.Structure):
         ("n_groups", C.c_int32), ("n_terms", C.c_int32), ("n_classes", C.c_int32),
         ("diag_t0", C.c_int32), ("diag_t1", C.c_int32), ("num_qubits", C.c_int32),
         ("bucket_bits", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
+class SqdkInst(C.Structure):
+    _fields_ = [("bstart_base", C.c_int64), ("fdir_base", C.c_int64), ("class_base", C.c_int32),
+                ("n_classes", C.c_int32), ("diag_t0", C.c_int32), ("diag_t1", C.c_int32),
+                ("bucket_bits", C.c_int32), ("fine_bits", C.c_int32)]
+
+
+class SqdkBatch(C.Structure):
+    _fields_ = [
+        ("keys", C.c_void_p), ("bstart", C.c_void_p), ("fdir", C.c_void_p), ("items", C.c_void_p),
+        ("item_inst", C.c_void_p), ("item_base", C.c_void_p), ("row_base", C.c_void_p),
+        ("xmask", C.c_void_p), ("grp_off", C.c_void_p), ("zmask", C.c_void_p),
+        ("coef", C.c_void_p), ("coef32", C.c_void_p), ("class_hi", C.c_void_p),
+        ("class_off", C.c_void_p), ("inst", C.c_void_p),
+        ("n", C.c_int64), ("n_inst", C.c_int32), ("n_items", C.c_int32),
+        ("chunk_rows", C.c_int32), ("max_bucket", C.c_int32), ("max_fine_bits", C.c_int32),
+        ("max_bucket_bits", C.c_int32),
     ]
 
 
@@ -93,6 +112,15 @@ SIGNATURES = {
     "sqdk_matvec": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace), C.c_int32,
                               C.c_void_p, C.c_void_p, C.POINTER(SqdkTune), C.c_void_p,
                               C.c_size_t, C.c_void_p]),
+    "sqdk_batch_matvec_workspace": (C.c_int, [C.POINTER(SqdkBatch), C.POINTER(SqdkTune),
+                                              C.c_int32, C.POINTER(C.c_size_t)]),
+    "sqdk_batch_matvec": (C.c_int, [C.POINTER(SqdkBatch), C.c_int32, C.c_void_p, C.c_void_p,
+                                    C.POINTER(SqdkTune), C.c_void_p, C.c_size_t, C.c_void_p]),
+    "sqdk_batch_locg_workspace": (C.c_int, [C.POINTER(SqdkBatch), C.POINTER(SqdkTune),
+                                            C.POINTER(SqdkLocgOpts), C.POINTER(C.c_size_t)]),
+    "sqdk_batch_locg_solve": (C.c_int, [C.POINTER(SqdkBatch), C.POINTER(SqdkTune), C.c_void_p,
+                                        C.POINTER(SqdkLocgOpts), C.c_void_p, C.c_size_t,
+                                        C.POINTER(SqdkLocgResult), C.c_void_p]),
     "sqdk_dots": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
                             C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_size_t,
                             C.c_void_p]),

@@ -1,0 +1,268 @@
+"""Many independent SQD instances solved in one persistent launch per step.
+
+``SQDBatch`` builds each instance's subspace and Pauli tables (the same
+``Subspace`` / ``PauliOperator`` as a single solve, with one common rows-per-
+thread and bucket cap), then lays them end to end on the device with global
+row, group and term indices plus a descriptor per instance. One
+``sqdk_batch_matvec`` launch then covers every instance, and
+``sqdk_batch_locg_solve`` runs the fused FP32/FP64 LOBPCG on all of them at
+once with HIP-graph-captured iterations; converged instances drop out of the
+work while the others continue.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .ground_locg import LocgInfo, _CRITERIA, _PRECISIONS
+from .operator import DEFAULT_BUCKET_CAP, PauliOperator, Subspace
+
+__all__ = ["SQDBatch", "autotune_batch"]
+
+
+def _i64(a):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(torch.int64)
+
+
+@dataclass
+class BatchSolve:
+    vectors: torch.Tensor          # concatenated eigenvectors (solver precision)
+    infos: list                    # LocgInfo per instance
+    seconds: float = 0.0
+
+
+class SQDBatch:
+    """A batch of (hamiltonian, keys, num_qubits) instances on one GPU."""
+
+    def __init__(self, instances, *, device=None, bucket_cap: int = DEFAULT_BUCKET_CAP,
+                 rows_per_thread: int = 4, group_split: int = 1):
+        dev = torch.device(device if device is not None else "cuda")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
+        if not instances:
+            raise ValueError("empty batch")
+        if len(instances) > 65535:
+            raise ValueError("at most 65535 instances per batch")
+        self.spaces, self.ops = [], []
+        for ham, keys, nq in instances:
+            sp = Subspace(keys, nq, device=dev, bucket_cap=bucket_cap,
+                          rows_per_thread=rows_per_thread)
+            self.spaces.append(sp)
+            self.ops.append(PauliOperator(ham, sp))
+        self.rows_per_thread = rows_per_thread
+        self._concat()
+        self.tune = _lib.SqdkTune(rows_per_thread, int(group_split), 1, 0)
+        self._work = {}
+
+    def _concat(self):
+        dev = self.device
+        keys, bstart, fdir, items, item_inst = [], [], [], [], []
+        xmask, grp_off, class_hi, class_off = [], [], [], []
+        gz, gc, gc32, dz, dc, dc32 = [], [], [], [], [], []
+        descs = (_lib.SqdkInst * len(self.spaces))()
+        row_base = [0]
+        item_base = [0]
+        # Term layout: every instance's group terms first (so that group ranges
+        # stay contiguous across instances), then every instance's X = 0 terms.
+        total_group_terms = sum(op.n_terms - op.diag_terms for op in self.ops)
+        b_off = f_off = g_off = gt_off = dt_off = c_off = 0
+        max_bucket = max_fine = max_bits = 0
+        for i, (sp, op) in enumerate(zip(self.spaces, self.ops)):
+            r0 = row_base[-1]
+            keys.append(sp.keys)
+            bstart.append(sp.bstart.to(torch.int64) + r0)
+            fdir.append(sp.fdir)
+            items.append(sp.items[: sp.n_items].to(torch.int64) + r0)
+            item_inst.append(torch.full((sp.n_items,), i, dtype=torch.int32, device=dev))
+            nd = op.diag_terms
+            xmask.append(op.xmask[: op.n_groups])
+            grp_off.append(op.grp_off[:-1].to(torch.int64) - nd + gt_off)
+            gz.append(op.zmask[nd:op.n_terms])
+            gc.append(op.coef[2 * nd:2 * op.n_terms])
+            gc32.append(op.coef32[2 * nd:2 * op.n_terms])
+            dz.append(op.zmask[:nd])
+            dc.append(op.coef[:2 * nd])
+            dc32.append(op.coef32[:2 * nd])
+            class_hi.append(op.class_hi[: op.n_classes])
+            class_off.append(op.class_off[:-1].to(torch.int64) + g_off)
+            d = descs[i]
+            d.bstart_base = b_off
+            d.fdir_base = f_off
+            d.class_base = c_off
+            d.n_classes = op.n_classes
+            d.diag_t0 = total_group_terms + dt_off
+            d.diag_t1 = total_group_terms + dt_off + nd
+            d.bucket_bits = sp.bucket_bits
+            d.fine_bits = sp.fine_bits
+            b_off += sp.bstart.numel()
+            f_off += sp.fdir.numel()
+            g_off += op.n_groups
+            gt_off += op.n_terms - nd
+            dt_off += nd
+            c_off += op.n_classes
+            row_base.append(r0 + sp.n)
+            item_base.append(item_base[-1] + sp.n_items)
+            max_bucket = max(max_bucket, sp.max_bucket)
+            max_fine = max(max_fine, sp.fine_bits)
+            max_bits = max(max_bits, sp.bucket_bits)
+        zmask = gz + dz
+        coef = gc + dc
+        coef32 = gc32 + dc32
+        t_off = gt_off + dt_off
+        # closing entries of the ragged arrays
+        grp_off.append(torch.tensor([gt_off], dtype=torch.int64, device=dev))
+        class_off.append(torch.tensor([g_off], dtype=torch.int64, device=dev))
+        if row_base[-1] >= 2 ** 31:
+            raise ValueError("batch holds 2^31 rows or more")
+        self.n = int(row_base[-1])
+        self.row_base_host = np.asarray(row_base, np.int64)
+        self.keys = torch.cat(keys)
+        self.bstart = torch.cat(bstart).to(torch.int32)
+        self.fdir = torch.cat(fdir)
+        self.items = torch.cat(items).to(torch.int32)
+        self.item_inst = torch.cat(item_inst).to(torch.int16)
+        self.item_base = torch.tensor(item_base, dtype=torch.int32, device=dev)
+        self.row_base = torch.tensor(row_base, dtype=torch.int64, device=dev)
+        self.xmask = torch.cat(xmask) if g_off else torch.zeros(1, dtype=torch.int64, device=dev)
+        self.grp_off = torch.cat(grp_off).to(torch.int32)
+        self.zmask = torch.cat(zmask) if t_off else torch.zeros(1, dtype=torch.int64, device=dev)
+        self.coef = torch.cat(coef) if t_off else torch.zeros(2, dtype=torch.float64, device=dev)
+        self.coef32 = torch.cat(coef32) if t_off else torch.zeros(2, dtype=torch.float32,
+                                                                   device=dev)
+        self.class_hi = torch.cat(class_hi) if c_off else torch.zeros(1, dtype=torch.int32,
+                                                                       device=dev)
+        self.class_off = torch.cat(class_off).to(torch.int32)
+        raw = bytes(descs)
+        self.inst = torch.frombuffer(bytearray(raw), dtype=torch.uint8).to(dev)
+        self.n_items = int(item_base[-1])
+        self.n_inst = len(self.spaces)
+        s = _lib.SqdkBatch()
+        s.keys = self.keys.data_ptr()
+        s.bstart = self.bstart.data_ptr()
+        s.fdir = self.fdir.data_ptr()
+        s.items = self.items.data_ptr()
+        s.item_inst = self.item_inst.data_ptr()
+        s.item_base = self.item_base.data_ptr()
+        s.row_base = self.row_base.data_ptr()
+        s.xmask = self.xmask.data_ptr()
+        s.grp_off = self.grp_off.data_ptr()
+        s.zmask = self.zmask.data_ptr()
+        s.coef = self.coef.data_ptr()
+        s.coef32 = self.coef32.data_ptr()
+        s.class_hi = self.class_hi.data_ptr()
+        s.class_off = self.class_off.data_ptr()
+        s.inst = self.inst.data_ptr()
+        s.n = self.n
+        s.n_inst = self.n_inst
+        s.n_items = self.n_items
+        s.chunk_rows = 256 * self.rows_per_thread
+        s.max_bucket = max_bucket
+        s.max_fine_bits = max_fine
+        s.max_bucket_bits = max_bits
+        self.cstruct = s
+
+    # -- views -------------------------------------------------------------------
+
+    def segment(self, vec: torch.Tensor, i: int) -> torch.Tensor:
+        return vec[self.row_base_host[i]:self.row_base_host[i + 1]]
+
+    def set_tune(self, group_split: int = 1):
+        self.tune.group_split = int(group_split)
+        self._work.clear()
+
+    def _workspace(self, prec):
+        key = (prec, self.tune.group_split)
+        if key not in self._work:
+            nb = C.c_size_t(0)
+            _lib.check(_lib.lib().sqdk_batch_matvec_workspace(
+                C.byref(self.cstruct), C.byref(self.tune), prec, C.byref(nb)),
+                "sqdk_batch_matvec_workspace")
+            self._work[key] = torch.empty(nb.value, dtype=torch.uint8, device=self.device) \
+                if nb.value else None
+        return self._work[key]
+
+    def matvec(self, v: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
+        """Every instance's H_i v_i in one launch (concatenated complex vectors)."""
+        prec = _lib.C128 if v.dtype == torch.complex128 else _lib.C64
+        if v.dtype not in (torch.complex128, torch.complex64) or v.numel() != self.n:
+            raise ValueError("vector must be complex of the batch's total length")
+        v = v.contiguous()
+        out = torch.empty_like(v) if out is None else out
+        work = self._workspace(prec)
+        _lib.check(_lib.lib().sqdk_batch_matvec(
+            C.byref(self.cstruct), prec, v.data_ptr(), out.data_ptr(), C.byref(self.tune),
+            0 if work is None else work.data_ptr(), 0 if work is None else work.numel(),
+            _lib.stream_ptr(self.device)), "sqdk_batch_matvec")
+        return out
+
+    def solve(self, x0: torch.Tensor, *, precision: str = "c64", criterion: str = "relative",
+              tol: float = 1e-6, maxiter: int = 1000, refresh_every: int = 8,
+              use_graph: bool = True) -> BatchSolve:
+        """Fused LOBPCG on every instance from the concatenated start vectors x0."""
+        L = _lib.lib()
+        prec = _PRECISIONS[precision]
+        dtype = torch.complex128 if prec == _lib.C128 else torch.complex64
+        if x0.numel() != self.n:
+            raise ValueError("start vector must have the batch's total length")
+        x = x0.reshape(-1).to(self.device, dtype).clone()
+        opts = _lib.SqdkLocgOpts()
+        opts.mode = _lib.LOCG_FUSED
+        opts.precision = prec
+        opts.criterion = _CRITERIA[criterion]
+        opts.maxiter = int(maxiter)
+        opts.tol = float(tol)
+        opts.n_for_tol = 0
+        opts.check_every = int(refresh_every)
+        opts.refresh_every = int(refresh_every)
+        opts.use_graph = 1 if use_graph else 0
+        nb = C.c_size_t(0)
+        _lib.check(L.sqdk_batch_locg_workspace(C.byref(self.cstruct), C.byref(self.tune),
+                                               C.byref(opts), C.byref(nb)),
+                   "sqdk_batch_locg_workspace")
+        work = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+        res = (_lib.SqdkLocgResult * self.n_inst)()
+        import time
+        torch.cuda.synchronize(self.device)
+        t0 = time.perf_counter()
+        _lib.check(L.sqdk_batch_locg_solve(C.byref(self.cstruct), C.byref(self.tune), x.data_ptr(),
+                                           C.byref(opts), work.data_ptr(), nb.value, res,
+                                           _lib.stream_ptr(self.device)), "sqdk_batch_locg_solve")
+        torch.cuda.synchronize(self.device)
+        dt = time.perf_counter() - t0
+        infos = [LocgInfo(r.eigval, r.niter, bool(r.converged), r.matvecs, r.resid_norm,
+                          r.ax_norm, r.status) for r in res]
+        return BatchSolve(x, infos, dt)
+
+
+def autotune_batch(instances, *, device=None, caps=(2048, 8192), rows=(4,), splits=(1, 2, 4),
+                   reps: int = 3):
+    """Fastest (bucket cap, rows per thread, group split) of the batched matvec."""
+    best = None
+    table = []
+    for cap in caps:
+        for r in rows:
+            b = SQDBatch(instances, device=device, bucket_cap=cap, rows_per_thread=r)
+            g = torch.Generator(device="cpu").manual_seed(0)
+            v = torch.complex(torch.randn(b.n, generator=g), torch.randn(b.n, generator=g)).to(
+                torch.complex64).to(b.device)
+            for s in splits:
+                b.set_tune(group_split=s)
+                out = b.matvec(v)
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    b.matvec(v, out)
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                table.append((cap, r, s, round(ms, 4)))
+                if best is None or ms < best[3]:
+                    best = (cap, r, s, ms)
+            del b
+    return best, table

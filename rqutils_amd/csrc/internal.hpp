@@ -17,6 +17,11 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                 const void* v, void* out, const sqdk_tune* tune, void* work, size_t work_bytes,
                 hipStream_t s, const int32_t* skip = nullptr, int32_t* counter = nullptr);
 
+// Batched matvec; inst_skip (per instance, may be NULL) skips converged instances.
+int batch_matvec_impl(const sqdk_batch* b, int32_t precision, const void* v, void* out,
+                      const sqdk_tune* tune, void* work, size_t work_bytes, hipStream_t s,
+                      const int32_t* skip, const int32_t* inst_skip, int32_t* counter);
+
 // Workspace bytes the matvec needs for this tuning (partial sums of a group split).
 size_t matvec_work_bytes(const sqdk_space* space, const sqdk_tune* tune, int32_t precision);
 

@@ -15,6 +15,8 @@
 // groups: no atomics, deterministic summation order.
 #include "internal.hpp"
 
+#include <cstring>
+
 namespace sqdk {
 
 namespace {
@@ -28,18 +30,17 @@ template <> struct CoefT<float2> { using type = float2; using real = float; };
 // Scalar launch parameters; every table is a separate __restrict__ kernel
 // argument so that the compiler can read uniform entries with scalar loads.
 struct MatvecParams {
-  const int32_t* skip;
+  const int32_t* skip;       // whole launch skipped when *skip != 0
+  const int32_t* inst_skip;  // per instance: its chunks skipped when != 0 (or NULL)
   int32_t* counter;
   int32_t n_items;
-  int32_t m;          // bucket bits
-  int32_t d;          // fine bits
-  int32_t n_classes;
-  int32_t diag_t0, diag_t1;
   int32_t xcd_remap;
-  int32_t max_bucket;
-  int32_t split;      // workgroups sharing one row chunk (groups dealt round robin)
-  int64_t n;          // rows, for the partial-sum layout
-  void* partial;      // (split - 1) x n partial sums when split > 1
+  int32_t max_bucket;        // largest bucket over all instances
+  int32_t dmax;              // largest fine_bits over all instances (LDS layout)
+  int32_t split;             // workgroups sharing one row chunk (groups dealt round robin)
+  int64_t n;                 // rows, for the partial-sum layout
+  void* partial;             // (split - 1) x n partial sums when split > 1
+  sqdk_inst single;          // the instance when no descriptor array is given
 };
 
 template <typename VT>
@@ -56,6 +57,8 @@ struct MatvecTables {
   const int32_t* class_off;
   const VT* v;
   VT* out;
+  const sqdk_inst* descs;
+  const uint16_t* item_inst;
 };
 
 // Workgroup i runs on XCD i % 8; give each XCD a contiguous range of items so
@@ -90,7 +93,9 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
                    const int32_t* __restrict__ grp_off, const uint64_t* __restrict__ zmask,
                    const typename CoefT<VT>::type* __restrict__ coef,
                    const uint32_t* __restrict__ class_hi, const int32_t* __restrict__ class_off,
-                   const VT* __restrict__ v, VT* __restrict__ out, MatvecParams a) {
+                   const VT* __restrict__ v, VT* __restrict__ out,
+                   const sqdk_inst* __restrict__ descs, const uint16_t* __restrict__ item_inst,
+                   MatvecParams a) {
   using CT = typename CoefT<VT>::type;
   using RT = typename CoefT<VT>::real;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -101,13 +106,19 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   if (w >= a.n_items) return;
   if (a.skip && a.skip[0] != 0) return;
   if (a.counter && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(a.counter, 1);
+  // The chunk's instance: its directories and classes (all indices are global).
+  const int inst_id = item_inst ? (int)item_inst[w] : 0;
+  if (a.inst_skip && a.inst_skip[inst_id] != 0) return;
+  const sqdk_inst D = descs ? descs[inst_id] : a.single;
+  const uint32_t* __restrict__ bst = bstart + D.bstart_base;
+  const uint16_t* __restrict__ fdr = fdir + D.fdir_base;
 
-  const int m = a.m;
-  const int d = a.d;
+  const int m = D.bucket_bits;
+  const int d = D.fine_bits;
   const int f = m - d;
   const uint64_t lowmask = (m >= 64) ? ~0ull : ((1ull << m) - 1ull);
   const int ndir = 1 << d;
-  const LdsLayout lay = lds_layout(d, a.max_bucket, sizeof(KT), STAGE_V ? sizeof(VT) : 0);
+  const LdsLayout lay = lds_layout(a.dmax, a.max_bucket, sizeof(KT), STAGE_V ? sizeof(VT) : 0);
   uint32_t* lpair = reinterpret_cast<uint32_t*>(smem + lay.pairs);
   int* s_occ = reinterpret_cast<int*>(smem + lay.occ);
   KT* lkeys = reinterpret_cast<KT*>(smem + lay.keys);
@@ -116,7 +127,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   const uint32_t row0 = items[w];
   const uint64_t bucket = keys[row0] >> m;
   const uint64_t bucket_base = m >= 64 ? 0ull : (bucket << m);
-  const uint32_t bend = bstart[bucket + 1];
+  const uint32_t bend = bst[bucket + 1];
   const uint32_t row1 = min(bend, row0 + (uint32_t)(ROWS * kBlock));
 
   // Each row keeps only its low key bits: the high bits are the bucket's.
@@ -134,24 +145,27 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   const bool wave_active = __any(valid[0]);
 
   // X = 0 group: the diagonal (first share only).
-  if (gs == 0 && a.diag_t1 > a.diag_t0) {
+  if (gs == 0 && D.diag_t1 > D.diag_t0) {
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       if (!valid[r]) continue;
       RT cr, ci;
-      group_coef<CT>(zmask, coef, a.diag_t0, a.diag_t1, bucket_base | (uint64_t)sl[r], cr, ci);
+      group_coef<CT>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base | (uint64_t)sl[r], cr, ci);
       acc[r].fma(cr, ci, v[row0 + tid + r * kBlock]);
     }
   }
 
-  for (int c = 0; c < a.n_classes; ++c) {
+  // Groups are dealt round robin by their index within the instance, so a
+  // batched launch splits every instance exactly as its own launch would.
+  const int ginst = D.n_classes > 0 ? class_off[D.class_base] : 0;
+  for (int c = D.class_base; c < D.class_base + D.n_classes; ++c) {
     const int g0 = class_off[c], g1 = class_off[c + 1];
-    // First group of this class dealt to share gs (groups go round robin).
-    const int gfirst = g0 + ((gs - g0 % a.split) + a.split) % a.split;
+    // First group of this class dealt to share gs.
+    const int gfirst = g0 + ((gs - (g0 - ginst) % a.split) + a.split) % a.split;
     if (gfirst >= g1) continue;
     const uint64_t q = bucket ^ (uint64_t)class_hi[c];
-    const uint32_t qs = bstart[q];
-    const uint32_t qe = bstart[q + 1];
+    const uint32_t qs = bst[q];
+    const uint32_t qe = bst[q + 1];
     if (qs == qe) continue;  // empty target bucket: no group of the class can hit
     const int qn = (int)(qe - qs);
     __syncthreads();  // the previous class is done with the LDS tables
@@ -164,8 +178,8 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     const uint64_t fbase = q << d;
     int occ = 0;
     for (int e = tid; e < ndir; e += kBlock) {
-      const uint32_t lo = fdir[fbase + e];
-      const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdir[fbase + e + 1] : (uint32_t)qn;
+      const uint32_t lo = fdr[fbase + e];
+      const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
       lpair[e] = lo | (hi << 16);
       occ = max(occ, (int)(hi - lo));
     }
@@ -252,6 +266,7 @@ split_reduce_kernel(int64_t n, int nparts, const VT* __restrict__ partial, VT* _
 
 template <typename VT, typename KT, int ROWS, bool SV>
 int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds, hipStream_t s) {
+  // (t.descs / t.item_inst are NULL for a single instance)
   auto kern = bucket_join_kernel<VT, KT, ROWS, SV>;
   if (lds > 65536) {
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -261,7 +276,7 @@ int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds,
   if (grid >= (1ll << 31)) return fail(SQDK_ERANGE, "grid of %lld workgroups too large", (long long)grid);
   kern<<<(unsigned)grid, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.items, t.xmask, t.grp_off,
                                            t.zmask, t.coef, t.class_hi, t.class_off, t.v, t.out,
-                                           args);
+                                           t.descs, t.item_inst, args);
   SQDK_LAUNCHED("bucket_join_kernel");
   if (args.split > 1) {
     const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(args.n, kBlock), 4096));
@@ -285,6 +300,21 @@ int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, siz
 }
 
 template <typename VT>
+int launch_common(const MatvecTables<VT>& t, MatvecParams& args, int narrow_keys, int rows,
+                  int stage_v, hipStream_t s) {
+  const LdsLayout lay = lds_layout(args.dmax, args.max_bucket, narrow_keys ? 4 : 8,
+                                   stage_v ? sizeof(VT) : 0);
+  if (lay.total > 160 * 1024)
+    return fail(SQDK_ERANGE, "LDS tables of %zu bytes exceed 160 KiB (largest bucket %d keys)",
+                lay.total, args.max_bucket);
+  if (stage_v)
+    return narrow_keys ? launch_kt<VT, uint32_t, true>(t, args, rows, lay.total, s)
+                       : launch_kt<VT, uint64_t, true>(t, args, rows, lay.total, s);
+  return narrow_keys ? launch_kt<VT, uint32_t, false>(t, args, rows, lay.total, s)
+                     : launch_kt<VT, uint64_t, false>(t, args, rows, lay.total, s);
+}
+
+template <typename VT>
 int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const void* v,
               void* out, int rows, int xcd, int stage_v, int split, void* work,
               hipStream_t s, const int32_t* skip, int32_t* counter) {
@@ -301,31 +331,62 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   t.class_off = h->class_off;
   t.v = reinterpret_cast<const VT*>(v);
   t.out = reinterpret_cast<VT*>(out);
+  t.descs = nullptr;
+  t.item_inst = nullptr;
   MatvecParams args;
   args.skip = skip;
+  args.inst_skip = nullptr;
   args.counter = counter;
   args.n_items = sp->n_items;
-  args.m = sp->bucket_bits;
-  args.d = sp->fine_bits;
-  args.n_classes = h->n_classes;
-  args.diag_t0 = h->diag_t0;
-  args.diag_t1 = h->diag_t1;
   args.xcd_remap = xcd;
   args.max_bucket = sp->max_bucket;
+  args.dmax = sp->fine_bits;
   args.split = split;
   args.n = sp->n;
   args.partial = work;
-  const bool narrow = sp->bucket_bits <= 32;
-  const LdsLayout lay = lds_layout(sp->fine_bits, sp->max_bucket, narrow ? 4 : 8,
-                                   stage_v ? sizeof(VT) : 0);
-  if (lay.total > 160 * 1024)
-    return fail(SQDK_ERANGE, "LDS tables of %zu bytes exceed 160 KiB (largest bucket %d keys)",
-                lay.total, sp->max_bucket);
-  if (stage_v)
-    return narrow ? launch_kt<VT, uint32_t, true>(t, args, rows, lay.total, s)
-                  : launch_kt<VT, uint64_t, true>(t, args, rows, lay.total, s);
-  return narrow ? launch_kt<VT, uint32_t, false>(t, args, rows, lay.total, s)
-                : launch_kt<VT, uint64_t, false>(t, args, rows, lay.total, s);
+  args.single.bstart_base = 0;
+  args.single.fdir_base = 0;
+  args.single.class_base = 0;
+  args.single.n_classes = h->n_classes;
+  args.single.diag_t0 = h->diag_t0;
+  args.single.diag_t1 = h->diag_t1;
+  args.single.bucket_bits = sp->bucket_bits;
+  args.single.fine_bits = sp->fine_bits;
+  return launch_common<VT>(t, args, sp->bucket_bits <= 32, rows, stage_v, s);
+}
+
+template <typename VT>
+int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out, int rows,
+                 int xcd, int stage_v, int split, void* work, hipStream_t s,
+                 const int32_t* skip, const int32_t* inst_skip, int32_t* counter) {
+  MatvecTables<VT> t;
+  t.keys = b->keys;
+  t.bstart = b->bstart;
+  t.fdir = b->fdir;
+  t.items = b->items;
+  t.xmask = b->xmask;
+  t.grp_off = b->grp_off;
+  t.zmask = b->zmask;
+  t.coef = reinterpret_cast<const typename CoefT<VT>::type*>(coef);
+  t.class_hi = b->class_hi;
+  t.class_off = b->class_off;
+  t.v = reinterpret_cast<const VT*>(v);
+  t.out = reinterpret_cast<VT*>(out);
+  t.descs = b->inst;
+  t.item_inst = b->item_inst;
+  MatvecParams args;
+  args.skip = skip;
+  args.inst_skip = inst_skip;
+  args.counter = counter;
+  args.n_items = b->n_items;
+  args.xcd_remap = xcd;
+  args.max_bucket = b->max_bucket;
+  args.dmax = b->max_fine_bits;
+  args.split = split;
+  args.n = b->n;
+  args.partial = work;
+  memset(&args.single, 0, sizeof(args.single));
+  return launch_common<VT>(t, args, b->max_bucket_bits <= 32, rows, stage_v, s);
 }
 
 }  // namespace
@@ -386,10 +447,67 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }
 
+int batch_matvec_impl(const sqdk_batch* b, int32_t precision, const void* v, void* out,
+                      const sqdk_tune* tune, void* work, size_t work_bytes, hipStream_t s,
+                      const int32_t* skip, const int32_t* inst_skip, int32_t* counter) {
+  SQDK_REQUIRE(b != nullptr, "batch is NULL");
+  SQDK_REQUIRE(v && out, "NULL vector");
+  SQDK_REQUIRE(b->keys && b->bstart && b->fdir && b->items && b->item_inst && b->inst,
+               "batch has NULL subspace tables");
+  SQDK_REQUIRE(b->n >= 1 && b->n < (1ll << 31), "batch size %lld out of range", (long long)b->n);
+  SQDK_REQUIRE(b->n_inst >= 1 && b->n_inst <= 65535, "n_inst %d out of range", b->n_inst);
+  SQDK_REQUIRE(b->max_bucket >= 1 && b->max_bucket < 65536, "max_bucket %d out of range",
+               b->max_bucket);
+  SQDK_REQUIRE(b->max_fine_bits >= 0 && b->max_fine_bits <= 15, "max_fine_bits %d invalid",
+               b->max_fine_bits);
+  SQDK_REQUIRE(b->max_bucket_bits >= 0 && b->max_bucket_bits <= 63, "max_bucket_bits %d invalid",
+               b->max_bucket_bits);
+  if (b->n_items == 0) return SQDK_OK;
+  int rows = 4, xcd = 1, stage_v = 0, split = 1;
+  if (tune) {
+    if (tune->rows_per_thread) rows = tune->rows_per_thread;
+    if (tune->group_split > 1) split = tune->group_split;
+    xcd = tune->xcd_remap;
+    stage_v = tune->stage_v;
+  }
+  SQDK_REQUIRE(split <= 1024, "group_split %d too large", split);
+  SQDK_REQUIRE(rows * kBlock == b->chunk_rows,
+               "rows_per_thread %d does not match the batch's chunk of %d rows", rows,
+               b->chunk_rows);
+  const size_t need = split > 1 ? (size_t)(split - 1) * (size_t)b->n * (precision == SQDK_C128 ? 16 : 8) : 0;
+  SQDK_REQUIRE(need == 0 || (work && work_bytes >= need),
+               "matvec workspace of %zu bytes, %zu needed for group_split %d", work_bytes, need,
+               split);
+  if (precision == SQDK_C128)
+    return launch_batch<double2>(b, b->coef, v, out, rows, xcd, stage_v, split, work, s, skip,
+                                 inst_skip, counter);
+  if (precision == SQDK_C64)
+    return launch_batch<float2>(b, b->coef32, v, out, rows, xcd, stage_v, split, work, s, skip,
+                                inst_skip, counter);
+  return fail(SQDK_EINVAL, "unknown precision %d", precision);
+}
+
 }  // namespace sqdk
+
+extern "C" int sqdk_batch_matvec_workspace(const sqdk_batch* batch, const sqdk_tune* tune,
+                                           int32_t precision, size_t* bytes) {
+  using namespace sqdk;
+  SQDK_REQUIRE(batch && bytes, "NULL pointer argument");
+  const int split = (tune && tune->group_split > 1) ? tune->group_split : 1;
+  *bytes = split > 1 ? (size_t)(split - 1) * (size_t)batch->n * (precision == SQDK_C128 ? 16 : 8) : 0;
+  return SQDK_OK;
+}
+
+extern "C" int sqdk_batch_matvec(const sqdk_batch* batch, int32_t precision, const void* v,
+                                 void* out, const sqdk_tune* tune, void* work, size_t work_bytes,
+                                 void* stream) {
+  return sqdk::batch_matvec_impl(batch, precision, v, out, tune, work, work_bytes,
+                                 sqdk::as_stream(stream), nullptr, nullptr, nullptr);
+}
 
 extern "C" int sqdk_matvec_workspace(const sqdk_space* space, const sqdk_tune* tune,
                                      int32_t precision, size_t* bytes) {
+  using namespace sqdk;
   SQDK_REQUIRE(space && bytes, "NULL pointer argument");
   SQDK_REQUIRE(precision == SQDK_C128 || precision == SQDK_C64, "unknown precision %d", precision);
   *bytes = sqdk::matvec_work_bytes(space, tune, precision);
