@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 4
+#define SQDK_ABI_VERSION 5
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -312,6 +312,45 @@ int sqdk_batch_locg_solve(const sqdk_batch* batch, const sqdk_tune* tune, void* 
  * the eigenvalue then 2*dim doubles of eigenvector. dim = 2 or 3. */
 int sqdk_small_eigenpair(int32_t dim, const double* mat, double* out,
                          void* stream);
+
+/* ---- start-vector network (BF16 MLP on the matrix cores) --------------- *
+ * No counterpart in the reference (it starts LOBPCG from a one-hot vector,
+ * rqutils/sqd.py:373-398). bf16 values are passed as uint16 bit patterns. */
+
+/* C = op(A) op(B) (+ bias[N]) with bf16 A, B and FP32 accumulation on MFMA.
+ * op(A)(i,k) = trans_a ? A[k*lda + i] : A[i*lda + k]; op(B)(k,j) likewise
+ * with ldb. ksplit > 1 splits K over that many partial tiles, reduced in a
+ * fixed order (workspace: sqdk_gemm_bf16_workspace). */
+int sqdk_gemm_bf16_workspace(int64_t M, int64_t N, int32_t ksplit, size_t* bytes);
+int sqdk_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t lda,
+                   int32_t trans_a, const uint16_t* B, int64_t ldb, int32_t trans_b,
+                   float* C, int64_t ldc, const float* bias, int32_t ksplit, float* work,
+                   size_t work_bytes, void* stream);
+
+/* out[i*kpad + q] = 1 - 2 bit_q(keys[i]) (bf16) for q < num_qubits, else 0. */
+int sqdk_mlp_encode(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t kpad,
+                    uint16_t* out, void* stream);
+
+/* op 0: out = gelu(z); op 1: out = g * gelu'(z); op 2: out = z; GELU is the
+ * exact erf form. Writes bf16 and/or FP32 outputs (either may be NULL). */
+int sqdk_mlp_act(int32_t op, int64_t n, const float* z, const float* g, uint16_t* out_bf16,
+                 float* out_f32, void* stream);
+
+/* Weighted loss of outputs y[i*ldy + {0,1}] (log-amplitude, sign logit)
+ * against target[3i..3i+2] = (log-amplitude, sign probability, weight):
+ * sum w [(y0 - a)^2 + lambda BCE(sigmoid(y1), p)] / wsum. Writes dL/dy
+ * (n x 2, FP32 and bf16) and nblocks partial sums of the loss. */
+int sqdk_mlp_loss(int64_t n, const float* y, int64_t ldy, const float* target, double wsum,
+                  double lambda, float* dy_f32, uint16_t* dy_bf16, double* partial,
+                  int32_t nblocks, void* stream);
+
+/* out[j] = sum over rows i of x[i*cols + j] (FP64 accumulation, row order). */
+int sqdk_colsum(int64_t n, int64_t cols, const float* x, float* out, void* stream);
+
+/* One Adam step on FP32 weights w with gradient g and moments m, v (step >= 1),
+ * refreshing the bf16 copy w_bf16 (may be NULL). */
+int sqdk_adam(int64_t n, float* w, const float* g, float* m, float* v, uint16_t* w_bf16,
+              float lr, float beta1, float beta2, float eps, int32_t step, void* stream);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -133,6 +133,23 @@ SIGNATURES = {
                                   C.POINTER(SqdkLocgOpts), C.c_void_p, C.c_size_t,
                                   C.POINTER(SqdkLocgResult), C.c_void_p]),
     "sqdk_small_eigenpair": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sqdk_gemm_bf16_workspace": (C.c_int, [C.c_int64, C.c_int64, C.c_int32,
+                                           C.POINTER(C.c_size_t)]),
+    "sqdk_gemm_bf16": (C.c_int, [C.c_int64, C.c_int64, C.c_int64, C.c_void_p, C.c_int64,
+                                 C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
+                                 C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t,
+                                 C.c_void_p]),
+    "sqdk_mlp_encode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                  C.c_void_p]),
+    "sqdk_mlp_act": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p]),
+    "sqdk_mlp_loss": (C.c_int, [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,
+                                C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32,
+                                C.c_void_p]),
+    "sqdk_colsum": (C.c_int, [C.c_int64, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sqdk_adam": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
+                            C.c_void_p]),
 }
 
 

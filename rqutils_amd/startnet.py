@@ -1,0 +1,242 @@
+"""Learned LOBPCG starting vectors: a BF16 MLP trained on ground vectors.
+
+``StartNet`` maps a bitstring (bits as +-1) through 256 -> 256 -> 2 units
+(GELU) to a log-amplitude and a sign logit. Its weights live in FP32 with
+BF16 copies; every product runs on the matrix cores through
+``sqdk_gemm_bf16`` (FP32 accumulation), the activations, loss and Adam in
+the library's elementwise kernels. It is trained on ground vectors that the
+FP64 LOBPCG itself produced (``targets_from_vector``) and then proposes
+``start_vector(keys)`` for new Hamiltonians; a plain Gaussian start remains
+the fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import torch
+
+from . import _lib
+
+__all__ = ["StartNet", "targets_from_vector", "ground_targets", "gaussian_start"]
+
+
+def _p(t):
+    return 0 if t is None else t.data_ptr()
+
+
+def targets_from_vector(psi: torch.Tensor) -> torch.Tensor:
+    """Training targets (n x 3: log-amplitude, sign probability, weight) of a
+    ground vector: the global phase is fixed so that the largest component is
+    real and positive, the sign is that of the real part, the log-amplitude is
+    centred by its |psi|^2-weighted mean and the weight is n |psi|^2."""
+    psi = psi.to(torch.complex128)
+    j = int(torch.argmax(psi.abs()).item())
+    psi = psi * (psi[j].conj() / psi[j].abs())
+    amp = psi.abs()
+    w = amp ** 2
+    w = w / w.sum()
+    la = torch.log(amp + 1e-300)
+    la = la - (w * la).sum()
+    n = psi.numel()
+    tgt = torch.stack([la, (psi.real > 0).to(torch.float64), w * n], dim=1)
+    return tgt.to(torch.float32).contiguous()
+
+
+def gaussian_start(n: int, device, seed: int = 0) -> torch.Tensor:
+    """Seeded complex Gaussian start vector (the method without the network)."""
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    return torch.complex(torch.randn(n, generator=g, dtype=torch.float64),
+                         torch.randn(n, generator=g, dtype=torch.float64)).to(device)
+
+
+def ground_targets(prob, *, tol: float = 1e-10, maxiter: int = 5000, seed: int = 0):
+    """Ground vector of an ``SQDProblem`` from the FP64 fused LOBPCG (Gaussian
+    start) and its training targets. Returns (targets, vector, info)."""
+    x0 = gaussian_start(prob.n, prob.space.keys.device, seed)
+    vec, info = prob.solve(x0, mode="fused", precision="c128", criterion="relative", tol=tol,
+                           maxiter=maxiter)
+    return targets_from_vector(vec), vec, info
+
+
+class StartNet:
+    """BF16 MLP (num_qubits -> hidden -> hidden -> 2) with MFMA kernels."""
+
+    def __init__(self, num_qubits: int, hidden: int = 256, *, device=None, seed: int = 0,
+                 lam: float = 1.0):
+        self.nq = int(num_qubits)
+        self.kpad = 16 * math.ceil(self.nq / 16)
+        self.h = int(hidden)
+        self.lam = float(lam)
+        self.device = torch.device(device if device is not None else "cuda")
+        g = torch.Generator(device="cpu").manual_seed(seed)
+
+        def init(fan_in, shape):
+            bound = 1.0 / math.sqrt(fan_in)
+            return ((torch.rand(shape, generator=g, dtype=torch.float64) * 2 - 1) * bound).to(
+                torch.float32)
+
+        w1 = init(self.nq, (self.kpad, self.h))
+        w1[self.nq:] = 0
+        self.params = [w1, torch.zeros(self.h), init(self.h, (self.h, self.h)),
+                       torch.zeros(self.h), init(self.h, (self.h, 2)), torch.zeros(2)]
+        self.params = [p.to(self.device).contiguous() for p in self.params]
+        self.m = [torch.zeros_like(p) for p in self.params]
+        self.v = [torch.zeros_like(p) for p in self.params]
+        self.grads = [torch.zeros_like(p) for p in self.params]
+        self.bf16 = [torch.empty(p.shape, dtype=torch.int16, device=self.device)
+                     if i % 2 == 0 else None for i, p in enumerate(self.params)]
+        self._sync_bf16()
+        self.step_count = 0
+        self._ws = {}
+
+    # -- helpers ---------------------------------------------------------------
+
+    def _stream(self):
+        return _lib.stream_ptr(self.device)
+
+    def _sync_bf16(self):
+        for p, q in zip(self.params, self.bf16):
+            if q is not None:
+                _lib.check(_lib.lib().sqdk_mlp_act(2, p.numel(), p.data_ptr(), None, q.data_ptr(),
+                                                   None, self._stream()), "sqdk_mlp_act")
+
+    def _gemm(self, M, N, K, A, lda, ta, B, ldb, tb, Cout, ldc, bias=None, ksplit=1):
+        L = _lib.lib()
+        work = None
+        nbytes = 0
+        if ksplit > 1:
+            nb = C.c_size_t(0)
+            _lib.check(L.sqdk_gemm_bf16_workspace(M, N, ksplit, C.byref(nb)), "gemm workspace")
+            key = ("gemm", nb.value)
+            if key not in self._ws:
+                self._ws[key] = torch.empty(nb.value, dtype=torch.uint8, device=self.device)
+            work = self._ws[key]
+            nbytes = nb.value
+        _lib.check(L.sqdk_gemm_bf16(M, N, K, A.data_ptr(), lda, ta, B.data_ptr(), ldb, tb,
+                                    Cout.data_ptr(), ldc, _p(bias), ksplit, _p(work), nbytes,
+                                    self._stream()), "sqdk_gemm_bf16")
+
+    def _act(self, op, z, g=None, want16=True, want32=False):
+        out16 = torch.empty(z.shape, dtype=torch.int16, device=self.device) if want16 else None
+        out32 = torch.empty(z.shape, dtype=torch.float32, device=self.device) if want32 else None
+        _lib.check(_lib.lib().sqdk_mlp_act(op, z.numel(), z.data_ptr(), _p(g), _p(out16), _p(out32),
+                                           self._stream()), "sqdk_mlp_act")
+        return out16, out32
+
+    def encode(self, keys: torch.Tensor) -> torch.Tensor:
+        n = keys.numel()
+        x = torch.empty((n, self.kpad), dtype=torch.int16, device=self.device)
+        _lib.check(_lib.lib().sqdk_mlp_encode(keys.data_ptr(), n, self.nq, self.kpad, x.data_ptr(),
+                                              self._stream()), "sqdk_mlp_encode")
+        return x
+
+    # -- passes ----------------------------------------------------------------
+
+    def forward(self, keys: torch.Tensor, keep: bool = False):
+        """Outputs y (n x 2, FP32) for uint64 keys (torch int64 tensor)."""
+        n = keys.numel()
+        w1, b1, w2, b2, w3, b3 = self.params
+        x = self.encode(keys)
+        z1 = torch.empty((n, self.h), dtype=torch.float32, device=self.device)
+        self._gemm(n, self.h, self.kpad, x, self.kpad, 0, self.bf16[0], self.h, 0, z1, self.h, b1)
+        h1, _ = self._act(0, z1)
+        z2 = torch.empty_like(z1)
+        self._gemm(n, self.h, self.h, h1, self.h, 0, self.bf16[2], self.h, 0, z2, self.h, b2)
+        h2, _ = self._act(0, z2)
+        y = torch.empty((n, 2), dtype=torch.float32, device=self.device)
+        self._gemm(n, 2, self.h, h2, self.h, 0, self.bf16[4], 2, 0, y, 2, b3)
+        if keep:
+            self._cache = (x, z1, h1, z2, h2)
+        return y
+
+    def loss_grads(self, keys: torch.Tensor, target: torch.Tensor, wsum: float | None = None):
+        """Loss and gradients (into ``self.grads``) of one batch.
+
+        ``target`` is n x 3 (log-amplitude, sign probability, weight) in FP32;
+        the loss is normalised by ``wsum`` (default: the batch's weight sum).
+        Returns the loss as a 0-d device tensor (no host synchronisation)."""
+        n = keys.numel()
+        y = self.forward(keys, keep=True)
+        x, z1, h1, z2, h2 = self._cache
+        if wsum is None:
+            wsum = float(target[:, 2].sum().item())
+        nblocks = 64
+        dy32 = torch.empty((n, 2), dtype=torch.float32, device=self.device)
+        dy16 = torch.empty((n, 2), dtype=torch.int16, device=self.device)
+        part = torch.empty(nblocks, dtype=torch.float64, device=self.device)
+        L = _lib.lib()
+        s = self._stream()
+        _lib.check(L.sqdk_mlp_loss(n, y.data_ptr(), 2, target.data_ptr(), wsum, self.lam,
+                                   dy32.data_ptr(), dy16.data_ptr(), part.data_ptr(), nblocks, s),
+                   "sqdk_mlp_loss")
+        gw1, gb1, gw2, gb2, gw3, gb3 = self.grads
+        ks = max(1, min(64, n // 512))     # K-split of the batch for the weight gradients
+        # output layer: dW3 = H2^T dY, db3 = colsum dY, dH2 = dY W3^T
+        self._gemm(self.h, 2, n, h2, self.h, 1, dy16, 2, 0, gw3, 2, ksplit=ks)
+        _lib.check(L.sqdk_colsum(n, 2, dy32.data_ptr(), gb3.data_ptr(), s), "sqdk_colsum")
+        dh2 = torch.empty((n, self.h), dtype=torch.float32, device=self.device)
+        self._gemm(n, self.h, 2, dy16, 2, 0, self.bf16[4], 2, 1, dh2, self.h)
+        dz2_16, dz2 = self._act(1, z2, dh2, want16=True, want32=True)
+        # hidden layer: dW2 = H1^T dZ2, db2 = colsum dZ2, dH1 = dZ2 W2^T
+        self._gemm(self.h, self.h, n, h1, self.h, 1, dz2_16, self.h, 0, gw2, self.h, ksplit=ks)
+        _lib.check(L.sqdk_colsum(n, self.h, dz2.data_ptr(), gb2.data_ptr(), s), "sqdk_colsum")
+        dh1 = torch.empty_like(dh2)
+        self._gemm(n, self.h, self.h, dz2_16, self.h, 0, self.bf16[2], self.h, 1, dh1, self.h)
+        dz1_16, dz1 = self._act(1, z1, dh1, want16=True, want32=True)
+        # input layer: dW1 = X^T dZ1, db1 = colsum dZ1
+        self._gemm(self.kpad, self.h, n, x, self.kpad, 1, dz1_16, self.h, 0, gw1, self.h,
+                   ksplit=ks)
+        _lib.check(L.sqdk_colsum(n, self.h, dz1.data_ptr(), gb1.data_ptr(), s), "sqdk_colsum")
+        self._cache = None
+        return part.sum()
+
+    def adam_step(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999,
+                  eps: float = 1e-8):
+        """One Adam update of every parameter (refreshes the BF16 copies).
+        The padding rows of W1 see zero inputs, hence zero gradients, and stay 0."""
+        self.step_count += 1
+        L = _lib.lib()
+        for i, (p, g, m, v) in enumerate(zip(self.params, self.grads, self.m, self.v)):
+            _lib.check(L.sqdk_adam(p.numel(), p.data_ptr(), g.data_ptr(), m.data_ptr(),
+                                   v.data_ptr(), _p(self.bf16[i]), lr, beta1, beta2, eps,
+                                   self.step_count, self._stream()), "sqdk_adam")
+
+    def train(self, keys_list, targets_list, *, steps: int = 3000, batch: int = 4096,
+              lr: float = 3e-3, seed: int = 0, log_every: int = 0):
+        """Adam with a cosine learning-rate schedule on (state, target) pairs
+        drawn uniformly from the concatenated instances. Each instance's weights
+        have mean 1, so the batch size normalises the loss. Returns the losses
+        (host floats) of every ``max(1, log_every)``-th step."""
+        keys = torch.cat([k.to(self.device, torch.int64) for k in keys_list]).contiguous()
+        tgts = torch.cat([t.to(self.device, torch.float32) for t in targets_list]).contiguous()
+        gen = torch.Generator(device=self.device).manual_seed(seed)
+        losses = []
+        every = max(1, log_every)
+        for step in range(steps):
+            idx = torch.randint(0, keys.numel(), (batch,), generator=gen, device=self.device)
+            loss = self.loss_grads(keys[idx], tgts[idx].contiguous(), wsum=float(batch))
+            self.adam_step(lr=lr * 0.5 * (1.0 + math.cos(math.pi * step / steps)))
+            if step % every == 0 or step == steps - 1:
+                losses.append(float(loss.item()))
+                if log_every:
+                    print(f"[startnet] step {step} loss {losses[-1]:.4f}", flush=True)
+        return losses
+
+    def sign_accuracy(self, keys: torch.Tensor, target: torch.Tensor) -> float:
+        """Weighted fraction of states whose predicted sign matches the target."""
+        y = self.forward(keys)
+        ok = ((y[:, 1] > 0) == (target[:, 1] > 0.5)).to(torch.float64)
+        w = target[:, 2].to(torch.float64)
+        return float((ok * w).sum() / w.sum())
+
+    def start_vector(self, keys: torch.Tensor, chunk: int = 1 << 18) -> torch.Tensor:
+        """Normalised complex128 start vector exp(y0) sign(y1) over the keys."""
+        outs = []
+        for s in range(0, keys.numel(), chunk):
+            outs.append(self.forward(keys[s:s + chunk].contiguous()))
+        y = torch.cat(outs).to(torch.float64)
+        a = y[:, 0] - y[:, 0].max()
+        x = torch.exp(a) * torch.where(y[:, 1] > 0, 1.0, -1.0)
+        x = x / torch.linalg.vector_norm(x)
+        return x.to(torch.complex128)

@@ -197,8 +197,10 @@ def small16(seed: int = 1) -> Workload:
                     {"terms": 200, "sector_ones": 8})
 
 
-def heisenberg28(seed: int = 2, n_states: int = 1 << 22, n_random: int = 2000,
+def heisenberg28(seed: int = 2, n_states: int | None = 1 << 22, n_random: int = 2000,
                  num_qubits: int = 28) -> Workload:
+    """Heisenberg chain plus disorder; ``n_states=None`` (or a count at least
+    the sector size) takes the whole half-filling sector."""
     rng = np.random.default_rng(seed)
     n = num_qubits
     labels, coeffs = [], []
@@ -212,10 +214,19 @@ def heisenberg28(seed: int = 2, n_states: int = 1 << 22, n_random: int = 2000,
     rl, _ = random_pauli_strings(rng, n, n_random, 2, 4)
     labels += rl
     coeffs = np.concatenate([np.asarray(coeffs), rng.normal(0.0, 0.1, size=n_random)])
-    keys = domain_wall_keys(rng, n, n // 2, n_states, beta=0.5)
-    return Workload("heisenberg28", n, labels, coeffs, keys, seed,
+    sector = comb(n, n // 2)
+    if n_states is None or n_states >= sector:
+        if n > 24:
+            raise ValueError("whole-sector enumeration is limited to 24 qubits")
+        allk = np.arange(1 << n, dtype=np.uint64)
+        keys = allk[_popcount(allk) == n // 2]
+        n_states = sector
+    else:
+        keys = domain_wall_keys(rng, n, n // 2, n_states, beta=0.5)
+    name = "heisenberg28" if (n, n_states) == (28, 1 << 22) else f"heisenberg{n}_{n_states}"
+    return Workload(name, n, labels, coeffs, keys, seed,
                     {"bonds": n - 1, "random_terms": n_random, "sector_ones": n // 2,
-                     "beta": 0.5})
+                     "beta": 0.5, "states": n_states})
 
 
 def random40(seed: int = 3, n_states: int = 1 << 24, n_terms: int = 20000,

@@ -177,3 +177,98 @@ def test_golden_phase_table():
         cols = np.stack([p.matvec(np.eye(2 ** n)[:, j].astype(complex)) for j in range(2 ** n)],
                         axis=1)
         assert np.allclose(cols, mat[: 2 ** n, : 2 ** n]), str(lab)
+
+
+# ---- start-vector network oracle ---------------------------------------------------
+
+from oracle import mlp_oracle as mo  # noqa: E402
+
+
+def _mlp_params(rng, kin, h):
+    return [rng.normal(size=(kin, h)) / np.sqrt(kin), 0.1 * rng.normal(size=h),
+            rng.normal(size=(h, h)) / np.sqrt(h), 0.1 * rng.normal(size=h),
+            rng.normal(size=(h, 2)) / np.sqrt(h), 0.1 * rng.normal(size=2)]
+
+
+def _mlp_batch(rng, n, nq, kpad):
+    keys = rng.integers(0, 2 ** nq, size=n).astype(np.uint64)
+    tgt = np.stack([rng.normal(size=n), rng.integers(0, 2, n).astype(float),
+                    rng.uniform(0.1, 2.0, n)], axis=1)
+    return mo.encode(keys, nq, kpad), tgt
+
+
+def test_mlp_oracle_gradients_match_finite_differences():
+    # Central differences on every parameter tensor: pins the chain rule of the
+    # backward pass (a dropped GELU derivative or a transposed weight fails).
+    rng = np.random.default_rng(0)
+    params = _mlp_params(rng, 8, 6)
+    x, tgt = _mlp_batch(rng, 24, 5, 8)
+    _, grads, _, _ = mo.loss_and_grads(params, x, tgt, lam=0.7)
+    eps = 1e-6
+    for p, g in zip(params, grads):
+        flat, gf = p.reshape(-1), g.reshape(-1)
+        for j in rng.choice(flat.size, size=min(8, flat.size), replace=False):
+            old = flat[j]
+            flat[j] = old + eps
+            lp = mo.loss_and_grads(params, x, tgt, lam=0.7)[0]
+            flat[j] = old - eps
+            lm = mo.loss_and_grads(params, x, tgt, lam=0.7)[0]
+            flat[j] = old
+            fd = (lp - lm) / (2 * eps)
+            assert abs(fd - gf[j]) <= 2e-7 * max(1.0, abs(fd)), (p.shape, j, fd, gf[j])
+
+
+def test_mlp_oracle_loss_gelu_adam_match_torch():
+    # Library routines: torch's exact-erf GELU, BCE-with-logits and Adam.
+    import torch
+    import torch.nn.functional as F
+    rng = np.random.default_rng(1)
+    z = np.linspace(-9, 9, 301)
+    zt = torch.tensor(z, requires_grad=True)
+    gt = F.gelu(zt)
+    gt.sum().backward()
+    assert np.allclose(mo.gelu(z), gt.detach().numpy(), rtol=1e-14, atol=1e-15)
+    assert np.allclose(mo.gelu_grad(z), zt.grad.numpy(), rtol=1e-13, atol=1e-15)
+    params = _mlp_params(rng, 16, 10)
+    x, tgt = _mlp_batch(rng, 40, 9, 16)
+    tgt[:3, 0] = 0.0
+    loss, _, y, _ = mo.loss_and_grads(params, x, tgt, lam=0.5)
+    yt = torch.tensor(y)
+    tt = torch.tensor(tgt)
+    ref = ((tt[:, 2] * (yt[:, 0] - tt[:, 0]) ** 2).sum()
+           + 0.5 * F.binary_cross_entropy_with_logits(yt[:, 1], tt[:, 1], weight=tt[:, 2],
+                                                      reduction="sum")) / tt[:, 2].sum()
+    assert abs(loss - float(ref)) < 1e-12 * abs(float(ref))
+    # Adam, two steps, against torch.optim.Adam in float64
+    w = rng.normal(size=50)
+    g1, g2 = rng.normal(size=50), rng.normal(size=50)
+    wt = torch.tensor(w.copy(), requires_grad=True)
+    opt = torch.optim.Adam([wt], lr=0.01, betas=(0.8, 0.95), eps=1e-7)
+    m = v = np.zeros(50)
+    ww = w.copy()
+    for step, g in enumerate((g1, g2), start=1):
+        wt.grad = torch.tensor(g)
+        opt.step()
+        ww, m, v = mo.adam(ww, g, m, v, 0.01, 0.8, 0.95, 1e-7, step)
+    assert np.allclose(ww, wt.detach().numpy(), rtol=1e-13, atol=1e-15)
+
+
+def test_bf16_round_matches_torch():
+    import torch
+    rng = np.random.default_rng(2)
+    x = np.concatenate([rng.normal(size=4000) * 10.0 ** rng.integers(-6, 6, size=4000),
+                        [0.0, -0.0, 1.0, -1.0, 65504.0, 1e-40]]).astype(np.float32)
+    # exact ties: low 16 bits 0x8000 on odd and even upper halves
+    ties = (rng.integers(0x3F00, 0x4100, size=200).astype(np.uint32) << 16) | 0x8000
+    x = np.concatenate([x, ties.view(np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(mo.bf16_round(x), ref)
+
+
+def test_mlp_encode_bits():
+    keys = np.array([0, 1, 0b1011, (1 << 39) | 5], dtype=np.uint64)
+    x = mo.encode(keys, 40, 48)
+    for i, k in enumerate(keys.tolist()):
+        bits = format(k, "040b")[::-1]
+        assert list(x[i, :40]) == [(-1.0 if b == "1" else 1.0) for b in bits]
+        assert not x[i, 40:].any()
