@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heisenberg28",
-                    choices=["heisenberg28", "random40", "small16", "batch64"])
+                    choices=["heisenberg28", "random40", "small16", "batch64", "netstart"])
     ap.add_argument("--instances", type=int, default=64, help="instances of batch64")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solution block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -94,6 +94,11 @@ def main():
     ap.add_argument("--group-split", type=int, default=1)
     ap.add_argument("--stage-v", type=int, default=0, help="stage v slices in LDS (0/1)")
     ap.add_argument("--bucket-cap", type=int, default=None, help="max keys per bucket")
+    ap.add_argument("--no-net", action="store_true", help="skip the network-start block")
+    ap.add_argument("--net-train", type=int, default=16,
+                    help="training Hamiltonians of the network-start block")
+    ap.add_argument("--net-test", type=int, default=3, help="held-out Hamiltonians")
+    ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -109,6 +114,9 @@ def main():
     dev = torch.device("cuda", 0)
     if args.workload == "batch64":
         bench_batch(args, dev)
+        return
+    if args.workload == "netstart":
+        bench_netstart(args, dev)
         return
     if args.calibrate_copy:
         # A plain copy of known size: 1 GiB read + 1 GiB written per launch.
@@ -220,6 +228,12 @@ def main():
 
     if not args.no_solve:
         record["time_to_solution"] = time_to_solution(prob, w, dev)
+    if not args.no_net and w.name == "heisenberg28":
+        record["network_start"] = {
+            "heisenberg28_2^20": network_start(dev, "heisenberg28_2^20", args.net_train,
+                                               args.net_test, args.net_steps),
+            "heisenberg16_sector": network_start(dev, "heisenberg16_sector", 6, 2,
+                                                 args.net_steps)}
     if not args.no_cpu:
         log("cpu baseline ...")
         record["cpu_baseline"] = cpu_baseline(w, j_s, args.cpu_groups, b_mv)
@@ -247,6 +261,18 @@ def time_to_solution(prob, w, dev):
                      "resid_rel": info.resid_norm / abs(info.eigval)}
         log(f"{name}: {dt:.2f} s, {info.niter} it, {info.matvecs} matvecs, E={info.eigval:.12f}, "
             f"r={info.resid_norm / abs(info.eigval):.2e}")
+    # the reference's default start: one-hot at the smallest diagonal entry
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    vec, info = prob.solve(prob.vinit_min_diag(), mode="fused", precision="c64",
+                           criterion="relative", tol=1e-6, maxiter=3000)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res["fused_c64_min_diag_start"] = {"seconds": round(dt, 3), "iterations": info.niter,
+                                       "matvecs": info.matvecs, "energy": info.eigval,
+                                       "converged": info.converged,
+                                       "resid_rel": info.resid_norm / abs(info.eigval)}
+    log(f"fused_c64 from the min-diagonal one-hot: {dt:.2f} s, {info.niter} it")
     # FP64 energy at a tight residual as the reference value
     vec, info = prob.solve(x0, mode="fused", precision="c128", criterion="relative", tol=1e-10,
                            maxiter=5000)
@@ -257,6 +283,127 @@ def time_to_solution(prob, w, dev):
     base = res["reference_c128"]["seconds"]
     res["speedup_fused_c64_vs_plain_fp64"] = round(base / res["fused_c64"]["seconds"], 2)
     return res
+
+
+NET_STARTS = ("gaussian", "min_diag", "network_bits", "network_bits_diag")
+
+
+def network_start(dev, family, n_train, n_test, steps):
+    """LOBPCG (fused, FP32 vectors, 1e-6 relative residual) on held-out
+    Hamiltonians of a family from four starts: a seeded Gaussian, the one-hot
+    at the smallest diagonal entry, the start network on bits, and the start
+    network on bits plus the scaled diagonal energy. Both networks are trained
+    on the FP64 ground vectors of ``n_train`` other members of the family."""
+    import functools
+    import multiprocessing as mp
+    import torch
+    from rqutils_amd import workloads as W
+    from rqutils_amd.sqd import SQDProblem
+    from rqutils_amd.startnet import StartNet, diag_feature, gaussian_start, ground_targets
+
+    if family == "heisenberg28_2^20":
+        nq, make = 28, functools.partial(W.heisenberg28, n_states=1 << 20)
+    elif family == "heisenberg16_sector":
+        nq, make = 16, functools.partial(W.heisenberg28, n_states=None, n_random=200,
+                                         num_qubits=16)
+    else:
+        raise ValueError(family)
+    seeds = list(range(1000, 1000 + n_train)) + list(range(2000, 2000 + n_test))
+    t = time.time()
+    with mp.get_context("spawn").Pool(min(8, len(seeds))) as pool:
+        ws = pool.map(make, seeds)
+    gen_s = time.time() - t
+    log(f"[{family}] {len(ws)} instances generated ({gen_s:.1f} s)")
+    keys, tgts, feats = [], [], []
+    torch.cuda.synchronize()
+    t = time.time()
+    for w in ws[:n_train]:
+        p = SQDProblem(w.hamiltonian, w.keys, nq, device=dev)
+        tg, _, info = ground_targets(p)
+        keys.append(p.space.keys)
+        tgts.append(tg)
+        feats.append(diag_feature(p))
+    torch.cuda.synchronize()
+    ground_s = time.time() - t
+    log(f"[{family}] {n_train} FP64 ground vectors ({ground_s:.1f} s)")
+    nets, train_s, final_loss = {}, {}, {}
+    for ne in (0, 1):
+        net = StartNet(nq, device=dev, seed=0, n_extra=ne)
+        t = time.time()
+        losses = net.train(keys, tgts, feats if ne else None, steps=steps, lr=3e-3,
+                           log_every=max(1, steps // 3))
+        torch.cuda.synchronize()
+        name = "network_bits_diag" if ne else "network_bits"
+        nets[name], train_s[name], final_loss[name] = net, time.time() - t, losses[-1]
+        log(f"[{family}] {name}: {steps} Adam steps ({train_s[name]:.1f} s), loss {losses[-1]:.4f}")
+    del keys, tgts, feats
+    per = {k: {"iterations": [], "seconds": [], "start_seconds": [], "overlap": [],
+               "energy_rel_err": [], "converged": 0} for k in NET_STARTS}
+    for w in ws[n_train:]:
+        p = SQDProblem(w.hamiltonian, w.keys, nq, device=dev)
+        _, psi, info = ground_targets(p)
+        e_ref = info.eigval
+        for name in NET_STARTS:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            if name == "gaussian":
+                x0 = gaussian_start(p.n, dev, w.seed)
+            elif name == "min_diag":
+                x0 = p.vinit_min_diag()
+            elif name == "network_bits":
+                x0 = nets[name].start_vector(p.space.keys)
+            else:
+                x0 = nets[name].start_vector(p.space.keys, diag_feature(p))
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            _, inf = p.solve(x0, mode="fused", precision="c64", criterion="relative", tol=1e-6,
+                             maxiter=3000)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            r = per[name]
+            r["iterations"].append(inf.niter)
+            r["seconds"].append(t2 - t0)
+            r["start_seconds"].append(t1 - t0)
+            r["overlap"].append(abs(torch.vdot(psi, x0.to(psi.dtype)).item())
+                                / (psi.norm().item() * x0.norm().item()))
+            r["energy_rel_err"].append(abs(inf.eigval - e_ref) / abs(e_ref))
+            r["converged"] += int(inf.converged)
+        log(f"[{family}] seed {w.seed}: " + ", ".join(
+            f"{k} {per[k]['iterations'][-1]} it" for k in NET_STARTS))
+    out = {"family": family, "train": n_train, "held_out": n_test, "states": ws[-1].n,
+           "generate_s": round(gen_s, 2), "ground_vectors_s": round(ground_s, 2),
+           "train_s": {k: round(v, 2) for k, v in train_s.items()},
+           "final_loss": {k: round(v, 5) for k, v in final_loss.items()}}
+    for k, r in per.items():
+        out[k] = {"iterations_mean": round(float(np.mean(r["iterations"])), 2),
+                  "iterations_median": float(np.median(r["iterations"])),
+                  "seconds_mean": round(float(np.mean(r["seconds"])), 4),
+                  "start_seconds_mean": round(float(np.mean(r["start_seconds"])), 4),
+                  "overlap_mean": round(float(np.mean(r["overlap"])), 4),
+                  "energy_rel_err_max": float(np.max(r["energy_rel_err"])),
+                  "converged": r["converged"]}
+    g = out["gaussian"]["seconds_mean"]
+    for k in NET_STARTS[1:]:
+        out[k]["speedup_vs_gaussian"] = round(g / out[k]["seconds_mean"], 2)
+    return out
+
+
+def bench_netstart(args, dev):
+    """The network-start protocol alone: ``--net-train`` training and
+    ``--net-test`` held-out 28-qubit Hamiltonians at 2^20 states."""
+    res = network_start(dev, "heisenberg28_2^20", args.net_train, args.net_test,
+                        args.net_steps)
+    best = res["network_bits_diag"]
+    record = {"metric": METRIC, "value": best["seconds_mean"], "unit": "s",
+              "n_gpus": 1, "steps": args.net_steps, "warmup": 0,
+              "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
+              "vs_baseline": None, "dtype": "bf16 network, c64 LOBPCG (fp64 Gram/RR)",
+              "data": "synthetic (seeded generator, rqutils_amd.workloads.heisenberg28)",
+              "config": {"workload": "netstart", "family": res["family"],
+                         "train": args.net_train, "held_out": args.net_test,
+                         "parallelism": "single GPU"},
+              "network_start": res}
+    print(json.dumps(record), flush=True)
 
 
 def bench_batch(args, dev):

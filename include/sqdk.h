@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 5
+#define SQDK_ABI_VERSION 6
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -327,9 +327,12 @@ int sqdk_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t l
                    float* C, int64_t ldc, const float* bias, int32_t ksplit, float* work,
                    size_t work_bytes, void* stream);
 
-/* out[i*kpad + q] = 1 - 2 bit_q(keys[i]) (bf16) for q < num_qubits, else 0. */
-int sqdk_mlp_encode(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t kpad,
-                    uint16_t* out, void* stream);
+/* Network input rows (bf16): out[i*kpad + q] = 1 - 2 bit_q(keys[i]) for
+ * q < num_qubits, extra[i*n_extra + q - num_qubits] for the next n_extra
+ * columns (per-state features, e.g. the scaled diagonal energy; extra may be
+ * NULL when n_extra = 0), 0 up to kpad. */
+int sqdk_mlp_encode(const uint64_t* keys, int64_t n, int32_t num_qubits, const float* extra,
+                    int32_t n_extra, int32_t kpad, uint16_t* out, void* stream);
 
 /* op 0: out = gelu(z); op 1: out = g * gelu'(z); op 2: out = z; GELU is the
  * exact erf form. Writes bf16 and/or FP32 outputs (either may be NULL). */

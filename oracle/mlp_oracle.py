@@ -26,12 +26,16 @@ def bf16_round(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def encode(keys, num_qubits, kpad):
-    """x[i, q] = 1 - 2 bit_q(key_i) for q < num_qubits, 0 up to kpad."""
+def encode(keys, num_qubits, kpad, extra=None):
+    """x[i, q] = 1 - 2 bit_q(key_i) for q < num_qubits, then the columns of
+    ``extra`` (n x n_extra, used as given), 0 up to kpad."""
     keys = np.asarray(keys, dtype=np.uint64)
     bits = ((keys[:, None] >> np.arange(num_qubits, dtype=np.uint64)) & np.uint64(1)).astype(float)
     x = np.zeros((keys.size, kpad))
     x[:, :num_qubits] = 1.0 - 2.0 * bits
+    if extra is not None:
+        extra = np.asarray(extra, dtype=float).reshape(keys.size, -1)
+        x[:, num_qubits:num_qubits + extra.shape[1]] = extra
     return x
 
 

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -139,8 +139,8 @@ SIGNATURES = {
                                  C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,
                                  C.c_int64, C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t,
                                  C.c_void_p]),
-    "sqdk_mlp_encode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
-                                  C.c_void_p]),
+    "sqdk_mlp_encode": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32,
+                                  C.c_int32, C.c_void_p, C.c_void_p]),
     "sqdk_mlp_act": (C.c_int, [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p]),
     "sqdk_mlp_loss": (C.c_int, [C.c_int64, C.c_void_p, C.c_int64, C.c_void_p, C.c_double,

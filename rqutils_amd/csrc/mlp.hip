@@ -97,16 +97,18 @@ gemm_reduce_kernel(int64_t M, int64_t N, int ksplit, const float* __restrict__ p
   C[i * ldc + j] = s;
 }
 
-// out[i][q] = 1 - 2 bit_q(key_i) for q < nq, 0 for nq <= q < kpad (bf16).
+// out[i][q] = 1 - 2 bit_q(key_i) for q < nq, extra[i][q - nq] for the next
+// n_extra columns, 0 up to kpad (bf16).
 __global__ void __launch_bounds__(kBlock)
-encode_kernel(const uint64_t* __restrict__ keys, int64_t n, int nq, int kpad,
-              uint16_t* __restrict__ out) {
+encode_kernel(const uint64_t* __restrict__ keys, int64_t n, int nq, const float* __restrict__ extra,
+              int n_extra, int kpad, uint16_t* __restrict__ out) {
   const int64_t idx = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= n * kpad) return;
   const int64_t i = idx / kpad;
   const int q = (int)(idx % kpad);
   float v = 0.f;
   if (q < nq) v = ((keys[i] >> q) & 1ull) ? -1.f : 1.f;
+  else if (q < nq + n_extra) v = extra[i * n_extra + (q - nq)];
   out[idx] = f2bf(v);
 }
 
@@ -229,13 +231,14 @@ int sqdk_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t l
   return SQDK_OK;
 }
 
-int sqdk_mlp_encode(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t kpad,
-                    uint16_t* out, void* stream) {
+int sqdk_mlp_encode(const uint64_t* keys, int64_t n, int32_t num_qubits, const float* extra,
+                    int32_t n_extra, int32_t kpad, uint16_t* out, void* stream) {
   SQDK_REQUIRE(keys && out, "NULL pointer argument");
-  SQDK_REQUIRE(n >= 1 && num_qubits >= 1 && num_qubits <= 64 && kpad >= num_qubits,
+  SQDK_REQUIRE(n_extra >= 0 && (n_extra == 0 || extra), "extra features missing");
+  SQDK_REQUIRE(n >= 1 && num_qubits >= 1 && num_qubits <= 64 && kpad >= num_qubits + n_extra,
                "bad encode shape");
-  encode_kernel<<<grid_for(n * kpad), kBlock, 0, as_stream(stream)>>>(keys, n, num_qubits, kpad,
-                                                                     out);
+  encode_kernel<<<grid_for(n * kpad), kBlock, 0, as_stream(stream)>>>(keys, n, num_qubits, extra,
+                                                                     n_extra, kpad, out);
   SQDK_LAUNCHED("encode_kernel");
   return SQDK_OK;
 }

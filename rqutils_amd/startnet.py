@@ -1,7 +1,9 @@
 """Learned LOBPCG starting vectors: a BF16 MLP trained on ground vectors.
 
-``StartNet`` maps a bitstring (bits as +-1) through 256 -> 256 -> 2 units
-(GELU) to a log-amplitude and a sign logit. Its weights live in FP32 with
+``StartNet`` maps a bitstring (bits as +-1, optionally followed by per-state
+features such as the scaled diagonal energy, ``diag_feature``) through
+256 -> 256 -> 2 units (GELU) to a log-amplitude and a sign logit. Its weights
+live in FP32 with
 BF16 copies; every product runs on the matrix cores through
 ``sqdk_gemm_bf16`` (FP32 accumulation), the activations, loss and Adam in
 the library's elementwise kernels. It is trained on ground vectors that the
@@ -18,7 +20,8 @@ import torch
 
 from . import _lib
 
-__all__ = ["StartNet", "targets_from_vector", "ground_targets", "gaussian_start"]
+__all__ = ["StartNet", "targets_from_vector", "ground_targets", "gaussian_start",
+           "diag_feature"]
 
 
 def _p(t):
@@ -59,13 +62,25 @@ def ground_targets(prob, *, tol: float = 1e-10, maxiter: int = 5000, seed: int =
     return targets_from_vector(vec), vec, info
 
 
+def diag_feature(prob) -> torch.Tensor:
+    """Per-state feature (n x 1, FP32): the diagonal energy <b|H|b> over the
+    subspace, shifted by its minimum and divided by its standard deviation,
+    so that the lowest-energy configuration of every subspace reads 0."""
+    d = prob.op.diagonal_of_group0().real.to(torch.float64)
+    sd = float(d.std().item()) if d.numel() > 1 else 1.0
+    f = (d - d.min()) / (sd if sd > 0 else 1.0)
+    return f.to(torch.float32).reshape(-1, 1).contiguous()
+
+
 class StartNet:
     """BF16 MLP (num_qubits -> hidden -> hidden -> 2) with MFMA kernels."""
 
     def __init__(self, num_qubits: int, hidden: int = 256, *, device=None, seed: int = 0,
-                 lam: float = 1.0):
+                 lam: float = 1.0, n_extra: int = 0):
         self.nq = int(num_qubits)
-        self.kpad = 16 * math.ceil(self.nq / 16)
+        self.n_extra = int(n_extra)
+        self.n_in = self.nq + self.n_extra
+        self.kpad = 16 * math.ceil(self.n_in / 16)
         self.h = int(hidden)
         self.lam = float(lam)
         self.device = torch.device(device if device is not None else "cuda")
@@ -76,8 +91,8 @@ class StartNet:
             return ((torch.rand(shape, generator=g, dtype=torch.float64) * 2 - 1) * bound).to(
                 torch.float32)
 
-        w1 = init(self.nq, (self.kpad, self.h))
-        w1[self.nq:] = 0
+        w1 = init(self.n_in, (self.kpad, self.h))
+        w1[self.n_in:] = 0
         self.params = [w1, torch.zeros(self.h), init(self.h, (self.h, self.h)),
                        torch.zeros(self.h), init(self.h, (self.h, 2)), torch.zeros(2)]
         self.params = [p.to(self.device).contiguous() for p in self.params]
@@ -124,20 +139,34 @@ class StartNet:
                                            self._stream()), "sqdk_mlp_act")
         return out16, out32
 
-    def encode(self, keys: torch.Tensor) -> torch.Tensor:
+    def _check_extra(self, keys, extra):
+        if self.n_extra == 0:
+            if extra is not None:
+                raise ValueError("this network takes no extra features")
+            return None
+        if extra is None or tuple(extra.shape) != (keys.numel(), self.n_extra):
+            raise ValueError(f"extra features must be ({keys.numel()}, {self.n_extra})")
+        if extra.dtype != torch.float32 or not extra.is_contiguous() or extra.device != self.device:
+            raise ValueError("extra features must be contiguous float32 on the network's device")
+        return extra
+
+    def encode(self, keys: torch.Tensor, extra: torch.Tensor | None = None) -> torch.Tensor:
         n = keys.numel()
+        extra = self._check_extra(keys, extra)
         x = torch.empty((n, self.kpad), dtype=torch.int16, device=self.device)
-        _lib.check(_lib.lib().sqdk_mlp_encode(keys.data_ptr(), n, self.nq, self.kpad, x.data_ptr(),
+        _lib.check(_lib.lib().sqdk_mlp_encode(keys.data_ptr(), n, self.nq, _p(extra),
+                                              self.n_extra, self.kpad, x.data_ptr(),
                                               self._stream()), "sqdk_mlp_encode")
         return x
 
     # -- passes ----------------------------------------------------------------
 
-    def forward(self, keys: torch.Tensor, keep: bool = False):
-        """Outputs y (n x 2, FP32) for uint64 keys (torch int64 tensor)."""
+    def forward(self, keys: torch.Tensor, extra: torch.Tensor | None = None, keep: bool = False):
+        """Outputs y (n x 2, FP32) for uint64 keys (torch int64 tensor) and,
+        with ``n_extra`` > 0, their n x n_extra features."""
         n = keys.numel()
         w1, b1, w2, b2, w3, b3 = self.params
-        x = self.encode(keys)
+        x = self.encode(keys, extra)
         z1 = torch.empty((n, self.h), dtype=torch.float32, device=self.device)
         self._gemm(n, self.h, self.kpad, x, self.kpad, 0, self.bf16[0], self.h, 0, z1, self.h, b1)
         h1, _ = self._act(0, z1)
@@ -150,14 +179,15 @@ class StartNet:
             self._cache = (x, z1, h1, z2, h2)
         return y
 
-    def loss_grads(self, keys: torch.Tensor, target: torch.Tensor, wsum: float | None = None):
+    def loss_grads(self, keys: torch.Tensor, target: torch.Tensor, wsum: float | None = None,
+                   extra: torch.Tensor | None = None):
         """Loss and gradients (into ``self.grads``) of one batch.
 
         ``target`` is n x 3 (log-amplitude, sign probability, weight) in FP32;
         the loss is normalised by ``wsum`` (default: the batch's weight sum).
         Returns the loss as a 0-d device tensor (no host synchronisation)."""
         n = keys.numel()
-        y = self.forward(keys, keep=True)
+        y = self.forward(keys, extra, keep=True)
         x, z1, h1, z2, h2 = self._cache
         if wsum is None:
             wsum = float(target[:, 2].sum().item())
@@ -202,20 +232,44 @@ class StartNet:
                                    v.data_ptr(), _p(self.bf16[i]), lr, beta1, beta2, eps,
                                    self.step_count, self._stream()), "sqdk_adam")
 
-    def train(self, keys_list, targets_list, *, steps: int = 3000, batch: int = 4096,
-              lr: float = 3e-3, seed: int = 0, log_every: int = 0):
-        """Adam with a cosine learning-rate schedule on (state, target) pairs
-        drawn uniformly from the concatenated instances. Each instance's weights
-        have mean 1, so the batch size normalises the loss. Returns the losses
+    def train(self, keys_list, targets_list, extra_list=None, *, steps: int = 3000,
+              batch: int = 4096, lr: float = 3e-3, seed: int = 0, log_every: int = 0,
+              floor: float = 0.05):
+        """Adam with a cosine learning-rate schedule.
+
+        Ground vectors are often concentrated on few states, so half of each
+        batch is drawn with probability proportional to the target weight and
+        half uniformly over the concatenated instances; each sample carries the
+        importance weight (w + floor) / (w/2 + 1/2) (weights have mean 1), an
+        unbiased estimate of the weighted loss plus ``floor`` times a uniform
+        one that keeps every state's amplitude supervised. Returns the losses
         (host floats) of every ``max(1, log_every)``-th step."""
         keys = torch.cat([k.to(self.device, torch.int64) for k in keys_list]).contiguous()
         tgts = torch.cat([t.to(self.device, torch.float32) for t in targets_list]).contiguous()
+        extras = None
+        if self.n_extra:
+            if extra_list is None:
+                raise ValueError("this network needs extra features for every instance")
+            extras = torch.cat([e.to(self.device, torch.float32) for e in extra_list]).contiguous()
+        n_all = keys.numel()
+        w_all = tgts[:, 2].to(torch.float64)
+        mean_w = float(w_all.mean().item())
+        cdf = torch.cumsum(w_all, 0)
+        total = float(cdf[-1].item())
         gen = torch.Generator(device=self.device).manual_seed(seed)
+        half = batch // 2
         losses = []
         every = max(1, log_every)
         for step in range(steps):
-            idx = torch.randint(0, keys.numel(), (batch,), generator=gen, device=self.device)
-            loss = self.loss_grads(keys[idx], tgts[idx].contiguous(), wsum=float(batch))
+            u = torch.rand(half, generator=gen, device=self.device, dtype=torch.float64) * total
+            i1 = torch.searchsorted(cdf, u).clamp_(max=n_all - 1)
+            i2 = torch.randint(0, n_all, (batch - half,), generator=gen, device=self.device)
+            idx = torch.cat([i1, i2])
+            tb = tgts[idx]
+            wi = w_all[idx] / mean_w
+            tb[:, 2] = ((wi + floor) / (0.5 * wi + 0.5)).to(torch.float32)
+            loss = self.loss_grads(keys[idx], tb.contiguous(), wsum=float(batch),
+                                   extra=None if extras is None else extras[idx].contiguous())
             self.adam_step(lr=lr * 0.5 * (1.0 + math.cos(math.pi * step / steps)))
             if step % every == 0 or step == steps - 1:
                 losses.append(float(loss.item()))
@@ -223,19 +277,27 @@ class StartNet:
                     print(f"[startnet] step {step} loss {losses[-1]:.4f}", flush=True)
         return losses
 
-    def sign_accuracy(self, keys: torch.Tensor, target: torch.Tensor) -> float:
+    def predict(self, keys: torch.Tensor, extra: torch.Tensor | None = None,
+                chunk: int = 1 << 18) -> torch.Tensor:
+        """Outputs y (n x 2, FP32) over any number of states, in chunks."""
+        outs = []
+        for s in range(0, keys.numel(), chunk):
+            e = None if extra is None else extra[s:s + chunk].contiguous()
+            outs.append(self.forward(keys[s:s + chunk].contiguous(), e))
+        return torch.cat(outs)
+
+    def sign_accuracy(self, keys: torch.Tensor, target: torch.Tensor,
+                      extra: torch.Tensor | None = None) -> float:
         """Weighted fraction of states whose predicted sign matches the target."""
-        y = self.forward(keys)
+        y = self.predict(keys, extra)
         ok = ((y[:, 1] > 0) == (target[:, 1] > 0.5)).to(torch.float64)
         w = target[:, 2].to(torch.float64)
         return float((ok * w).sum() / w.sum())
 
-    def start_vector(self, keys: torch.Tensor, chunk: int = 1 << 18) -> torch.Tensor:
+    def start_vector(self, keys: torch.Tensor, extra: torch.Tensor | None = None,
+                     chunk: int = 1 << 18) -> torch.Tensor:
         """Normalised complex128 start vector exp(y0) sign(y1) over the keys."""
-        outs = []
-        for s in range(0, keys.numel(), chunk):
-            outs.append(self.forward(keys[s:s + chunk].contiguous()))
-        y = torch.cat(outs).to(torch.float64)
+        y = self.predict(keys, extra, chunk).to(torch.float64)
         a = y[:, 0] - y[:, 0].max()
         x = torch.exp(a) * torch.where(y[:, 1] > 0, 1.0, -1.0)
         x = x / torch.linalg.vector_norm(x)

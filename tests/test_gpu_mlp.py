@@ -90,14 +90,19 @@ def test_gemm_bf16_rejects_bad_arguments(gpu):
                                     y.data_ptr(), 16, None, 1, None, 0, _lib.stream_ptr(gpu)), "g")
 
 
-def test_encode_bit_exact(gpu):
+@pytest.mark.parametrize("n_extra", [0, 2])
+def test_encode_bit_exact(gpu, n_extra):
     rng = np.random.default_rng(5)
     keys = rng.integers(0, 1 << 40, size=3001, dtype=np.uint64)
     kt = torch.from_numpy(keys.view(np.int64)).to(gpu)
+    extra = rng.normal(size=(keys.size, n_extra)).astype(np.float32)
+    et = _f32(extra, gpu)
     out = torch.empty((keys.size, 48), dtype=torch.int16, device=gpu)
-    _lib.check(_lib.lib().sqdk_mlp_encode(kt.data_ptr(), keys.size, 40, 48, out.data_ptr(),
-                                          _lib.stream_ptr(gpu)), "encode")
-    assert np.array_equal(_bf16_host(out), mo.encode(keys, 40, 48))
+    _lib.check(_lib.lib().sqdk_mlp_encode(kt.data_ptr(), keys.size, 40,
+                                          et.data_ptr() if n_extra else None, n_extra, 48,
+                                          out.data_ptr(), _lib.stream_ptr(gpu)), "encode")
+    ref = mo.encode(keys, 40, 48, extra=mo.bf16_round(extra) if n_extra else None)
+    assert np.array_equal(_bf16_host(out), ref)
 
 
 def test_activation_kernels(gpu):
@@ -175,10 +180,11 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def test_startnet_forward_backward_matches_oracle(gpu):
+@pytest.mark.parametrize("n_extra", [0, 1])
+def test_startnet_forward_backward_matches_oracle(gpu, n_extra):
     from rqutils_amd.startnet import StartNet
     nq, n = 20, 3000
-    net = StartNet(nq, hidden=256, device=gpu, seed=3, lam=0.7)
+    net = StartNet(nq, hidden=256, device=gpu, seed=3, lam=0.7, n_extra=n_extra)
     rng = np.random.default_rng(9)
     for i in (1, 3, 5):        # non-zero biases so that every term is exercised
         net.params[i].copy_(_f32(0.1 * rng.normal(size=tuple(net.params[i].shape)), gpu))
@@ -186,21 +192,23 @@ def test_startnet_forward_backward_matches_oracle(gpu):
     keys = rng.integers(0, 1 << nq, size=n, dtype=np.uint64)
     tgt = np.stack([rng.normal(size=n), rng.integers(0, 2, n), rng.uniform(0.1, 3, n)],
                    axis=1).astype(np.float32)
+    extra = rng.uniform(0, 3, size=(n, n_extra)).astype(np.float32)
     kt = torch.from_numpy(keys.view(np.int64)).to(gpu)
     tt = _f32(tgt, gpu)
-    loss = float(net.loss_grads(kt, tt).item())
-    y_dev = net.forward(kt).cpu().numpy().astype(np.float64)
+    et = _f32(extra, gpu) if n_extra else None
+    loss = float(net.loss_grads(kt, tt, extra=et).item())
+    y_dev = net.forward(kt, et).cpu().numpy().astype(np.float64)
     grads_dev = [g.cpu().numpy().astype(np.float64) for g in net.grads]
     p = [t.cpu().numpy().astype(np.float64) for t in net.params]
     pb = [mo.bf16_round(p[0]), p[1], mo.bf16_round(p[2]), p[3], mo.bf16_round(p[4]), p[5]]
-    x = mo.encode(keys, nq, net.kpad)
+    x = mo.encode(keys, nq, net.kpad, extra=mo.bf16_round(extra) if n_extra else None)
     td = tgt.astype(np.float64)
     loss_o, grads_o, y_o, _ = mo.loss_and_grads(pb, x, td, lam=0.7, rounding=mo.bf16_round)
     assert _rel(y_dev, y_o) < 1e-4
     assert abs(loss - loss_o) < 1e-4 * abs(loss_o)
     for k, (gd, go) in enumerate(zip(grads_dev, grads_o)):
         if k == 0:   # W1 padding rows see zero inputs
-            assert not gd[nq:].any()
+            assert not gd[nq + n_extra:].any()
         assert _rel(gd, go) < 2e-3, (k, _rel(gd, go))
     _, grads_x, _, _ = mo.loss_and_grads(pb, x, td, lam=0.7)
     for k, (gd, gx) in enumerate(zip(grads_dev, grads_x)):
@@ -212,7 +220,8 @@ def test_startnet_start_beats_gaussian(gpu):
     the FP64 LOBPCG), then start the FP32/FP64 LOBPCG on two held-out chains
     from the network and from a Gaussian vector: both reach the FP64 energy
     within 1e-6 relative; the network start needs at most 3/4 of the
-    iterations."""
+    iterations of the Gaussian start and no more than the one-hot start at the
+    smallest diagonal entry (the reference's default)."""
     from rqutils_amd import workloads as W
     from rqutils_amd.sqd import SQDProblem
     from rqutils_amd.startnet import StartNet, gaussian_start, ground_targets
@@ -240,7 +249,8 @@ def test_startnet_start_beats_gaussian(gpu):
         overlap = abs(torch.vdot(psi, x_net).item()) / psi.norm().item()
         acc = net.sign_accuracy(prob.space.keys, t)
         iters = {}
-        for name, x0 in (("net", x_net), ("gauss", gaussian_start(prob.n, gpu, seed))):
+        for name, x0 in (("net", x_net), ("gauss", gaussian_start(prob.n, gpu, seed)),
+                         ("min_diag", prob.vinit_min_diag())):
             _, inf = prob.solve(x0, mode="fused", precision="c64", criterion="relative",
                                 tol=1e-5, maxiter=2000)
             assert inf.converged, name
@@ -249,3 +259,4 @@ def test_startnet_start_beats_gaussian(gpu):
         print(f"seed {seed}: overlap {overlap:.4f} sign acc {acc:.4f} iters {iters}", flush=True)
         assert overlap > 0.8
         assert iters["net"] <= 0.75 * iters["gauss"]
+        assert iters["net"] <= iters["min_diag"]
