@@ -85,6 +85,30 @@ __host__ __device__ inline LdsLayout lds_layout(int fine_bits, int max_bucket, s
   return l;
 }
 
+// sum_k c_k (-1)^{popcount(z_k & s_r)} for every row r of a thread, terms in
+// index order (the order of group_coef, so results are identical). The term
+// tables are read once per term for all rows.
+template <typename CT, typename KT, int ROWS>
+__device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* __restrict__ coef,
+                                 int t0, int t1, uint64_t bucket_base, const KT (&sl)[ROWS],
+                                 decltype(CT::x) (&cr)[ROWS], decltype(CT::x) (&ci)[ROWS]) {
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    cr[r] = 0;
+    ci[r] = 0;
+  }
+  for (int k = t0; k < t1; ++k) {
+    const uint64_t z = zmask[k];
+    const CT c = coef[k];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const bool odd = (__popcll(z & (bucket_base | (uint64_t)sl[r])) & 1) != 0;
+      cr[r] += odd ? -c.x : c.x;
+      ci[r] += odd ? -c.y : c.y;
+    }
+  }
+}
+
 template <typename VT, typename KT, int ROWS, bool STAGE_V>
 __global__ void __launch_bounds__(kBlock)
 bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
@@ -144,14 +168,14 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   // Waves of a short chunk that hold no row skip the per-group work.
   const bool wave_active = __any(valid[0]);
 
-  // X = 0 group: the diagonal (first share only).
-  if (gs == 0 && D.diag_t1 > D.diag_t0) {
+  // X = 0 group: the diagonal (first share only), one pass over its terms
+  // for all rows of the thread.
+  if (gs == 0 && D.diag_t1 > D.diag_t0 && wave_active) {
+    RT cr[ROWS], ci[ROWS];
+    rows_coef<CT, KT, ROWS>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base, sl, cr, ci);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      if (!valid[r]) continue;
-      RT cr, ci;
-      group_coef<CT>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base | (uint64_t)sl[r], cr, ci);
-      acc[r].fma(cr, ci, v[row0 + tid + r * kBlock]);
+      if (valid[r]) acc[r].fma(cr[r], ci[r], v[row0 + tid + r * kBlock]);
     }
   }
 
@@ -197,19 +221,19 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     for (int g = gfirst; g < g1; g += a.split) {
       const KT xl = (KT)(xmask[g] & lowmask);
       KT tl[ROWS];
-      int base[ROWS], n[ROWS], n0[ROWS];
-      bool ok[ROWS];
+      int base[ROWS], n[ROWS];
+      bool live[ROWS];
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         tl[r] = sl[r] ^ xl;
-        ok[r] = valid[r];
         const uint32_t pr = lpair[(uint32_t)(tl[r] >> f)];
         base[r] = (int)(pr & 0xFFFFu);
-        n0[r] = (int)(pr >> 16) - base[r];
-        n[r] = n0[r];
+        n[r] = (int)(pr >> 16) - base[r];
+        live[r] = valid[r] && n[r] > 0;
       }
-      // Branch-free search for the last key <= target: after ceil(log2(n0))
+      // Branch-free search for the last key <= target: after ceil(log2(n))
       // halvings base is that key, so one equality test decides membership.
+      // base stays in [lo, qn], and lkeys[qn] exists, so every read is in range.
       for (int st = 0; st < steps; ++st) {
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
@@ -219,23 +243,24 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
         }
       }
       int idx[ROWS];
+      bool any_hit = false;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r)
-        idx[r] = (ok[r] && n0[r] > 0 && lkeys[base[r]] == tl[r]) ? base[r] : -1;
-      // Hits: gather first (all rows), then coefficients and accumulation.
+      for (int r = 0; r < ROWS; ++r) {
+        const KT kb = lkeys[base[r]];
+        idx[r] = (live[r] && kb == tl[r]) ? base[r] : -1;
+        any_hit |= idx[r] >= 0;
+      }
+      if (!__any(any_hit)) continue;
+      // Hits: gather (a miss reads the slice's first element, in range and
+      // shared by the wave), then one pass over the group's terms for all rows.
       VT vv[ROWS];
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        if (idx[r] >= 0) vv[r] = STAGE_V ? lv[idx[r]] : vq[idx[r]];
-      }
-      const int t0 = grp_off[g], t1 = grp_off[g + 1];
+      for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
+      RT cr[ROWS], ci[ROWS];
+      rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
-        if (idx[r] >= 0) {
-          RT cr, ci;
-          group_coef<CT>(zmask, coef, t0, t1, bucket_base | (uint64_t)sl[r], cr, ci);
-          acc[r].fma(cr, ci, vv[r]);
-        }
+        if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
       }
     }
   }
