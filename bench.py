@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--group-split", type=int, default=1)
     ap.add_argument("--stage-v", type=int, default=0, help="stage v slices in LDS (0/1)")
     ap.add_argument("--bucket-cap", type=int, default=None, help="max keys per bucket")
+    ap.add_argument("--dense-bits", type=int, default=0,
+                    help="dense lookup at this bucket width (0: autotune or search)")
     ap.add_argument("--no-net", action="store_true", help="skip the network-start block")
     ap.add_argument("--net-train", type=int, default=16,
                     help="training Hamiltonians of the network-start block")
@@ -131,15 +133,18 @@ def main():
     w = W.make(args.workload)
     log(f"workload {w.name}: {w.n} states, {len(w.labels)} terms ({time.time() - t:.1f} s)")
     t = time.time()
-    if args.bucket_cap or args.rows_per_thread:
-        space_kw = {"rows_per_thread": args.rows_per_thread or 4}
+    if args.bucket_cap or args.rows_per_thread or args.dense_bits:
+        space_kw = {"rows_per_thread": args.rows_per_thread or 4, "dense": False}
         if args.bucket_cap:
             space_kw["bucket_cap"] = args.bucket_cap
+        if args.dense_bits:
+            space_kw.update(dense=True, bucket_bits=args.dense_bits)
         prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev,
                           group_split=args.group_split, **space_kw)
     else:
         prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, tune="auto")
         log(f"autotune: {[t for t in prob.tuned.table]} -> cap={prob.tuned.bucket_cap} "
+            f"dense_bits={prob.tuned.dense_bits} "
             f"rows={prob.tuned.rows_per_thread} split={prob.tuned.group_split} "
             f"({time.time() - t:.1f} s incl. tuning)")
     prob.op.tune.stage_v = args.stage_v
@@ -211,6 +216,7 @@ def main():
                    "groups_pruned_odd": op.host.pruned_groups, "classes": op.n_classes,
                    "hits": hits, "bucket_bits": sp.bucket_bits, "fine_bits": sp.fine_bits,
                    "rows_per_thread": sp.rows_per_thread, "group_split": op.tune.group_split,
+                   "lookup": "dense bitmap" if sp.dense else "search",
                    "stage_v": args.stage_v, "autotuned": prob.tuned is not None,
                    "parallelism": "single GPU"},
         "matvecs_per_s": round(1e3 / ms, 2),
@@ -221,7 +227,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": read_traffic(os.path.join(
                          ROOT, "profiles", f"bucket_join_kernel_{w.name}.md")),
-                     "kernel": f"bucket_join_kernel<float2,uint32_t,{sp.rows_per_thread},{bool(args.stage_v)}>",
+                     "kernel": f"bucket_join_kernel<float2,uint32_t,{sp.rows_per_thread},"
+                               f"{str(bool(args.stage_v)).lower()},{str(sp.dense).lower()}>",
                      "bytes_per_launch": b_mv},
         "wall_s_timed": round(wall, 3),
     }

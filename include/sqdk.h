@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 6
+#define SQDK_ABI_VERSION 7
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -67,6 +67,10 @@ typedef struct sqdk_space {
   int32_t chunk_rows;      /* rows per item; items never cross a bucket */
   int32_t n_items;
   int32_t max_bucket;      /* keys in the largest bucket (<= 16384) */
+  const uint32_t* fbits;   /* NULL, or with fine_bits == bucket_bits - 5 the occupancy
+                              bitmap of every 32-value sub-bucket (same index as fdir):
+                              the matvec then looks targets up in O(1) instead of
+                              searching (sqdk_space_bits) */
 } sqdk_space;
 
 /* Pauli sum grouped by X mask and ordered by class (device pointers).
@@ -170,6 +174,15 @@ int sqdk_space_dir(const uint64_t* keys, int64_t n, int32_t num_qubits,
 int sqdk_space_fine(const uint64_t* keys, int64_t n, int32_t num_qubits,
                     int32_t bucket_bits, int32_t fine_bits,
                     const uint32_t* bstart, uint16_t* fdir, void* stream);
+
+/* Occupancy bitmaps for the dense lookup: with fine_bits == bucket_bits - 5,
+ * fbits[e] has bit j set iff key (e << 5) | j is in the subspace, for every
+ * fine-directory entry e (2^(num_qubits - 5) entries; needs fdir built first).
+ * No counterpart in the reference (its lookup is a sort-join,
+ * rqutils/sqd.py:448-503). */
+int sqdk_space_bits(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t bucket_bits,
+                    int32_t fine_bits, const uint32_t* bstart, const uint16_t* fdir,
+                    uint32_t* fbits, void* stream);
 
 /* Row chunks of at most chunk_rows rows that never cross a bucket boundary:
  * items[k] = first row of chunk k; the count goes to n_items_out (device

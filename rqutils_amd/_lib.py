@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -28,7 +28,7 @@ class SqdkSpace(C.Structure):
         ("items", C.c_void_p), ("pc_mask", C.c_void_p),
         ("n", C.c_int64), ("num_qubits", C.c_int32), ("bucket_bits", C.c_int32),
         ("fine_bits", C.c_int32), ("chunk_rows", C.c_int32), ("n_items", C.c_int32),
-        ("max_bucket", C.c_int32),
+        ("max_bucket", C.c_int32), ("fbits", C.c_void_p),
     ]
 
 
@@ -102,6 +102,8 @@ SIGNATURES = {
                                  C.c_void_p]),
     "sqdk_space_fine": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sqdk_space_bits": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "sqdk_space_items": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int64,
                                    C.c_void_p, C.c_void_p]),
     "sqdk_xsource": (C.c_int, [C.POINTER(SqdkSpace), C.c_uint64, C.c_void_p, C.c_void_p]),

@@ -7,7 +7,9 @@ Three knobs decide the matvec's speed on a given problem:
 * the rows per thread (a row chunk is 256 x this), which trades staging reuse
   against idle lanes on small buckets;
 * the group split, which deals a chunk's groups to several workgroups so that
-  small subspaces still fill the 256 CUs (partial sums are reduced after).
+  small subspaces still fill the 256 CUs (partial sums are reduced after);
+* the lookup: a search in the staged keys, or, where its directory fits
+  (``dense_fits``), the dense bitmap lookup at a few bucket widths.
 
 ``autotune`` times every candidate with HIP events on the real problem (a few
 matvecs each) and returns the fastest; results are memoised per problem
@@ -25,13 +27,14 @@ import numpy as np
 import torch
 
 from . import _lib
-from .operator import PauliOperator, Subspace
+from .operator import PauliOperator, Subspace, dense_fits
 from .paulis.symplectic import PauliSumXZ
 
 __all__ = ["TuneResult", "autotune", "signature", "DEFAULT_CAPS", "DEFAULT_ROWS"]
 
 DEFAULT_CAPS = (512, 1024, 2048, 4096, 8192)
 DEFAULT_ROWS = (4, 8)
+DEFAULT_DENSE_BITS = (15, 16, 17)
 CU_COUNT = 256
 
 _MEMO: dict = {}
@@ -44,10 +47,15 @@ class TuneResult:
     group_split: int
     ms: float
     precision: str
-    table: list = field(default_factory=list)   # (cap, rows, split, ms) of every candidate
+    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits) per candidate
+    dense_bits: int = 0                         # bucket bits of the dense lookup, 0: search
 
     def space_kw(self) -> dict:
-        return {"bucket_cap": self.bucket_cap, "rows_per_thread": self.rows_per_thread}
+        if self.dense_bits:
+            return {"dense": True, "bucket_bits": self.dense_bits,
+                    "rows_per_thread": self.rows_per_thread}
+        return {"bucket_cap": self.bucket_cap, "rows_per_thread": self.rows_per_thread,
+                "dense": False}
 
 
 def signature(hamiltonian: PauliSumXZ, keys: torch.Tensor, num_qubits: int, precision: str) -> str:
@@ -109,9 +117,10 @@ def _split_candidates(n_items: int):
 
 
 def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str = "c64",
-             caps=DEFAULT_CAPS, rows=DEFAULT_ROWS, reps: int = 3, use_cache: bool = True,
-             verbose: bool = False) -> TuneResult:
-    """Fastest (bucket cap, rows per thread, group split) for this problem."""
+             caps=DEFAULT_CAPS, rows=DEFAULT_ROWS, dense_bits=DEFAULT_DENSE_BITS, reps: int = 3,
+             use_cache: bool = True, verbose: bool = False) -> TuneResult:
+    """Fastest (lookup, bucket cap or width, rows per thread, group split) for
+    this problem."""
     if not isinstance(hamiltonian, PauliSumXZ):
         hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True, add_padding=True)
     dev = torch.device(device if device is not None else "cuda")
@@ -135,29 +144,37 @@ def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str 
     table = []
     best = None
     seen_m = {}
-    for cap in sorted(set(int(c) for c in caps)):
-        for r in rows:
-            try:
+    candidates = [(int(c), r, 0) for c in sorted(set(int(c) for c in caps)) for r in rows]
+    if dense_fits(num_qubits, n):
+        lo_m, hi_m = max(5, num_qubits - 26), min(num_qubits, 19)
+        candidates += [(0, r, int(m)) for m in dense_bits if lo_m <= m <= hi_m for r in rows]
+    for cap, r, dm in candidates:
+        try:
+            if dm:
+                sp = Subspace(kt, num_qubits, device=dev, dense=True, bucket_bits=dm,
+                              rows_per_thread=r)
+            else:
                 sp = Subspace(kt, num_qubits, device=dev, bucket_cap=cap, rows_per_thread=r)
-            except ValueError:
-                continue            # keys too concentrated for this cap
-            if seen_m.get((sp.bucket_bits, r)):
-                continue            # same bucket level as a smaller cap: same kernel
-            seen_m[(sp.bucket_bits, r)] = True
-            op = PauliOperator(hamiltonian, sp)
-            for split in _split_candidates(sp.n_items):
-                op.set_tune(group_split=split)
-                ms = _time_matvec(op, v, reps)
-                table.append((cap, r, split, round(ms, 4)))
-                if verbose:
-                    print(f"[autotune] cap={cap} m={sp.bucket_bits} rows={r} split={split}: "
-                          f"{ms:.3f} ms", flush=True)
-                if best is None or ms < best[3]:
-                    best = (cap, r, split, ms)
-            del op, sp
+        except ValueError:
+            continue            # keys too concentrated for this cap
+        key = (sp.bucket_bits, r, sp.dense)
+        if seen_m.get(key):
+            continue            # same bucket level as a smaller cap: same kernel
+        seen_m[key] = True
+        op = PauliOperator(hamiltonian, sp)
+        for split in _split_candidates(sp.n_items):
+            op.set_tune(group_split=split)
+            ms = _time_matvec(op, v, reps)
+            table.append((cap, r, split, round(ms, 4), dm))
+            if verbose:
+                print(f"[autotune] {'dense' if dm else 'cap=%d' % cap} m={sp.bucket_bits} "
+                      f"rows={r} split={split}: {ms:.3f} ms", flush=True)
+            if best is None or ms < best[3]:
+                best = (cap, r, split, ms, dm)
+        del op, sp
     if best is None:
         raise ValueError("no matvec configuration fits this subspace")
-    res = TuneResult(best[0], best[1], best[2], best[3], precision, table)
+    res = TuneResult(best[0], best[1], best[2], best[3], precision, table, best[4])
     _MEMO[sig] = res
     _save_disk(sig, res)
     return res

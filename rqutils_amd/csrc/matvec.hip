@@ -59,6 +59,7 @@ struct MatvecTables {
   VT* out;
   const sqdk_inst* descs;
   const uint16_t* item_inst;
+  const uint32_t* fbits;     // dense lookup bitmaps (or NULL: search)
 };
 
 // Workgroup i runs on XCD i % 8; give each XCD a contiguous range of items so
@@ -109,10 +110,11 @@ __device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* _
   }
 }
 
-template <typename VT, typename KT, int ROWS, bool STAGE_V>
+template <typename VT, typename KT, int ROWS, bool STAGE_V, bool DENSE>
 __global__ void __launch_bounds__(kBlock)
 bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
-                   const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ items,
+                   const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ fbits,
+                   const uint32_t* __restrict__ items,
                    const uint64_t* __restrict__ xmask,
                    const int32_t* __restrict__ grp_off, const uint64_t* __restrict__ zmask,
                    const typename CoefT<VT>::type* __restrict__ coef,
@@ -147,6 +149,9 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   int* s_occ = reinterpret_cast<int*>(smem + lay.occ);
   KT* lkeys = reinterpret_cast<KT*>(smem + lay.keys);
   VT* lv = reinterpret_cast<VT*>(smem + lay.vals);
+  // Dense lookup: (occupancy bitmap, first-key offset) of every 32-value
+  // sub-bucket of the staged target bucket.
+  uint2* ldir = reinterpret_cast<uint2*>(smem);
 
   const uint32_t row0 = items[w];
   const uint64_t bucket = keys[row0] >> m;
@@ -192,75 +197,110 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     const uint32_t qe = bst[q + 1];
     if (qs == qe) continue;  // empty target bucket: no group of the class can hit
     const int qn = (int)(qe - qs);
-    __syncthreads();  // the previous class is done with the LDS tables
-    for (int i = tid; i < qn; i += kBlock) lkeys[i] = (KT)(keys[qs + i] & lowmask);
-    if (tid == 0) lkeys[qn] = (KT)0;
-    if (STAGE_V) {
-      for (int i = tid; i < qn; i += kBlock) lv[i] = v[qs + i];
-    }
-    // Sub-bucket ranges as (lo | hi << 16) and the largest occupancy.
-    const uint64_t fbase = q << d;
-    int occ = 0;
-    for (int e = tid; e < ndir; e += kBlock) {
-      const uint32_t lo = fdr[fbase + e];
-      const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
-      lpair[e] = lo | (hi << 16);
-      occ = max(occ, (int)(hi - lo));
-    }
-#pragma unroll
-    for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
-    if ((tid & (kWave - 1)) == 0) s_occ[tid / kWave] = occ;
-    __syncthreads();
-    int maxocc = 0;
-#pragma unroll
-    for (int i = 0; i < kBlock / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
-    // Steps of the branch-free lower bound: ceil(log2(maxocc)).
-    const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
-    if (!wave_active) continue;  // the barriers above are still reached by every wave
-    const VT* __restrict__ vq = v + qs;
-    for (int g = gfirst; g < g1; g += a.split) {
-      const KT xl = (KT)(xmask[g] & lowmask);
-      KT tl[ROWS];
-      int base[ROWS], n[ROWS];
-      bool live[ROWS];
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        tl[r] = sl[r] ^ xl;
-        const uint32_t pr = lpair[(uint32_t)(tl[r] >> f)];
-        base[r] = (int)(pr & 0xFFFFu);
-        n[r] = (int)(pr >> 16) - base[r];
-        live[r] = valid[r] && n[r] > 0;
-      }
-      // Branch-free search for the last key <= target: after ceil(log2(n))
-      // halvings base is that key, so one equality test decides membership.
-      // base stays in [lo, qn], and lkeys[qn] exists, so every read is in range.
-      for (int st = 0; st < steps; ++st) {
+    if constexpr (DENSE) {
+      __syncthreads();  // the previous class is done with the LDS table
+      const uint64_t fbase = q << d;
+      for (int e = tid; e < ndir; e += kBlock) ldir[e] = make_uint2(fbits[fbase + e], fdr[fbase + e]);
+      __syncthreads();
+      if (!wave_active) continue;  // the barriers above are still reached by every wave
+      const VT* __restrict__ vq = v + qs;
+      for (int g = gfirst; g < g1; g += a.split) {
+        const uint32_t xl = (uint32_t)(xmask[g] & lowmask);
+        int idx[ROWS];
+        bool any_hit = false;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-          const int half = n[r] >> 1;
-          base[r] = lkeys[base[r] + half] <= tl[r] ? base[r] + half : base[r];
-          n[r] -= half;
+          // sub-bucket of 32 values: one LDS read, a bit test and a rank
+          const uint32_t t = (uint32_t)sl[r] ^ xl;
+          const uint2 pr = ldir[t >> 5];
+          const uint32_t b = t & 31u;
+          const bool hit = valid[r] && ((pr.x >> b) & 1u);
+          idx[r] = hit ? (int)(pr.y + __popc(pr.x & ((1u << b) - 1u))) : -1;
+          any_hit |= hit;
+        }
+        if (!__any(any_hit)) continue;
+        VT vv[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) vv[r] = vq[max(idx[r], 0)];
+        RT cr[ROWS], ci[ROWS];
+        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
         }
       }
-      int idx[ROWS];
-      bool any_hit = false;
-#pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        const KT kb = lkeys[base[r]];
-        idx[r] = (live[r] && kb == tl[r]) ? base[r] : -1;
-        any_hit |= idx[r] >= 0;
+      (void)qn;
+    } else {
+      __syncthreads();  // the previous class is done with the LDS tables
+      for (int i = tid; i < qn; i += kBlock) lkeys[i] = (KT)(keys[qs + i] & lowmask);
+      if (tid == 0) lkeys[qn] = (KT)0;
+      if (STAGE_V) {
+        for (int i = tid; i < qn; i += kBlock) lv[i] = v[qs + i];
       }
-      if (!__any(any_hit)) continue;
-      // Hits: gather (a miss reads the slice's first element, in range and
-      // shared by the wave), then one pass over the group's terms for all rows.
-      VT vv[ROWS];
+      // Sub-bucket ranges as (lo | hi << 16) and the largest occupancy.
+      const uint64_t fbase = q << d;
+      int occ = 0;
+      for (int e = tid; e < ndir; e += kBlock) {
+        const uint32_t lo = fdr[fbase + e];
+        const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
+        lpair[e] = lo | (hi << 16);
+        occ = max(occ, (int)(hi - lo));
+      }
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
-      RT cr[ROWS], ci[ROWS];
-      rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
+      for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
+      if ((tid & (kWave - 1)) == 0) s_occ[tid / kWave] = occ;
+      __syncthreads();
+      int maxocc = 0;
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) {
-        if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
+      for (int i = 0; i < kBlock / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
+      // Steps of the branch-free lower bound: ceil(log2(maxocc)).
+      const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
+      if (!wave_active) continue;  // the barriers above are still reached by every wave
+      const VT* __restrict__ vq = v + qs;
+      for (int g = gfirst; g < g1; g += a.split) {
+        const KT xl = (KT)(xmask[g] & lowmask);
+        KT tl[ROWS];
+        int base[ROWS], n[ROWS];
+        bool live[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          tl[r] = sl[r] ^ xl;
+          const uint32_t pr = lpair[(uint32_t)(tl[r] >> f)];
+          base[r] = (int)(pr & 0xFFFFu);
+          n[r] = (int)(pr >> 16) - base[r];
+          live[r] = valid[r] && n[r] > 0;
+        }
+        // Branch-free search for the last key <= target: after ceil(log2(n))
+        // halvings base is that key, so one equality test decides membership.
+        // base stays in [lo, qn], and lkeys[qn] exists, so every read is in range.
+        for (int st = 0; st < steps; ++st) {
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r) {
+            const int half = n[r] >> 1;
+            base[r] = lkeys[base[r] + half] <= tl[r] ? base[r] + half : base[r];
+            n[r] -= half;
+          }
+        }
+        int idx[ROWS];
+        bool any_hit = false;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          const KT kb = lkeys[base[r]];
+          idx[r] = (live[r] && kb == tl[r]) ? base[r] : -1;
+          any_hit |= idx[r] >= 0;
+        }
+        if (!__any(any_hit)) continue;
+        // Hits: gather (a miss reads the slice's first element, in range and
+        // shared by the wave), then one pass over the group's terms for all rows.
+        VT vv[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
+        RT cr[ROWS], ci[ROWS];
+        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+          if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
+        }
       }
     }
   }
@@ -289,17 +329,18 @@ split_reduce_kernel(int64_t n, int nparts, const VT* __restrict__ partial, VT* _
   }
 }
 
-template <typename VT, typename KT, int ROWS, bool SV>
+template <typename VT, typename KT, int ROWS, bool SV, bool DENSE>
 int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds, hipStream_t s) {
   // (t.descs / t.item_inst are NULL for a single instance)
-  auto kern = bucket_join_kernel<VT, KT, ROWS, SV>;
+  auto kern = bucket_join_kernel<VT, KT, ROWS, SV, DENSE>;
   if (lds > 65536) {
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   const int64_t grid = (int64_t)args.n_items * args.split;
   if (grid >= (1ll << 31)) return fail(SQDK_ERANGE, "grid of %lld workgroups too large", (long long)grid);
-  kern<<<(unsigned)grid, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.items, t.xmask, t.grp_off,
+  kern<<<(unsigned)grid, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask,
+                                           t.grp_off,
                                            t.zmask, t.coef, t.class_hi, t.class_off, t.v, t.out,
                                            t.descs, t.item_inst, args);
   SQDK_LAUNCHED("bucket_join_kernel");
@@ -313,13 +354,13 @@ int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds,
   return SQDK_OK;
 }
 
-template <typename VT, typename KT, bool SV>
+template <typename VT, typename KT, bool SV, bool DENSE = false>
 int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, size_t lds,
               hipStream_t s) {
   switch (rows) {
-    case 4: return launch_rows<VT, KT, 4, SV>(t, args, lds, s);
-    case 8: return launch_rows<VT, KT, 8, SV>(t, args, lds, s);
-    case 16: return launch_rows<VT, KT, 16, SV>(t, args, lds, s);
+    case 4: return launch_rows<VT, KT, 4, SV, DENSE>(t, args, lds, s);
+    case 8: return launch_rows<VT, KT, 8, SV, DENSE>(t, args, lds, s);
+    case 16: return launch_rows<VT, KT, 16, SV, DENSE>(t, args, lds, s);
     default: return fail(SQDK_EINVAL, "rows_per_thread %d unsupported (4, 8 or 16)", rows);
   }
 }
@@ -327,6 +368,13 @@ int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, siz
 template <typename VT>
 int launch_common(const MatvecTables<VT>& t, MatvecParams& args, int narrow_keys, int rows,
                   int stage_v, hipStream_t s) {
+  if (t.fbits) {
+    // Dense lookup: only the (bitmap, offset) table of the target bucket is staged.
+    if (stage_v) return fail(SQDK_EINVAL, "v staging is not available with the dense lookup");
+    if (((size_t)8 << args.dmax) > 160 * 1024)
+      return fail(SQDK_ERANGE, "dense lookup table of 2^%d entries exceeds 160 KiB of LDS", args.dmax);
+    return launch_kt<VT, uint32_t, false, true>(t, args, rows, (size_t)8 << args.dmax, s);
+  }
   const LdsLayout lay = lds_layout(args.dmax, args.max_bucket, narrow_keys ? 4 : 8,
                                    stage_v ? sizeof(VT) : 0);
   if (lay.total > 160 * 1024)
@@ -358,6 +406,10 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   t.out = reinterpret_cast<VT*>(out);
   t.descs = nullptr;
   t.item_inst = nullptr;
+  t.fbits = sp->fbits;
+  if (sp->fbits && (sp->fine_bits != sp->bucket_bits - 5 || sp->bucket_bits > 20))
+    return fail(SQDK_EINVAL, "dense lookup needs fine_bits = bucket_bits - 5 <= 15 (got %d, %d)",
+                sp->fine_bits, sp->bucket_bits);
   MatvecParams args;
   args.skip = skip;
   args.inst_skip = nullptr;
@@ -399,6 +451,7 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
   t.out = reinterpret_cast<VT*>(out);
   t.descs = b->inst;
   t.item_inst = b->item_inst;
+  t.fbits = nullptr;
   MatvecParams args;
   args.skip = skip;
   args.inst_skip = inst_skip;

@@ -82,6 +82,23 @@ fine_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bsta
   fdir[idx] = (uint16_t)(j - lo);
 }
 
+// fbits[e] = OR of 1 << (key & 31) over the keys of 32-value sub-bucket e
+// (fine directory with d = m - 5); one thread per sub-bucket, no atomics.
+__global__ void __launch_bounds__(kBlock)
+bits_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
+            const uint16_t* __restrict__ fdir, int d, int64_t total, uint32_t* __restrict__ fbits) {
+  const int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+  if (idx >= total) return;
+  const int64_t b = idx >> d;
+  const int64_t e = idx & ((1ll << d) - 1);
+  const int64_t base = bstart[b];
+  const int64_t lo = base + fdir[idx];
+  const int64_t hi = (e + 1 < (1ll << d)) ? base + fdir[idx + 1] : (int64_t)bstart[b + 1];
+  uint32_t bits = 0;
+  for (int64_t j = lo; j < hi; ++j) bits |= 1u << (uint32_t)(keys[j] & 31u);
+  fbits[idx] = bits;
+}
+
 // Row chunks. Single block: serial scan over buckets in block-sized strides.
 __global__ void __launch_bounds__(kBlock)
 items_kernel(const uint32_t* __restrict__ bstart, int64_t nb, int chunk,
@@ -206,6 +223,23 @@ int sqdk_space_fine(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t
   fine_kernel<<<(unsigned)ceil_div(total, kBlock), kBlock, 0, as_stream(stream)>>>(
       keys, bstart, bucket_bits, fine_bits, total, fdir);
   SQDK_LAUNCHED("fine_kernel");
+  return SQDK_OK;
+}
+
+int sqdk_space_bits(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t bucket_bits,
+                    int32_t fine_bits, const uint32_t* bstart, const uint16_t* fdir,
+                    uint32_t* fbits, void* stream) {
+  SQDK_REQUIRE(keys && bstart && fdir && fbits, "NULL pointer argument");
+  SQDK_REQUIRE(n >= 1 && n < (1ll << 31), "n=%lld out of range", (long long)n);
+  SQDK_REQUIRE(bucket_bits >= 5 && bucket_bits <= num_qubits && bucket_bits <= 20 &&
+                   num_qubits - bucket_bits <= 26,
+               "bucket_bits %d invalid for the dense lookup", bucket_bits);
+  SQDK_REQUIRE(fine_bits == bucket_bits - 5, "dense lookup needs fine_bits = bucket_bits - 5 "
+               "(got %d, %d)", fine_bits, bucket_bits);
+  const int64_t total = (1ll << (num_qubits - bucket_bits)) << fine_bits;
+  bits_kernel<<<(unsigned)ceil_div(total, kBlock), kBlock, 0, as_stream(stream)>>>(
+      keys, bstart, fdir, fine_bits, total, fbits);
+  SQDK_LAUNCHED("bits_kernel");
   return SQDK_OK;
 }
 

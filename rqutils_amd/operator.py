@@ -22,12 +22,22 @@ import torch
 from . import _lib
 from .paulis.symplectic import PauliSumXZ, _bits_to_u64
 
-__all__ = ["Subspace", "PauliOperator", "states_to_keys", "keys_to_states",
+__all__ = ["Subspace", "PauliOperator", "dense_fits", "states_to_keys", "keys_to_states",
            "full_space_matmul", "DEFAULT_BUCKET_CAP", "DEFAULT_ROWS_PER_THREAD"]
 
 DEFAULT_BUCKET_CAP = 8192       # keys per bucket staged in LDS (32 KiB as uint32)
 DEFAULT_ROWS_PER_THREAD = 4     # rows per thread; a chunk is 256 x this
 MAX_DIR_BITS = 26               # at most 2^26 buckets
+DENSE_BUCKET_BITS = 16          # default bucket width of the dense lookup (16 KiB of LDS)
+
+
+def dense_fits(num_qubits: int, n: int) -> bool:
+    """Whether the dense lookup is worth its 6 * 2^(num_qubits - 5) bytes: a
+    subspace of at least 2^16 keys, and at most 64 MiB or 48 bytes per key."""
+    if num_qubits < 10 or num_qubits - 5 > 26 or n < (1 << 16):
+        return False
+    nbytes = 6 * (1 << (num_qubits - 5))
+    return nbytes <= (64 << 20) or nbytes <= 48 * n
 
 
 def states_to_keys(states: np.ndarray) -> np.ndarray:
@@ -64,11 +74,19 @@ class Subspace:
     ``keys``: numpy uint64 array or a torch int64/uint64 tensor, strictly
     ascending, each below ``2**num_qubits``. ``bucket_bits`` defaults to the
     largest m whose biggest bucket holds at most ``bucket_cap`` keys.
+
+    ``dense=True`` builds the dense lookup instead: 32-value sub-buckets
+    (``fine_bits = bucket_bits - 5``) with an occupancy bitmap each, so the
+    matvec finds a target with one LDS read and a popcount. It costs
+    6 * 2^(num_qubits - 5) bytes of HBM whatever the subspace size
+    (``dense_fits``); ``bucket_bits`` then defaults to 16. ``dense=None``
+    picks it when it fits.
     """
 
     def __init__(self, keys, num_qubits: int, *, device=None, bucket_bits: int | None = None,
                  bucket_cap: int = DEFAULT_BUCKET_CAP,
-                 rows_per_thread: int = DEFAULT_ROWS_PER_THREAD, fine_bits: int | None = None):
+                 rows_per_thread: int = DEFAULT_ROWS_PER_THREAD, fine_bits: int | None = None,
+                 dense: bool | None = False):
         L = _lib.lib()
         self.device = _device(device)
         if not 1 <= num_qubits <= 64:
@@ -96,7 +114,14 @@ class Subspace:
                                  f"({int(err.item())} violations)")
             self.rows_per_thread = int(rows_per_thread)
             self.chunk_rows = 256 * self.rows_per_thread
-            self._plan(bucket_bits, bucket_cap, fine_bits)
+            if dense is None:
+                dense = dense_fits(self.num_qubits, self.n)
+            self.dense = bool(dense)
+            self.fbits = None
+            if self.dense:
+                self._plan_dense(bucket_bits)
+            else:
+                self._plan(bucket_bits, bucket_cap, fine_bits)
 
     # -- planning --------------------------------------------------------------
 
@@ -112,6 +137,27 @@ class Subspace:
     @staticmethod
     def _max_bucket(d: torch.Tensor) -> int:
         return int((d[1:] - d[:-1]).max().item())
+
+    def _plan_dense(self, bucket_bits):
+        nq = self.num_qubits
+        if nq < 5:
+            raise ValueError("the dense lookup needs at least 5 qubits")
+        lo_m = max(5, nq - MAX_DIR_BITS)
+        m = min(nq, DENSE_BUCKET_BITS) if bucket_bits is None else int(bucket_bits)
+        m = max(m, lo_m)
+        if not lo_m <= m <= min(nq, 19):
+            raise ValueError(f"bucket_bits={m} outside {lo_m}..{min(nq, 19)} for the dense lookup")
+        # Buckets must hold fewer than 2^16 keys (16-bit offsets).
+        while m > lo_m and self._max_bucket(self._dir(m)) >= 65536:
+            m -= 1
+        self._plan(m, None, m - 5)
+        L = _lib.lib()
+        self.fbits = torch.empty(self.n_buckets << self.fine_bits, dtype=torch.int32,
+                                 device=self.device)
+        _lib.check(L.sqdk_space_bits(self.keys.data_ptr(), self.n, nq, self.bucket_bits,
+                                     self.fine_bits, self.bstart.data_ptr(), self.fdir.data_ptr(),
+                                     self.fbits.data_ptr(), _lib.stream_ptr(self.device)),
+                   "sqdk_space_bits")
 
     def _plan(self, bucket_bits, cap, fine_bits):
         L = _lib.lib()
@@ -179,6 +225,7 @@ class Subspace:
         s.chunk_rows = self.chunk_rows
         s.n_items = self.n_items
         s.max_bucket = self.max_bucket
+        s.fbits = self.fbits.data_ptr() if self.fbits is not None else None
         return s
 
     def keys_numpy(self) -> np.ndarray:

@@ -41,6 +41,8 @@ class SQDProblem:
 
     ``keys``: sorted distinct uint64 keys (numpy or torch); ``hamiltonian``:
     ``PauliSumXZ`` or ``(labels, coeffs)``. Exposes ``matvec`` and ``solve``.
+    Without explicit directory arguments the subspace takes the dense lookup
+    where it fits (``operator.dense_fits``); ``tune="auto"`` times both kinds.
     """
 
     def __init__(self, hamiltonian, keys, num_qubits: Optional[int] = None, *, device=None,
@@ -57,6 +59,8 @@ class SQDProblem:
                                   precision="c64" if tune == "auto" else tune)
             space_kw = {**space_kw, **self.tuned.space_kw()}
             group_split = self.tuned.group_split
+        if not any(k in space_kw for k in ("dense", "bucket_cap", "bucket_bits", "fine_bits")):
+            space_kw["dense"] = None
         self.space = Subspace(keys, nq, device=device, **space_kw)
         self.op = PauliOperator(hamiltonian, self.space)
         self.op.set_tune(group_split=group_split)
@@ -211,7 +215,7 @@ def get_xsource(xsignature: np.ndarray, states: np.ndarray) -> torch.Tensor:
         raise ValueError("fill rows must come after every real row")
     out = torch.full((states.shape[0],), -1, dtype=torch.int32, device="cuda")
     if nreal:
-        space = Subspace(keys[:nreal], nbits)
+        space = Subspace(keys[:nreal], nbits, dense=None)
         out[:nreal] = space.xsource(x)
     return out
 

@@ -265,3 +265,56 @@ def test_autotune_picks_a_valid_launch(gpu):
     v = _rand_vec(np.random.default_rng(0), p.keys.size)
     got = prob.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
     assert _rel(got, p.matvec(v)) < 1e-12
+
+
+@pytest.mark.parametrize("m,rows,split", [(5, 4, 1), (8, 8, 1), (11, 4, 3), (14, 16, 1),
+                                          (13, 4, 2)])
+def test_dense_lookup_matches_search_bit_for_bit(gpu, m, rows, split):
+    """The dense lookup (bitmap + rank) finds exactly the keys the search finds,
+    in the same class and group order, so both give the same bits; both match
+    the oracle."""
+    from rqutils_amd.operator import PauliOperator, Subspace
+    p = small_problem(50 + m, n=14, terms=160, nstates=5000)
+    dense_sp = Subspace(p.keys, p.n, dense=True, bucket_bits=m, rows_per_thread=rows)
+    assert dense_sp.fine_bits == m - 5 and dense_sp.fbits is not None
+    # bitmaps against the keys: bit j of entry e set iff (e << 5) | j is a key
+    bits = dense_sp.fbits.cpu().numpy().view(np.uint32)
+    member = ((bits[:, None] >> np.arange(32, dtype=np.uint32)) & 1).astype(bool).reshape(-1)
+    assert np.array_equal(np.nonzero(member)[0].astype(np.uint64), p.keys)
+    search_sp = Subspace(p.keys, p.n, bucket_bits=m, rows_per_thread=rows)
+    rng = np.random.default_rng(m)
+    v = _rand_vec(rng, p.keys.size)
+    ref = p.matvec(v)
+    for dt in (np.complex128, np.complex64):
+        outs = []
+        for sp in (dense_sp, search_sp):
+            op = PauliOperator((p.labels, p.coeffs), sp)
+            op.set_tune(group_split=split)
+            outs.append(op.matvec(torch.from_numpy(v.astype(dt)).cuda()).cpu().numpy())
+        assert np.array_equal(outs[0].view(np.float64 if dt == np.complex128 else np.float32),
+                              outs[1].view(np.float64 if dt == np.complex128 else np.float32))
+        tol = 1e-12 if dt == np.complex128 else 2e-6
+        assert _rel(outs[0].astype(np.complex128), p.matvec(v.astype(dt).astype(complex))) < tol
+
+
+def test_dense_lookup_full_space_and_solver(gpu):
+    from rqutils_amd.operator import PauliOperator, Subspace
+    from rqutils_amd.sqd import SQDProblem
+    n = 12
+    rng = np.random.default_rng(77)
+    labels = random_labels(rng, n, 90, 1, 4)
+    coeffs = rng.normal(size=90)
+    keys = np.arange(2 ** n, dtype=np.uint64)
+    p = OracleProblem(labels, coeffs, keys, n)
+    prob = SQDProblem((labels, coeffs), keys, n, dense=True, bucket_bits=9)
+    assert prob.space.dense
+    v = _rand_vec(rng, keys.size)
+    got = prob.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+    assert _rel(got, p.matvec(v)) < 1e-12
+    vec, info = solve_fused(prob.op, v)
+    ev = p.ground()
+    assert info.converged and abs(info.eigval - ev) < 1e-9 * abs(ev), info
+    with pytest.raises(ValueError):
+        op = PauliOperator((labels, coeffs), prob.space)
+        op.tune.stage_v = 1
+        op.matvec(torch.from_numpy(v).cuda())
