@@ -86,6 +86,8 @@ __host__ __device__ inline LdsLayout lds_layout(int fine_bits, int max_bucket, s
   return l;
 }
 
+template <typename VT> constexpr bool kGatherFirst = sizeof(VT) == 8;
+
 // sum_k c_k (-1)^{popcount(z_k & s_r)} for every row r of a thread, terms in
 // index order (the order of group_coef, so results are identical). The term
 // tables are read once per term for all rows.
@@ -110,8 +112,10 @@ __device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* _
   }
 }
 
+// Occupancy: at 4 rows per thread the kernel must keep 5 waves per SIMD
+// (at most 96 VGPRs); the complex128 variant needs the hint to get there.
 template <typename VT, typename KT, int ROWS, bool STAGE_V, bool DENSE>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ROWS == 4 ? 5 : 1)))
 bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
                    const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ fbits,
                    const uint32_t* __restrict__ items,
@@ -219,6 +223,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
           any_hit |= hit;
         }
         if (!__any(any_hit)) continue;
+        // (the dense path has registers to spare: gather first in both precisions)
         VT vv[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) vv[r] = vq[max(idx[r], 0)];
@@ -291,14 +296,19 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
         }
         if (!__any(any_hit)) continue;
         // Hits: gather (a miss reads the slice's first element, in range and
-        // shared by the wave), then one pass over the group's terms for all rows.
+        // shared by the wave) and one pass over the group's terms for all rows.
+        // complex64 gathers first to hide the load behind the terms; complex128
+        // gathers after them, which keeps 16 fewer VGPRs live (occupancy).
         VT vv[ROWS];
+        if constexpr (kGatherFirst<VT>) {
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
+          for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
+        }
         RT cr[ROWS], ci[ROWS];
         rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
+          if constexpr (!kGatherFirst<VT>) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
           if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
         }
       }
