@@ -279,7 +279,9 @@ typedef struct sqdk_locg_result {
   int32_t niter;           /* loop iterations, counted as ground_locg.py:366-372 */
   int32_t converged;
   int32_t matvecs;
-  int32_t status;
+  int32_t status;          /* 0; 2: Rayleigh-Ritz breakdown (Cholesky pivot below the
+                              floor with P already dropped); 3: NaN or Inf in the
+                              Gram sums (non-finite operator or vectors) */
 } sqdk_locg_result;
 
 /* The operator: the built-in Pauli-sum matvec (ham + space + tune) when fn is

@@ -201,7 +201,7 @@ class SQDBatch:
         return out
 
     def solve(self, x0: torch.Tensor, *, precision: str = "c64", criterion: str = "relative",
-              tol: float = 1e-6, maxiter: int = 1000, refresh_every: int = 8,
+              tol: float = 1e-6, maxiter: int = 1000, refresh_every: int = 16,
               use_graph: bool = True) -> BatchSolve:
         """Fused LOBPCG on every instance from the concatenated start vectors x0."""
         L = _lib.lib()

@@ -143,6 +143,16 @@ __global__ void bscalar_kernel(int step, int n_inst, const int64_t* __restrict__
       s.coef = -s.theta;
       break;
     case kRR: {
+      // NaN or Inf in the sums ends this instance with status 3.
+      bool finite = isfinite(s.theta);
+      for (int k = 0; k < 13; ++k) finite = finite && isfinite(s.gram[k].x) && isfinite(s.gram[k].y);
+      if (!finite) {
+        s.status = 3;
+        s.matvecs += 1;
+        conv[i] = 1;
+        atomicAdd(&counters[0], 1);
+        break;
+      }
       const double rn = sqrt(fmax(s.gram[1].x, 0.0));
       const double axn = sqrt(fmax(s.gram[12].x, 0.0));
       double thr;
@@ -182,7 +192,7 @@ __global__ void bscalar_kernel(int step, int n_inst, const int64_t* __restrict__
       s.rn = rn;
       s.axn = axn;
       s.thr = thr;
-      conv[i] = (ok || s.status == 2) ? 1 : 0;
+      conv[i] = (ok || s.status >= 2) ? 1 : 0;
       break;
     }
     default:
@@ -380,7 +390,7 @@ struct BSolver {
 
 template <typename VT>
 int run_batch(BSolver& sv, const sqdk_locg_opts& o, int32_t* total_iters) {
-  const int every = o.refresh_every > 0 ? o.refresh_every : 8;
+  const int every = o.refresh_every > 0 ? o.refresh_every : 16;
   // Set-up: normalise X, A X, theta, R = A X - theta X.
   SQDK_TRY(sv.sums<VT>(false, sv.X, sv.X, sv.X, sv.X, false));
   SQDK_TRY(sv.scalar(kInitNorm, false));

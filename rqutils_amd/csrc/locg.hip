@@ -181,6 +181,15 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
     case kStepFusedRR: {
       const double2* g = S + kGram;
       const double theta = S[kTheta].x;
+      // A non-finite sum (NaN or Inf in the operator or the vectors) ends the
+      // solve with status 3 instead of iterating on it.
+      bool finite = isfinite(theta);
+      for (int i = 0; i < 13; ++i) finite = finite && isfinite(g[i].x) && isfinite(g[i].y);
+      if (!finite) {
+        F[kStatus] = 3;
+        F[kConv] = 1;
+        break;
+      }
       const double rn = sqrt(fmax(g[1].x, 0.0));
       const double axn = sqrt(fmax(g[12].x, 0.0));
       if (criterion_met(prm, rn, axn, theta, S)) {
@@ -453,7 +462,7 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
       SQDK_TRY(sv.dots({{AX, AX}, {R, R}}, false));
       SQDK_TRY(sv.scalar(kStepVerify, false));
       SQDK_TRY(sv.read_flags(flags_host));
-      if (flags_host[kConv] || done >= o.maxiter || flags_host[kStatus] == 2) break;
+      if (flags_host[kConv] || done >= o.maxiter || flags_host[kStatus] >= 2) break;
       // Not converged after all: A P is refreshed too and the loop goes on.
       SQDK_TRY(sv.apply(P, AP, false));
       since_refresh = 0;

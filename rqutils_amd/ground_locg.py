@@ -37,6 +37,9 @@ __all__ = ["ground_locg", "solve", "LocgInfo", "eigenpair_2x2", "eigenpair_3x3"]
 
 LOG = logging.getLogger(__name__)
 
+STATUS_BREAKDOWN = 2      # LocgInfo.status: Rayleigh-Ritz breakdown (Cholesky pivot floor)
+STATUS_NONFINITE = 3      # LocgInfo.status: NaN or Inf in the Gram sums
+
 _MODES = {"reference": _lib.LOCG_REFERENCE, "fused": _lib.LOCG_FUSED}
 _CRITERIA = {"reference": _lib.CRIT_REFERENCE, "relative": _lib.CRIT_RELATIVE}
 _PRECISIONS = {"c128": _lib.C128, "c64": _lib.C64}
@@ -104,7 +107,7 @@ class _Callback:
 
 def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = None,
           criterion: str = "relative", tol: float = 1e-6, maxiter: int = 1000,
-          n_for_tol: int | None = None, check_every: int = 4, refresh_every: int = 8,
+          n_for_tol: int | None = None, check_every: int = 4, refresh_every: int = 16,
           real: bool = False) -> tuple[torch.Tensor, LocgInfo]:
     """Lowest eigenpair of ``op`` from start vector ``x0`` (device tensor).
 
@@ -162,13 +165,18 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     _lib.check(rc, "sqdk_locg_solve")
     info = LocgInfo(res.eigval, res.niter, bool(res.converged), res.matvecs, res.resid_norm,
                     res.ax_norm, res.status)
+    if info.status == STATUS_NONFINITE:
+        LOG.warning("LOBPCG stopped at iteration %d: NaN or Inf in the operator or vectors",
+                    info.niter)
+    elif info.status == STATUS_BREAKDOWN:
+        LOG.warning("LOBPCG stopped at iteration %d: Rayleigh-Ritz breakdown", info.niter)
     return x, info
 
 
 def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | None = None,
                 vspace=None, log_level: int = logging.WARNING, *, mode: str = "reference",
                 precision: str | None = None, criterion: str = "reference",
-                check_every: int | None = None, refresh_every: int = 8,
+                check_every: int | None = None, refresh_every: int = 16,
                 device=None):
     r"""Single-vector LOBPCG (rqutils/ground_locg.py:128-157).
 

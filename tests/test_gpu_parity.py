@@ -318,3 +318,26 @@ def test_dense_lookup_full_space_and_solver(gpu):
         op = PauliOperator((labels, coeffs), prob.space)
         op.tune.stage_v = 1
         op.matvec(torch.from_numpy(v).cuda())
+
+
+def test_fused_solver_stops_on_nan(gpu):
+    """A NaN in the operator's output ends the fused solve with status 3 at
+    once instead of iterating on it (the reference has no such guard)."""
+    from rqutils_amd.ground_locg import STATUS_NONFINITE, solve
+    n = 64
+    a = torch.randn(n, n, dtype=torch.float64)
+    a = (a + a.T).to(torch.complex128).cuda()
+    calls = []
+
+    def op(x):
+        calls.append(1)
+        y = a @ x
+        if len(calls) > 3:
+            y[5] = float("nan")
+        return y
+
+    x0 = torch.randn(n, dtype=torch.complex128).cuda()
+    _, info = solve(op, x0, mode="fused", precision="c128", criterion="relative", tol=1e-12,
+                    maxiter=500)
+    assert info.status == STATUS_NONFINITE and not info.converged
+    assert len(calls) < 12
