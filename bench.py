@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--dense-bits", type=int, default=0,
                     help="dense lookup at this bucket width (0: autotune or search)")
     ap.add_argument("--no-net", action="store_true", help="skip the network-start block")
+    ap.add_argument("--no-intake", action="store_true", help="skip the sample-intake block")
     ap.add_argument("--net-train", type=int, default=16,
                     help="training Hamiltonians of the network-start block")
     ap.add_argument("--net-test", type=int, default=3, help="held-out Hamiltonians")
@@ -235,6 +236,8 @@ def main():
 
     if not args.no_solve:
         record["time_to_solution"] = time_to_solution(prob, w, dev)
+    if not args.no_intake:
+        record["intake"] = intake_block(w, dev)
     if not args.no_net and w.name == "heisenberg28":
         record["network_start"] = {
             "heisenberg28_2^20": network_start(dev, "heisenberg28_2^20", args.net_train,
@@ -290,6 +293,50 @@ def time_to_solution(prob, w, dev):
     base = res["reference_c128"]["seconds"]
     res["speedup_fused_c64_vs_plain_fp64"] = round(base / res["fused_c64"]["seconds"], 2)
     return res
+
+
+def intake_block(w, dev, repeats: int = 4, reps: int = 3):
+    """Sample intake on the device: ``repeats * N`` state rows (every key of
+    the workload at least once, the rest drawn with replacement, shuffled)
+    resident in HBM as uint8 [rows, n]; timed: pack into keys, radix sort and
+    unique. The result must be the workload's keys bit for bit. CPU baseline:
+    numpy (pack + np.unique) on a 2^22-row sample, scaled by the row count."""
+    import torch
+    from rqutils_amd.intake import pack_states, unique_keys, unpack_keys
+    from rqutils_amd.operator import states_to_keys
+    n, nq = w.n, w.num_qubits
+    rng = np.random.default_rng(5)
+    idx = np.concatenate([np.arange(n), rng.integers(0, n, size=(repeats - 1) * n)])
+    rng.shuffle(idx)
+    kd = torch.from_numpy(w.keys.view(np.int64)).to(dev)[torch.from_numpy(idx).to(dev)]
+    states = unpack_keys(kd, nq)
+    del kd
+    rows = states.shape[0]
+    keys = pack_states(states, dev)
+    u = unique_keys(keys, nq)                        # warm-up
+    ok = bool(np.array_equal(u.cpu().numpy().view(np.uint64), w.keys))
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(reps):
+        keys = pack_states(states, dev)
+        u = unique_keys(keys, nq)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    sample = states[: 1 << 22].cpu().numpy()
+    t0 = time.perf_counter()
+    np.unique(states_to_keys(sample))
+    cpu_s = (time.perf_counter() - t0) * rows / sample.shape[0]
+    log(f"intake: {rows} rows -> {u.numel()} distinct in {ms:.2f} ms on the GPU "
+        f"({rows / ms / 1e6:.0f} M rows/s), bit-exact={ok}; numpy {cpu_s:.2f} s (scaled)")
+    return {"rows": rows, "distinct": int(u.numel()), "ms": round(ms, 3),
+            "rows_per_s": round(rows / (ms * 1e-3), 1), "bit_exact_vs_workload_keys": ok,
+            "state_bytes": rows * nq,
+            "cpu_numpy_s_scaled": round(cpu_s, 3),
+            "cpu_sample": f"pack + np.unique of {sample.shape[0]} rows, x{rows / sample.shape[0]:.1f}"}
 
 
 NET_STARTS = ("gaussian", "min_diag", "network_bits", "network_bits_diag")

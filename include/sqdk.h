@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 7
+#define SQDK_ABI_VERSION 8
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -174,6 +174,28 @@ int sqdk_space_dir(const uint64_t* keys, int64_t n, int32_t num_qubits,
 int sqdk_space_fine(const uint64_t* keys, int64_t n, int32_t num_qubits,
                     int32_t bucket_bits, int32_t fine_bits,
                     const uint32_t* bstart, uint16_t* fdir, void* stream);
+
+/* ---- intake of sampled bitstrings (rqutils/sqd.py:237, 246, 414-445) ---- */
+
+/* keys[r] = sum_c (states[r*row_stride + c] != 0) << (num_qubits - 1 - c): column
+ * c of a row is qubit num_qubits-1-c, so ascending keys are the reference's
+ * order of packed rows (sqd.py:237). */
+int sqdk_intake_pack(const uint8_t* states, int64_t n_rows, int32_t num_qubits,
+                     int64_t row_stride, uint64_t* keys, void* stream);
+
+/* Sorted distinct keys: out[0..*n_unique) (out holds n entries; *n_unique is
+ * written on the device). A stable LSD radix sort over the low key_bits bits
+ * (keys must be below 2^key_bits) and a compaction; replaces the lexsort +
+ * unique of uniquify_states (sqd.py:414-445), whose fill rows are not
+ * materialised. Workspace: sqdk_intake_unique_workspace. */
+int sqdk_intake_unique_workspace(int64_t n, size_t* bytes);
+int sqdk_intake_unique(const uint64_t* keys, int64_t n, int32_t key_bits, uint64_t* out,
+                       int64_t* n_unique, void* work, size_t work_bytes, void* stream);
+
+/* states[r*num_qubits + c] = bit (num_qubits - 1 - c) of keys[r]: the
+ * basis_states the reference returns (sqd.py:246). */
+int sqdk_intake_unpack(const uint64_t* keys, int64_t n, int32_t num_qubits, uint8_t* states,
+                       void* stream);
 
 /* Occupancy bitmaps for the dense lookup: with fine_bits == bucket_bits - 5,
  * fbits[e] has bit j set iff key (e << 5) | j is in the subspace, for every

@@ -47,15 +47,15 @@ class TuneResult:
     group_split: int
     ms: float
     precision: str
-    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits) per candidate
+    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits, threads)
     dense_bits: int = 0                         # bucket bits of the dense lookup, 0: search
+    block_threads: int = 256
 
     def space_kw(self) -> dict:
+        kw = {"rows_per_thread": self.rows_per_thread, "block_threads": self.block_threads}
         if self.dense_bits:
-            return {"dense": True, "bucket_bits": self.dense_bits,
-                    "rows_per_thread": self.rows_per_thread}
-        return {"bucket_cap": self.bucket_cap, "rows_per_thread": self.rows_per_thread,
-                "dense": False}
+            return {**kw, "dense": True, "bucket_bits": self.dense_bits}
+        return {**kw, "bucket_cap": self.bucket_cap, "dense": False}
 
 
 def signature(hamiltonian: PauliSumXZ, keys: torch.Tensor, num_qubits: int, precision: str) -> str:
@@ -117,10 +117,13 @@ def _split_candidates(n_items: int):
 
 
 def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str = "c64",
-             caps=DEFAULT_CAPS, rows=DEFAULT_ROWS, dense_bits=DEFAULT_DENSE_BITS, reps: int = 3,
-             use_cache: bool = True, verbose: bool = False) -> TuneResult:
+             caps=DEFAULT_CAPS, rows=DEFAULT_ROWS, dense_bits=DEFAULT_DENSE_BITS,
+             threads=(256,), reps: int = 3, use_cache: bool = True,
+             verbose: bool = False) -> TuneResult:
     """Fastest (lookup, bucket cap or width, rows per thread, group split) for
-    this problem."""
+    this problem. ``threads=(256, 512)`` also times 512-thread workgroups
+    (off by default: measured faster only for complex64 on the sparse 40-qubit
+    subspace, and slower for complex128 there)."""
     if not isinstance(hamiltonian, PauliSumXZ):
         hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True, add_padding=True)
     dev = torch.device(device if device is not None else "cuda")
@@ -144,20 +147,24 @@ def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str 
     table = []
     best = None
     seen_m = {}
-    candidates = [(int(c), r, 0) for c in sorted(set(int(c) for c in caps)) for r in rows]
+    # (cap, rows, dense bits, threads); 512-thread workgroups only at 4 rows
+    shapes = [(r, 256) for r in rows] + ([(4, 512)] if 4 in rows and 512 in threads else [])
+    candidates = [(int(c), r, 0, t) for c in sorted(set(int(c) for c in caps)) for r, t in shapes]
     if dense_fits(num_qubits, n):
         lo_m, hi_m = max(5, num_qubits - 26), min(num_qubits, 19)
-        candidates += [(0, r, int(m)) for m in dense_bits if lo_m <= m <= hi_m for r in rows]
-    for cap, r, dm in candidates:
+        candidates += [(0, r, int(m), t) for m in dense_bits if lo_m <= m <= hi_m
+                       for r, t in shapes]
+    for cap, r, dm, nt in candidates:
         try:
             if dm:
                 sp = Subspace(kt, num_qubits, device=dev, dense=True, bucket_bits=dm,
-                              rows_per_thread=r)
+                              rows_per_thread=r, block_threads=nt)
             else:
-                sp = Subspace(kt, num_qubits, device=dev, bucket_cap=cap, rows_per_thread=r)
+                sp = Subspace(kt, num_qubits, device=dev, bucket_cap=cap, rows_per_thread=r,
+                              block_threads=nt)
         except ValueError:
             continue            # keys too concentrated for this cap
-        key = (sp.bucket_bits, r, sp.dense)
+        key = (sp.bucket_bits, r, sp.dense, nt)
         if seen_m.get(key):
             continue            # same bucket level as a smaller cap: same kernel
         seen_m[key] = True
@@ -165,16 +172,16 @@ def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str 
         for split in _split_candidates(sp.n_items):
             op.set_tune(group_split=split)
             ms = _time_matvec(op, v, reps)
-            table.append((cap, r, split, round(ms, 4), dm))
+            table.append((cap, r, split, round(ms, 4), dm, nt))
             if verbose:
                 print(f"[autotune] {'dense' if dm else 'cap=%d' % cap} m={sp.bucket_bits} "
-                      f"rows={r} split={split}: {ms:.3f} ms", flush=True)
+                      f"rows={r} threads={nt} split={split}: {ms:.3f} ms", flush=True)
             if best is None or ms < best[3]:
-                best = (cap, r, split, ms, dm)
+                best = (cap, r, split, ms, dm, nt)
         del op, sp
     if best is None:
         raise ValueError("no matvec configuration fits this subspace")
-    res = TuneResult(best[0], best[1], best[2], best[3], precision, table, best[4])
+    res = TuneResult(best[0], best[1], best[2], best[3], precision, table, best[4], best[5])
     _MEMO[sig] = res
     _save_disk(sig, res)
     return res

@@ -38,6 +38,7 @@ struct MatvecParams {
   int32_t max_bucket;        // largest bucket over all instances
   int32_t dmax;              // largest fine_bits over all instances (LDS layout)
   int32_t split;             // workgroups sharing one row chunk (groups dealt round robin)
+  int32_t threads;           // workgroup size: 256, or 512 with 4 rows per thread
   int64_t n;                 // rows, for the partial-sum layout
   void* partial;             // (split - 1) x n partial sums when split > 1
   sqdk_inst single;          // the instance when no descriptor array is given
@@ -114,8 +115,8 @@ __device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* _
 
 // Occupancy: at 4 rows per thread the kernel must keep 5 waves per SIMD
 // (at most 96 VGPRs); the complex128 variant needs the hint to get there.
-template <typename VT, typename KT, int ROWS, bool STAGE_V, bool DENSE>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ROWS == 4 ? 5 : 1)))
+template <typename VT, typename KT, int ROWS, bool STAGE_V, bool DENSE, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ROWS == 4 ? 5 : 1)))
 bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
                    const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ fbits,
                    const uint32_t* __restrict__ items,
@@ -161,7 +162,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   const uint64_t bucket = keys[row0] >> m;
   const uint64_t bucket_base = m >= 64 ? 0ull : (bucket << m);
   const uint32_t bend = bst[bucket + 1];
-  const uint32_t row1 = min(bend, row0 + (uint32_t)(ROWS * kBlock));
+  const uint32_t row1 = min(bend, row0 + (uint32_t)(ROWS * NT));
 
   // Each row keeps only its low key bits: the high bits are the bucket's.
   KT sl[ROWS];
@@ -169,7 +170,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   Acc<VT> acc[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    const uint32_t i = row0 + tid + r * kBlock;
+    const uint32_t i = row0 + tid + r * NT;
     valid[r] = i < row1;
     sl[r] = valid[r] ? (KT)(keys[i] & lowmask) : (KT)0;
   }
@@ -184,7 +185,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     rows_coef<CT, KT, ROWS>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base, sl, cr, ci);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      if (valid[r]) acc[r].fma(cr[r], ci[r], v[row0 + tid + r * kBlock]);
+      if (valid[r]) acc[r].fma(cr[r], ci[r], v[row0 + tid + r * NT]);
     }
   }
 
@@ -204,7 +205,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     if constexpr (DENSE) {
       __syncthreads();  // the previous class is done with the LDS table
       const uint64_t fbase = q << d;
-      for (int e = tid; e < ndir; e += kBlock) ldir[e] = make_uint2(fbits[fbase + e], fdr[fbase + e]);
+      for (int e = tid; e < ndir; e += NT) ldir[e] = make_uint2(fbits[fbase + e], fdr[fbase + e]);
       __syncthreads();
       if (!wave_active) continue;  // the barriers above are still reached by every wave
       const VT* __restrict__ vq = v + qs;
@@ -237,15 +238,15 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
       (void)qn;
     } else {
       __syncthreads();  // the previous class is done with the LDS tables
-      for (int i = tid; i < qn; i += kBlock) lkeys[i] = (KT)(keys[qs + i] & lowmask);
+      for (int i = tid; i < qn; i += NT) lkeys[i] = (KT)(keys[qs + i] & lowmask);
       if (tid == 0) lkeys[qn] = (KT)0;
       if (STAGE_V) {
-        for (int i = tid; i < qn; i += kBlock) lv[i] = v[qs + i];
+        for (int i = tid; i < qn; i += NT) lv[i] = v[qs + i];
       }
       // Sub-bucket ranges as (lo | hi << 16) and the largest occupancy.
       const uint64_t fbase = q << d;
       int occ = 0;
-      for (int e = tid; e < ndir; e += kBlock) {
+      for (int e = tid; e < ndir; e += NT) {
         const uint32_t lo = fdr[fbase + e];
         const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
         lpair[e] = lo | (hi << 16);
@@ -257,7 +258,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
       __syncthreads();
       int maxocc = 0;
 #pragma unroll
-      for (int i = 0; i < kBlock / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
+      for (int i = 0; i < NT / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
       // Steps of the branch-free lower bound: ceil(log2(maxocc)).
       const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
       if (!wave_active) continue;  // the barriers above are still reached by every wave
@@ -319,7 +320,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   VT* __restrict__ dst = gs == 0 ? out : reinterpret_cast<VT*>(a.partial) + (int64_t)(gs - 1) * a.n;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    if (valid[r]) dst[row0 + tid + r * kBlock] = acc[r].result();
+    if (valid[r]) dst[row0 + tid + r * NT] = acc[r].result();
   }
 }
 
@@ -339,17 +340,17 @@ split_reduce_kernel(int64_t n, int nparts, const VT* __restrict__ partial, VT* _
   }
 }
 
-template <typename VT, typename KT, int ROWS, bool SV, bool DENSE>
+template <typename VT, typename KT, int ROWS, bool SV, bool DENSE, int NT = kBlock>
 int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds, hipStream_t s) {
   // (t.descs / t.item_inst are NULL for a single instance)
-  auto kern = bucket_join_kernel<VT, KT, ROWS, SV, DENSE>;
+  auto kern = bucket_join_kernel<VT, KT, ROWS, SV, DENSE, NT>;
   if (lds > 65536) {
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   const int64_t grid = (int64_t)args.n_items * args.split;
   if (grid >= (1ll << 31)) return fail(SQDK_ERANGE, "grid of %lld workgroups too large", (long long)grid);
-  kern<<<(unsigned)grid, kBlock, lds, s>>>(t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask,
+  kern<<<(unsigned)grid, NT, lds, s>>>(t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask,
                                            t.grp_off,
                                            t.zmask, t.coef, t.class_hi, t.class_off, t.v, t.out,
                                            t.descs, t.item_inst, args);
@@ -367,6 +368,11 @@ int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds,
 template <typename VT, typename KT, bool SV, bool DENSE = false>
 int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, size_t lds,
               hipStream_t s) {
+  // 512-thread workgroups (2,048-row chunks) exist for 4 rows per thread only.
+  if (args.threads == 512) {
+    if (rows != 4) return fail(SQDK_EINVAL, "512-thread chunks need 4 rows per thread");
+    return launch_rows<VT, KT, 4, SV, DENSE, 512>(t, args, lds, s);
+  }
   switch (rows) {
     case 4: return launch_rows<VT, KT, 4, SV, DENSE>(t, args, lds, s);
     case 8: return launch_rows<VT, KT, 8, SV, DENSE>(t, args, lds, s);
@@ -429,6 +435,7 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.max_bucket = sp->max_bucket;
   args.dmax = sp->fine_bits;
   args.split = split;
+  args.threads = sp->chunk_rows / rows;
   args.n = sp->n;
   args.partial = work;
   args.single.bstart_base = 0;
@@ -471,6 +478,7 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
   args.max_bucket = b->max_bucket;
   args.dmax = b->max_fine_bits;
   args.split = split;
+  args.threads = b->chunk_rows / rows;
   args.n = b->n;
   args.partial = work;
   memset(&args.single, 0, sizeof(args.single));
@@ -515,7 +523,7 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
     stage_v = tune->stage_v;
   }
   SQDK_REQUIRE(split <= 1024, "group_split %d too large", split);
-  SQDK_REQUIRE(rows * kBlock == space->chunk_rows,
+  SQDK_REQUIRE(rows * kBlock == space->chunk_rows || (rows == 4 && rows * 512 == space->chunk_rows),
                "rows_per_thread %d does not match the space's chunk of %d rows", rows,
                space->chunk_rows);
   const size_t need = matvec_work_bytes(space, tune, precision);
@@ -559,7 +567,7 @@ int batch_matvec_impl(const sqdk_batch* b, int32_t precision, const void* v, voi
     stage_v = tune->stage_v;
   }
   SQDK_REQUIRE(split <= 1024, "group_split %d too large", split);
-  SQDK_REQUIRE(rows * kBlock == b->chunk_rows,
+  SQDK_REQUIRE(rows * kBlock == b->chunk_rows || (rows == 4 && rows * 512 == b->chunk_rows),
                "rows_per_thread %d does not match the batch's chunk of %d rows", rows,
                b->chunk_rows);
   const size_t need = split > 1 ? (size_t)(split - 1) * (size_t)b->n * (precision == SQDK_C128 ? 16 : 8) : 0;

@@ -81,12 +81,15 @@ class Subspace:
     6 * 2^(num_qubits - 5) bytes of HBM whatever the subspace size
     (``dense_fits``); ``bucket_bits`` then defaults to 16. ``dense=None``
     picks it when it fits.
+
+    ``block_threads=512`` (with 4 rows per thread) makes 2,048-row chunks, so
+    one staged target table serves twice the rows of a 256-thread chunk.
     """
 
     def __init__(self, keys, num_qubits: int, *, device=None, bucket_bits: int | None = None,
                  bucket_cap: int = DEFAULT_BUCKET_CAP,
                  rows_per_thread: int = DEFAULT_ROWS_PER_THREAD, fine_bits: int | None = None,
-                 dense: bool | None = False):
+                 dense: bool | None = False, block_threads: int = 256):
         L = _lib.lib()
         self.device = _device(device)
         if not 1 <= num_qubits <= 64:
@@ -113,7 +116,10 @@ class Subspace:
                 raise ValueError(f"keys must be strictly ascending and below 2^{num_qubits} "
                                  f"({int(err.item())} violations)")
             self.rows_per_thread = int(rows_per_thread)
-            self.chunk_rows = 256 * self.rows_per_thread
+            if block_threads not in (256, 512) or (block_threads == 512 and rows_per_thread != 4):
+                raise ValueError("block_threads must be 256, or 512 with 4 rows per thread")
+            self.block_threads = int(block_threads)
+            self.chunk_rows = self.block_threads * self.rows_per_thread
             if dense is None:
                 dense = dense_fits(self.num_qubits, self.n)
             self.dense = bool(dense)

@@ -26,6 +26,7 @@ from scipy.sparse import coo_array, csr_array
 
 from . import _lib
 from .ground_locg import solve
+from .intake import uniquify_states, unpack_keys
 from .operator import PauliOperator, Subspace, keys_to_states, states_to_keys
 from .paulis.symplectic import PauliSumXZ, _bits_to_u64, unpack_rows
 
@@ -124,7 +125,8 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
     if not isinstance(hamiltonian, PauliSumXZ):
         hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True, add_padding=True)
     num_qubits = states.shape[1]
-    keys = np.unique(states_to_keys(states))
+    # pack + sort + unique on the device (the distinct rows of uniquify_states)
+    keys = uniquify_states(states, device)
     start = time.time()
     prob = SQDProblem(hamiltonian, keys, num_qubits, device=device)
     real = not prob.op.is_complex
@@ -140,7 +142,7 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
         return eigval
     v = vec.cpu().numpy()
     v = v.real.astype(np.float64) if real else v.astype(np.complex128)
-    return eigval, v, keys_to_states(keys, num_qubits)
+    return eigval, v, unpack_keys(keys, num_qubits).cpu().numpy()
 
 
 def hproj(hamiltonian, states, unique_states: bool = False) -> csr_array:

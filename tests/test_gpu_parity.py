@@ -341,3 +341,25 @@ def test_fused_solver_stops_on_nan(gpu):
                     maxiter=500)
     assert info.status == STATUS_NONFINITE and not info.converged
     assert len(calls) < 12
+
+
+@pytest.mark.parametrize("dense,split", [(False, 1), (True, 1), (False, 2), (True, 3)])
+def test_matvec_512_thread_chunks_match_256(gpu, dense, split):
+    """2,048-row chunks (512-thread workgroups) change only which rows share a
+    staged table, not any row's summation order: same bits as 256 threads."""
+    from rqutils_amd.operator import PauliOperator, Subspace
+    p = small_problem(61, n=14, terms=150, nstates=9000)
+    kw = dict(dense=True, bucket_bits=12) if dense else dict(bucket_bits=11)
+    rng = np.random.default_rng(5)
+    v = _rand_vec(rng, p.keys.size)
+    outs = []
+    for nt in (256, 512):
+        sp = Subspace(p.keys, p.n, rows_per_thread=4, block_threads=nt, **kw)
+        assert sp.chunk_rows == 4 * nt
+        op = PauliOperator((p.labels, p.coeffs), sp)
+        op.set_tune(group_split=split)
+        outs.append(op.matvec(torch.from_numpy(v).cuda()).cpu().numpy())
+    assert np.array_equal(outs[0].view(np.float64), outs[1].view(np.float64))
+    assert _rel(outs[1], p.matvec(v)) < 1e-12
+    with pytest.raises(ValueError):
+        Subspace(p.keys, p.n, rows_per_thread=8, block_threads=512)
