@@ -331,7 +331,7 @@ def intake_block(w, dev, repeats: int = 4, reps: int = 3):
     np.unique(states_to_keys(sample))
     cpu_s = (time.perf_counter() - t0) * rows / sample.shape[0]
     log(f"intake: {rows} rows -> {u.numel()} distinct in {ms:.2f} ms on the GPU "
-        f"({rows / ms / 1e6:.0f} M rows/s), bit-exact={ok}; numpy {cpu_s:.2f} s (scaled)")
+        f"({rows / (ms * 1e-3) / 1e9:.2f} G rows/s), bit-exact={ok}; numpy {cpu_s:.2f} s (scaled)")
     return {"rows": rows, "distinct": int(u.numel()), "ms": round(ms, 3),
             "rows_per_s": round(rows / (ms * 1e-3), 1), "bit_exact_vs_workload_keys": ok,
             "state_bytes": rows * nq,
