@@ -119,12 +119,13 @@ template <> struct Acc<double2> {
 template <> struct Acc<float2> {
   float sr, er, si, ei;
   __device__ inline Acc() : sr(0.f), er(0.f), si(0.f), ei(0.f) {}
-  // Each real product is rounded once; the sums are compensated.
+  // Each part of the complex product c v is formed in FP32 with one FMA over
+  // one rounded product; the running sums are compensated (two-sum).
   __device__ inline void fma(float cr, float ci, float2 v) {
-    two_sum(sr, er, cr * v.x);
-    two_sum(sr, er, -(ci * v.y));
-    two_sum(si, ei, cr * v.y);
-    two_sum(si, ei, ci * v.x);
+    const float pr = __builtin_fmaf(cr, v.x, -(ci * v.y));
+    const float pi = __builtin_fmaf(cr, v.y, ci * v.x);
+    two_sum(sr, er, pr);
+    two_sum(si, ei, pi);
   }
   __device__ inline float2 result() const { return make_float2(sr + er, si + ei); }
 };

@@ -92,10 +92,12 @@ template <typename VT> constexpr bool kGatherFirst = sizeof(VT) == 8;
 // sum_k c_k (-1)^{popcount(z_k & s_r)} for every row r of a thread, terms in
 // index order (the order of group_coef, so results are identical). The term
 // tables are read once per term for all rows.
-template <typename CT, typename KT, int ROWS>
-__device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* __restrict__ coef,
-                                 int t0, int t1, uint64_t bucket_base, const KT (&sl)[ROWS],
-                                 decltype(CT::x) (&cr)[ROWS], decltype(CT::x) (&ci)[ROWS]) {
+// NARROW: every row key of the chunk is below 2^32, so the parity needs only
+// the low words.
+template <typename CT, typename KT, int ROWS, bool NARROW>
+__device__ inline void rows_coef_n(const uint64_t* __restrict__ zmask, const CT* __restrict__ coef,
+                                   int t0, int t1, uint64_t bucket_base, const KT (&sl)[ROWS],
+                                   decltype(CT::x) (&cr)[ROWS], decltype(CT::x) (&ci)[ROWS]) {
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     cr[r] = 0;
@@ -106,11 +108,24 @@ __device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* _
     const CT c = coef[k];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      const bool odd = (__popcll(z & (bucket_base | (uint64_t)sl[r])) & 1) != 0;
+      const int pc = NARROW ? __popc((uint32_t)z & ((uint32_t)bucket_base | (uint32_t)sl[r]))
+                            : __popcll(z & (bucket_base | (uint64_t)sl[r]));
+      const bool odd = (pc & 1) != 0;
       cr[r] += odd ? -c.x : c.x;
       ci[r] += odd ? -c.y : c.y;
     }
   }
+}
+
+template <typename CT, typename KT, int ROWS>
+__device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* __restrict__ coef,
+                                 int t0, int t1, uint64_t bucket_base, bool narrow,
+                                 const KT (&sl)[ROWS], decltype(CT::x) (&cr)[ROWS],
+                                 decltype(CT::x) (&ci)[ROWS]) {
+  if (narrow)
+    rows_coef_n<CT, KT, ROWS, true>(zmask, coef, t0, t1, bucket_base, sl, cr, ci);
+  else
+    rows_coef_n<CT, KT, ROWS, false>(zmask, coef, t0, t1, bucket_base, sl, cr, ci);
 }
 
 // Occupancy: at 4 rows per thread the kernel must keep 5 waves per SIMD
@@ -161,6 +176,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   const uint32_t row0 = items[w];
   const uint64_t bucket = keys[row0] >> m;
   const uint64_t bucket_base = m >= 64 ? 0ull : (bucket << m);
+  const bool narrow = ((bucket_base | lowmask) >> 32) == 0;  // all row keys below 2^32
   const uint32_t bend = bst[bucket + 1];
   const uint32_t row1 = min(bend, row0 + (uint32_t)(ROWS * NT));
 
@@ -182,7 +198,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   // for all rows of the thread.
   if (gs == 0 && D.diag_t1 > D.diag_t0 && wave_active) {
     RT cr[ROWS], ci[ROWS];
-    rows_coef<CT, KT, ROWS>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base, sl, cr, ci);
+    rows_coef<CT, KT, ROWS>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base, narrow, sl, cr, ci);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       if (valid[r]) acc[r].fma(cr[r], ci[r], v[row0 + tid + r * NT]);
@@ -229,7 +245,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) vv[r] = vq[max(idx[r], 0)];
         RT cr[ROWS], ci[ROWS];
-        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
+        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, narrow, sl, cr, ci);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
@@ -306,7 +322,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
           for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
         }
         RT cr[ROWS], ci[ROWS];
-        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, sl, cr, ci);
+        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, narrow, sl, cr, ci);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           if constexpr (!kGatherFirst<VT>) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
