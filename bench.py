@@ -371,12 +371,14 @@ def network_start(dev, family, n_train, n_test, steps):
     keys, tgts, feats = [], [], []
     torch.cuda.synchronize()
     t = time.time()
-    for w in ws[:n_train]:
+    for j, w in enumerate(ws[:n_train]):
         p = SQDProblem(w.hamiltonian, w.keys, nq, device=dev)
         tg, _, info = ground_targets(p)
         keys.append(p.space.keys)
         tgts.append(tg)
         feats.append(diag_feature(p))
+        if (j + 1) % 25 == 0:
+            log(f"[{family}] {j + 1}/{n_train} ground vectors")
     torch.cuda.synchronize()
     ground_s = time.time() - t
     log(f"[{family}] {n_train} FP64 ground vectors ({ground_s:.1f} s)")
