@@ -93,6 +93,8 @@ def main():
                     help="fix rows per thread (0: autotune the launch)")
     ap.add_argument("--group-split", type=int, default=1)
     ap.add_argument("--stage-v", type=int, default=0, help="stage v slices in LDS (0/1)")
+    ap.add_argument("--compact", type=int, default=0,
+                    help="with an explicit launch: apply hits through the per-wave LDS queue (0/1)")
     ap.add_argument("--bucket-cap", type=int, default=None, help="max keys per bucket")
     ap.add_argument("--dense-bits", type=int, default=0,
                     help="dense lookup at this bucket width (0: autotune or search)")
@@ -141,13 +143,13 @@ def main():
         if args.dense_bits:
             space_kw.update(dense=True, bucket_bits=args.dense_bits)
         prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev,
-                          group_split=args.group_split, **space_kw)
+                          group_split=args.group_split, compact=args.compact, **space_kw)
     else:
         prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, tune="auto")
         log(f"autotune: {[t for t in prob.tuned.table]} -> cap={prob.tuned.bucket_cap} "
             f"dense_bits={prob.tuned.dense_bits} "
             f"rows={prob.tuned.rows_per_thread} split={prob.tuned.group_split} "
-            f"({time.time() - t:.1f} s incl. tuning)")
+            f"compact={prob.tuned.compact} ({time.time() - t:.1f} s incl. tuning)")
     prob.op.tune.stage_v = args.stage_v
     torch.cuda.synchronize()
     sp, op = prob.space, prob.op
@@ -218,6 +220,7 @@ def main():
                    "hits": hits, "bucket_bits": sp.bucket_bits, "fine_bits": sp.fine_bits,
                    "rows_per_thread": sp.rows_per_thread, "group_split": op.tune.group_split,
                    "lookup": "dense bitmap" if sp.dense else "search",
+                   "compact_hits": bool(op.tune.compact),
                    "stage_v": args.stage_v, "autotuned": prob.tuned is not None,
                    "parallelism": "single GPU"},
         "matvecs_per_s": round(1e3 / ms, 2),
@@ -229,7 +232,8 @@ def main():
                      "traffic": read_traffic(os.path.join(
                          ROOT, "profiles", f"bucket_join_kernel_{w.name}.md")),
                      "kernel": f"bucket_join_kernel<float2,uint32_t,{sp.rows_per_thread},"
-                               f"{str(bool(args.stage_v)).lower()},{str(sp.dense).lower()}>",
+                               f"{str(bool(args.stage_v)).lower()},{str(sp.dense).lower()},"
+                               f"{str(bool(op.tune.compact)).lower()}>",
                      "bytes_per_launch": b_mv},
         "wall_s_timed": round(wall, 3),
     }
@@ -477,9 +481,9 @@ def bench_batch(args, dev):
     t = time.time()
     best, table = autotune_batch(insts, device=dev)
     log(f"batch autotune {table} -> cap={best[0]} rows={best[1]} split={best[2]} "
-        f"({time.time() - t:.1f} s)")
+        f"compact={best[4]} ({time.time() - t:.1f} s)")
     b = SQDBatch(insts, device=dev, bucket_cap=best[0], rows_per_thread=best[1],
-                 group_split=best[2])
+                 group_split=best[2], compact=best[4])
     # algorithmic bytes, summed over instances
     s_v = 8
     b_mv = 0
@@ -520,7 +524,8 @@ def bench_batch(args, dev):
     errs = [abs(a.eigval - r.eigval) / abs(r.eigval) for a, r in zip(res64.infos, ref.infos)]
     # sequential baseline: the same fused solver, one instance per call
     probs = [SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, bucket_cap=best[0],
-                        rows_per_thread=best[1], group_split=best[2]) for w in ws]
+                        rows_per_thread=best[1], group_split=best[2], compact=best[4])
+             for w in ws]
     starts = [b.segment(x0, i).contiguous() for i in range(len(ws))]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -545,7 +550,8 @@ def bench_batch(args, dev):
         "data": "synthetic (seeded generator, rqutils_amd.workloads.batch_instance)",
         "config": {"workload": f"batch{len(ws)}", "instances": len(ws), "states": b.n,
                    "hits": hits_total, "bucket_cap": best[0], "rows_per_thread": best[1],
-                   "group_split": best[2], "parallelism": "single GPU"},
+                   "group_split": best[2], "compact_hits": bool(best[4]),
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "bucket_join_kernel (batched)",

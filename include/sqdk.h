@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 8
+#define SQDK_ABI_VERSION 9
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -144,6 +144,8 @@ typedef struct sqdk_tune {
                               robin (<= 1: one); needs sqdk_matvec_workspace bytes */
   int32_t xcd_remap;       /* map consecutive items onto one XCD */
   int32_t stage_v;         /* stage each target bucket's slice of v in LDS */
+  int32_t compact;         /* apply hits through a per-wave LDS queue (4 or 8 rows
+                              per thread): one lane per hit instead of one per row */
 } sqdk_tune;
 
 /* ---- library ----------------------------------------------------------- */

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -64,7 +64,7 @@ class SqdkBatch(C.Structure):
 
 class SqdkTune(C.Structure):
     _fields_ = [("rows_per_thread", C.c_int32), ("group_split", C.c_int32),
-                ("xcd_remap", C.c_int32), ("stage_v", C.c_int32)]
+                ("xcd_remap", C.c_int32), ("stage_v", C.c_int32), ("compact", C.c_int32)]
 
 
 class SqdkLocgOpts(C.Structure):

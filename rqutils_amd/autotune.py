@@ -9,7 +9,10 @@ Three knobs decide the matvec's speed on a given problem:
 * the group split, which deals a chunk's groups to several workgroups so that
   small subspaces still fill the 256 CUs (partial sums are reduced after);
 * the lookup: a search in the staged keys, or, where its directory fits
-  (``dense_fits``), the dense bitmap lookup at a few bucket widths.
+  (``dense_fits``), the dense bitmap lookup at a few bucket widths;
+* how hits are applied: by every lane for all its rows (register mode), or
+  through a per-wave LDS queue, one lane per hit (compacted mode), which pays
+  off once a wave has hits in most groups.
 
 ``autotune`` times every candidate with HIP events on the real problem (a few
 matvecs each) and returns the fastest; results are memoised per problem
@@ -47,9 +50,10 @@ class TuneResult:
     group_split: int
     ms: float
     precision: str
-    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits, threads)
+    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits, threads, compact)
     dense_bits: int = 0                         # bucket bits of the dense lookup, 0: search
     block_threads: int = 256
+    compact: int = 0                            # hits through the per-wave LDS queue
 
     def space_kw(self) -> dict:
         kw = {"rows_per_thread": self.rows_per_thread, "block_threads": self.block_threads}
@@ -118,7 +122,7 @@ def _split_candidates(n_items: int):
 
 def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str = "c64",
              caps=DEFAULT_CAPS, rows=DEFAULT_ROWS, dense_bits=DEFAULT_DENSE_BITS,
-             threads=(256,), reps: int = 3, use_cache: bool = True,
+             threads=(256,), compact=(0, 1), reps: int = 3, use_cache: bool = True,
              verbose: bool = False) -> TuneResult:
     """Fastest (lookup, bucket cap or width, rows per thread, group split) for
     this problem. ``threads=(256, 512)`` also times 512-thread workgroups
@@ -169,19 +173,27 @@ def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str 
             continue            # same bucket level as a smaller cap: same kernel
         seen_m[key] = True
         op = PauliOperator(hamiltonian, sp)
-        for split in _split_candidates(sp.n_items):
-            op.set_tune(group_split=split)
-            ms = _time_matvec(op, v, reps)
-            table.append((cap, r, split, round(ms, 4), dm, nt))
-            if verbose:
-                print(f"[autotune] {'dense' if dm else 'cap=%d' % cap} m={sp.bucket_bits} "
-                      f"rows={r} threads={nt} split={split}: {ms:.3f} ms", flush=True)
-            if best is None or ms < best[3]:
-                best = (cap, r, split, ms, dm, nt)
+        for cm in compact:
+            if cm and r not in (4, 8):
+                continue
+            for split in _split_candidates(sp.n_items):
+                op.set_tune(group_split=split, compact=cm)
+                try:
+                    ms = _time_matvec(op, v, reps)
+                except (ValueError, RuntimeError):
+                    continue    # the compacted areas exceed the LDS for this shape
+                table.append((cap, r, split, round(ms, 4), dm, nt, cm))
+                if verbose:
+                    print(f"[autotune] {'dense' if dm else 'cap=%d' % cap} m={sp.bucket_bits} "
+                          f"rows={r} threads={nt} split={split} compact={cm}: {ms:.3f} ms",
+                          flush=True)
+                if best is None or ms < best[3]:
+                    best = (cap, r, split, ms, dm, nt, cm)
         del op, sp
     if best is None:
         raise ValueError("no matvec configuration fits this subspace")
-    res = TuneResult(best[0], best[1], best[2], best[3], precision, table, best[4], best[5])
+    res = TuneResult(best[0], best[1], best[2], best[3], precision, table, best[4], best[5],
+                     best[6])
     _MEMO[sig] = res
     _save_disk(sig, res)
     return res

@@ -39,7 +39,7 @@ class SQDBatch:
     """A batch of (hamiltonian, keys, num_qubits) instances on one GPU."""
 
     def __init__(self, instances, *, device=None, bucket_cap: int = DEFAULT_BUCKET_CAP,
-                 rows_per_thread: int = 4, group_split: int = 1):
+                 rows_per_thread: int = 4, group_split: int = 1, compact: int = 0):
         dev = torch.device(device if device is not None else "cuda")
         if dev.index is None:
             dev = torch.device("cuda", torch.cuda.current_device())
@@ -56,7 +56,7 @@ class SQDBatch:
             self.ops.append(PauliOperator(ham, sp))
         self.rows_per_thread = rows_per_thread
         self._concat()
-        self.tune = _lib.SqdkTune(rows_per_thread, int(group_split), 1, 0)
+        self.tune = _lib.SqdkTune(rows_per_thread, int(group_split), 1, 0, int(bool(compact)))
         self._work = {}
 
     def _concat(self):
@@ -171,8 +171,9 @@ class SQDBatch:
     def segment(self, vec: torch.Tensor, i: int) -> torch.Tensor:
         return vec[self.row_base_host[i]:self.row_base_host[i + 1]]
 
-    def set_tune(self, group_split: int = 1):
+    def set_tune(self, group_split: int = 1, compact: int = 0):
         self.tune.group_split = int(group_split)
+        self.tune.compact = int(bool(compact))
         self._work.clear()
 
     def _workspace(self, prec):
@@ -240,8 +241,9 @@ class SQDBatch:
 
 
 def autotune_batch(instances, *, device=None, caps=(2048, 8192), rows=(4,), splits=(1, 2, 4),
-                   reps: int = 3):
-    """Fastest (bucket cap, rows per thread, group split) of the batched matvec."""
+                   compact=(0, 1), reps: int = 3):
+    """Fastest (bucket cap, rows per thread, group split, compacted hits) of the
+    batched matvec; returns ((cap, rows, split, ms, compact), table)."""
     best = None
     table = []
     for cap in caps:
@@ -250,19 +252,23 @@ def autotune_batch(instances, *, device=None, caps=(2048, 8192), rows=(4,), spli
             g = torch.Generator(device="cpu").manual_seed(0)
             v = torch.complex(torch.randn(b.n, generator=g), torch.randn(b.n, generator=g)).to(
                 torch.complex64).to(b.device)
-            for s in splits:
-                b.set_tune(group_split=s)
-                out = b.matvec(v)
-                e0 = torch.cuda.Event(enable_timing=True)
-                e1 = torch.cuda.Event(enable_timing=True)
-                e0.record()
-                for _ in range(reps):
-                    b.matvec(v, out)
-                e1.record()
-                e1.synchronize()
-                ms = e0.elapsed_time(e1) / reps
-                table.append((cap, r, s, round(ms, 4)))
-                if best is None or ms < best[3]:
-                    best = (cap, r, s, ms)
+            for cm in compact:
+                for s in splits:
+                    try:
+                        b.set_tune(group_split=s, compact=cm)
+                        out = b.matvec(v)
+                    except (ValueError, RuntimeError):
+                        continue        # LDS budget exceeded for this shape
+                    e0 = torch.cuda.Event(enable_timing=True)
+                    e1 = torch.cuda.Event(enable_timing=True)
+                    e0.record()
+                    for _ in range(reps):
+                        b.matvec(v, out)
+                    e1.record()
+                    e1.synchronize()
+                    ms = e0.elapsed_time(e1) / reps
+                    table.append((cap, r, s, round(ms, 4), cm))
+                    if best is None or ms < best[3]:
+                        best = (cap, r, s, ms, cm)
             del b
     return best, table

@@ -105,7 +105,25 @@ __device__ inline void two_sum(float& s, float& e, float x) {
   e += err;
 }
 
+// Partial sum of the products of one batch of terms (the groups of one class
+// of the matvec): p += c v with two FMAs per part. A zero coefficient leaves
+// p unchanged bit for bit (p is never -0: it starts at +0 and only a sum of
+// two -0 gives -0).
+__device__ inline void part_fma(float2& p, float cr, float ci, float2 v) {
+  p.x = __builtin_fmaf(cr, v.x, p.x);
+  p.x = __builtin_fmaf(-ci, v.y, p.x);
+  p.y = __builtin_fmaf(cr, v.y, p.y);
+  p.y = __builtin_fmaf(ci, v.x, p.y);
+}
+__device__ inline void part_fma(double2& p, double cr, double ci, double2 v) {
+  p.x = __builtin_fma(cr, v.x, p.x);
+  p.x = __builtin_fma(-ci, v.y, p.x);
+  p.y = __builtin_fma(cr, v.y, p.y);
+  p.y = __builtin_fma(ci, v.x, p.y);
+}
+
 // Complex accumulator: FP64 for complex128 storage, FP32 two-sum for complex64.
+// `fma` adds one product; `fold` adds a partial (part_fma) and clears it.
 template <typename VT> struct Acc;
 template <> struct Acc<double2> {
   double sr, si;
@@ -113,6 +131,11 @@ template <> struct Acc<double2> {
   __device__ inline void fma(double cr, double ci, double2 v) {
     sr += cr * v.x - ci * v.y;
     si += cr * v.y + ci * v.x;
+  }
+  __device__ inline void fold(double2& p) {
+    sr += p.x;
+    si += p.y;
+    p = make_double2(0.0, 0.0);
   }
   __device__ inline double2 result() const { return make_double2(sr, si); }
 };
@@ -126,6 +149,12 @@ template <> struct Acc<float2> {
     const float pi = __builtin_fmaf(cr, v.y, ci * v.x);
     two_sum(sr, er, pr);
     two_sum(si, ei, pi);
+  }
+  // The partial enters the running sums through one two-sum per part.
+  __device__ inline void fold(float2& p) {
+    two_sum(sr, er, p.x);
+    two_sum(si, ei, p.y);
+    p = make_float2(0.f, 0.f);
   }
   __device__ inline float2 result() const { return make_float2(sr + er, si + ei); }
 };

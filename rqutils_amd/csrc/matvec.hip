@@ -13,6 +13,30 @@
 // LDS per row. Hits (rare) gather v from the target bucket's slice, which is
 // contiguous. Each output element is written once, by one thread, after all
 // groups: no atomics, deterministic summation order.
+//
+// Rows are laid out wave by wave: wave w of a chunk owns its rows
+// [64 R w, 64 R (w + 1)), slot r of lane l being row 64 R w + 64 r + l, so a
+// short chunk (the tail of a bucket) fills whole waves and leaves the others
+// idle instead of thinning out every wave.
+//
+// Summation: the products of one class (the groups sharing a target bucket)
+// are summed into a per-row partial with FMAs; when the class ends, if the
+// wave had any hit in it, every row's partial enters its running sum through
+// one two-sum per part (complex64) or one add (complex128). A miss gathers 0
+// (raw buffer loads return 0 past the slice) and so leaves the partial as it
+// is, which keeps the hit path free of selects.
+//
+// Two ways to apply the hits of a group:
+//  * register mode: every lane forms the group coefficient and the product
+//    for all its rows, hit or not (cheap when the hits are dense enough that
+//    the wave has some in most groups, and when they are so rare that the
+//    wave skips most groups);
+//  * compacted mode: the wave queues its hits in LDS (ballot + lane prefix
+//    count), then lanes take one queued hit each, so the coefficient, gather
+//    and product cost one lane per hit instead of one per row. The class
+//    partials live in LDS, one per row slot. Each hit runs the same
+//    arithmetic as in register mode, in the same group order, and the folds
+//    happen at the same class ends, so both modes give identical bits.
 #include "internal.hpp"
 
 #include <cstring>
@@ -39,6 +63,7 @@ struct MatvecParams {
   int32_t dmax;              // largest fine_bits over all instances (LDS layout)
   int32_t split;             // workgroups sharing one row chunk (groups dealt round robin)
   int32_t threads;           // workgroup size: 256, or 512 with 4 rows per thread
+  int32_t compact;           // hits applied through the per-wave LDS queue
   int64_t n;                 // rows, for the partial-sum layout
   void* partial;             // (split - 1) x n partial sums when split > 1
   sqdk_inst single;          // the instance when no descriptor array is given
@@ -89,30 +114,101 @@ __host__ __device__ inline LdsLayout lds_layout(int fine_bits, int max_bucket, s
 
 template <typename VT> constexpr bool kGatherFirst = sizeof(VT) == 8;
 
+// Compacted mode: per row slot of the chunk, its class partial, one queue
+// entry and its low key bits, placed after the lookup tables.
+struct CompactLayout {
+  size_t acc, queue, rows, total;
+};
+
+__host__ __device__ inline CompactLayout compact_layout(size_t table_bytes, int chunk_rows,
+                                                        size_t acc_bytes, size_t key_bytes) {
+  CompactLayout c;
+  c.acc = (table_bytes + 15) & ~(size_t)15;
+  c.queue = c.acc + acc_bytes * (size_t)chunk_rows;
+  c.rows = c.queue + 4 * (size_t)chunk_rows;
+  c.total = c.rows + key_bytes * (size_t)chunk_rows;
+  return c;
+}
+
+// Orders this wave's LDS writes before its later LDS reads (other lanes read
+// what a lane wrote); no cross-wave synchronisation is implied.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// +1 or -1 by bit 0 of a popcount: the IEEE one with bit 0 moved into the sign.
+__device__ inline float parity_sign(uint32_t pc, float) {
+  return __uint_as_float(0x3f800000u | (pc << 31));
+}
+__device__ inline double parity_sign(uint32_t pc, double) {
+  return __hiloint2double((int)(0x3ff00000u | (pc << 31)), 0);
+}
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename VT> __device__ inline VT vzero();
+template <> __device__ inline float2 vzero<float2>() { return make_float2(0.f, 0.f); }
+template <> __device__ inline double2 vzero<double2>() { return make_double2(0.0, 0.0); }
+
+// Gather from a target slice through a raw buffer resource covering exactly
+// the slice: buffer loads return 0 for an offset at or past the resource's
+// size, so a miss (index -1, a byte offset near 2^32) reads 0 without a
+// clamp or a select, and its product leaves the partial sum unchanged.
+constexpr int kBufferFlags = 0x00020000;  // gfx9 buffer descriptor word 3 (32-bit data)
+
+template <typename VT>
+__device__ inline __amdgpu_buffer_rsrc_t slice_rsrc(const VT* base, int count) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<VT*>(base), (short)0,
+                                           count * (int)sizeof(VT), kBufferFlags);
+}
+__device__ inline float2 gather0(__amdgpu_buffer_rsrc_t r, int idx, float2) {
+  const auto x = __builtin_amdgcn_raw_buffer_load_b64(r, (uint32_t)idx << 3, 0, 0);
+  return make_float2(__uint_as_float(x[0]), __uint_as_float(x[1]));
+}
+__device__ inline double2 gather0(__amdgpu_buffer_rsrc_t r, int idx, double2) {
+  const auto x = __builtin_amdgcn_raw_buffer_load_b128(r, (uint32_t)idx << 4, 0, 0);
+  return make_double2(__hiloint2double((int)x[1], (int)x[0]),
+                      __hiloint2double((int)x[3], (int)x[2]));
+}
+
 // sum_k c_k (-1)^{popcount(z_k & s_r)} for every row r of a thread, terms in
-// index order (the order of group_coef, so results are identical). The term
-// tables are read once per term for all rows.
+// index order (the order of group_coef, so results are identical: adding
+// (+-1) c with one FMA rounds exactly like adding +-c). The term tables are
+// read once per term for all rows.
 // NARROW: every row key of the chunk is below 2^32, so the parity needs only
-// the low words.
+// the low words. Needs t1 > t0 (callers skip groups without terms).
 template <typename CT, typename KT, int ROWS, bool NARROW>
 __device__ inline void rows_coef_n(const uint64_t* __restrict__ zmask, const CT* __restrict__ coef,
                                    int t0, int t1, uint64_t bucket_base, const KT (&sl)[ROWS],
                                    decltype(CT::x) (&cr)[ROWS], decltype(CT::x) (&ci)[ROWS]) {
+  using RT = decltype(CT::x);
+  {
+    // first term: (+-1) c + 0 with one FMA, as adding +-c to zero
+    const uint64_t z = zmask[t0];
+    const CT c = coef[t0];
 #pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-    cr[r] = 0;
-    ci[r] = 0;
+    for (int r = 0; r < ROWS; ++r) {
+      const uint32_t pc = NARROW
+                              ? (uint32_t)__popc((uint32_t)z & ((uint32_t)bucket_base | (uint32_t)sl[r]))
+                              : (uint32_t)__popcll(z & (bucket_base | (uint64_t)sl[r]));
+      const RT sgn = parity_sign(pc, RT(0));
+      cr[r] = fma_t(sgn, c.x, RT(0));
+      ci[r] = fma_t(sgn, c.y, RT(0));
+    }
   }
-  for (int k = t0; k < t1; ++k) {
+  for (int k = t0 + 1; k < t1; ++k) {
     const uint64_t z = zmask[k];
     const CT c = coef[k];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      const int pc = NARROW ? __popc((uint32_t)z & ((uint32_t)bucket_base | (uint32_t)sl[r]))
-                            : __popcll(z & (bucket_base | (uint64_t)sl[r]));
-      const bool odd = (pc & 1) != 0;
-      cr[r] += odd ? -c.x : c.x;
-      ci[r] += odd ? -c.y : c.y;
+      const uint32_t pc = NARROW
+                              ? (uint32_t)__popc((uint32_t)z & ((uint32_t)bucket_base | (uint32_t)sl[r]))
+                              : (uint32_t)__popcll(z & (bucket_base | (uint64_t)sl[r]));
+      const auto sgn = parity_sign(pc, cr[r]);
+      cr[r] = fma_t(sgn, c.x, cr[r]);
+      ci[r] = fma_t(sgn, c.y, ci[r]);
     }
   }
 }
@@ -128,10 +224,83 @@ __device__ inline void rows_coef(const uint64_t* __restrict__ zmask, const CT* _
     rows_coef_n<CT, KT, ROWS, false>(zmask, coef, t0, t1, bucket_base, sl, cr, ci);
 }
 
-// Occupancy: at 4 rows per thread the kernel must keep 5 waves per SIMD
-// (at most 96 VGPRs); the complex128 variant needs the hint to get there.
-template <typename VT, typename KT, int ROWS, bool STAGE_V, bool DENSE, int NT>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(ROWS == 4 ? 5 : 1)))
+// Compacted application of one group's hits for the calling wave. idx[r] is
+// the hit's index in the target slice vsrc (-1: miss) for row slot
+// r * 64 + lane; lq, lrow and lpart are this wave's queue, low key bits and
+// class partial sums (64 * ROWS entries each). A slot is hit at most once per
+// group, so the lanes of one pass never update the same partial. Returns
+// whether the wave had a hit.
+template <typename VT, typename KT, int ROWS>
+__device__ inline bool compact_group(const int (&idx)[ROWS], int lane, uint32_t* __restrict__ lq,
+                                     const KT* __restrict__ lrow, VT* __restrict__ lpart,
+                                     const VT* __restrict__ vsrc,
+                                     const uint64_t* __restrict__ zmask,
+                                     const typename CoefT<VT>::type* __restrict__ coef, int t0,
+                                     int t1, uint64_t bucket_base, bool narrow) {
+  using CT = typename CoefT<VT>::type;
+  using RT = typename CoefT<VT>::real;
+  uint64_t mk[ROWS];
+  int off[ROWS];
+  int total = 0;
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    mk[r] = __ballot(idx[r] >= 0);
+    off[r] = total;
+    total += __popcll(mk[r]);
+  }
+  if (total == 0) return false;
+  // Queue position: hits of slot 0 first, then slot 1, ...; lane order within.
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    if (idx[r] >= 0) {
+      const int below = (int)__builtin_amdgcn_mbcnt_hi(
+          (uint32_t)(mk[r] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk[r], 0u));
+      lq[off[r] + below] = (uint32_t)idx[r] | ((uint32_t)(r * kWave + lane) << 16);
+    }
+  }
+  wave_lds_sync();
+  for (int j = lane; j < total; j += kWave) {
+    const uint32_t e = lq[j];
+    const int slot = (int)(e >> 16);
+    const VT vv = vsrc[e & 0xFFFFu];
+    const KT s[1] = {lrow[slot]};
+    RT cr[1], ci[1];
+    rows_coef<CT, KT, 1>(zmask, coef, t0, t1, bucket_base, narrow, s, cr, ci);
+    VT p = lpart[slot];
+    part_fma(p, cr[0], ci[0], vv);
+    lpart[slot] = p;
+  }
+  wave_lds_sync();
+  return true;
+}
+
+// End of a class with hits in the wave: every row's class partial enters its
+// compensated sum (register partials, or this wave's LDS partials).
+template <typename VT, int ROWS, bool COMPACT>
+__device__ inline void fold_class(Acc<VT> (&acc)[ROWS], VT (&pacc)[ROWS], VT* __restrict__ lpart,
+                                  int lane) {
+  if constexpr (COMPACT) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      VT p = lpart[r * kWave + lane];
+      acc[r].fold(p);
+      lpart[r * kWave + lane] = p;
+    }
+    wave_lds_sync();
+  } else {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) acc[r].fold(pacc[r]);
+  }
+}
+
+// Occupancy: at 4 rows per thread the complex64 kernel with 32-bit LDS keys
+// keeps 7 waves per SIMD (at most 72 VGPRs, no spills), the others 5 (at most
+// 96 VGPRs).
+template <typename VT, typename KT, int ROWS>
+constexpr int kMinWaves = ROWS == 4 ? (sizeof(VT) == 8 && sizeof(KT) == 4 ? 7 : 5) : 1;
+
+template <typename VT, typename KT, int ROWS, bool STAGE_V, bool DENSE, bool COMPACT, int NT>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(kMinWaves<VT, KT, ROWS>)))
 bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
                    const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ fbits,
                    const uint32_t* __restrict__ items,
@@ -146,6 +315,8 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   using RT = typename CoefT<VT>::real;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wslot = (tid / kWave) * (kWave * ROWS);  // the wave's first row slot
   const int logical = a.xcd_remap ? remap_xcd(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int w = logical / a.split;   // row chunk
   const int gs = logical % a.split;  // this workgroup's share of the groups
@@ -172,6 +343,12 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   // Dense lookup: (occupancy bitmap, first-key offset) of every 32-value
   // sub-bucket of the staged target bucket.
   uint2* ldir = reinterpret_cast<uint2*>(smem);
+  // Compacted mode: this wave's class partials, hit queue and low key bits.
+  const CompactLayout cl = compact_layout(DENSE ? ((size_t)8 << a.dmax) : lay.total, ROWS * NT,
+                                          sizeof(VT), sizeof(KT));
+  VT* lpart = reinterpret_cast<VT*>(smem + cl.acc) + wslot;
+  uint32_t* lq = reinterpret_cast<uint32_t*>(smem + cl.queue) + wslot;
+  KT* lrow = reinterpret_cast<KT*>(smem + cl.rows) + wslot;
 
   const uint32_t row0 = items[w];
   const uint64_t bucket = keys[row0] >> m;
@@ -184,15 +361,29 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   KT sl[ROWS];
   bool valid[ROWS];
   Acc<VT> acc[ROWS];
+  // The products of one class (the groups sharing a target bucket) are summed
+  // into a partial per row, which enters the compensated sum at the class end.
+  VT pacc[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) pacc[r] = vzero<VT>();
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    const uint32_t i = row0 + tid + r * NT;
+    const uint32_t i = row0 + wslot + r * kWave + lane;
     valid[r] = i < row1;
     sl[r] = valid[r] ? (KT)(keys[i] & lowmask) : (KT)0;
   }
 
   // Waves of a short chunk that hold no row skip the per-group work.
   const bool wave_active = __any(valid[0]);
+  // Dense lookup: byte offset of each row's 32-value sub-bucket entry; the
+  // target's entry is this XOR the group's (x >> 5) << 3.
+  uint32_t sa[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) {
+    sa[r] = ((uint32_t)sl[r] >> 5) << 3;
+    // keep the precomputed offset: one XOR per lookup instead of a shift and mask
+    if constexpr (DENSE) __asm__ __volatile__("" : "+v"(sa[r]));
+  }
 
   // X = 0 group: the diagonal (first share only), one pass over its terms
   // for all rows of the thread.
@@ -201,7 +392,17 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     rows_coef<CT, KT, ROWS>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base, narrow, sl, cr, ci);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-      if (valid[r]) acc[r].fma(cr[r], ci[r], v[row0 + tid + r * NT]);
+      if (valid[r]) acc[r].fma(cr[r], ci[r], v[row0 + wslot + r * kWave + lane]);
+    }
+  }
+  if constexpr (COMPACT) {
+    if (wave_active) {
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        lpart[r * kWave + lane] = vzero<VT>();
+        lrow[r * kWave + lane] = sl[r];
+      }
+      wave_lds_sync();
     }
   }
 
@@ -225,32 +426,54 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
       __syncthreads();
       if (!wave_active) continue;  // the barriers above are still reached by every wave
       const VT* __restrict__ vq = v + qs;
+      const __amdgpu_buffer_rsrc_t vr = slice_rsrc(vq, qn);
+      bool class_hit = false;
+      // The next group's X mask and term range are loaded one group ahead.
+      uint64_t xm_n = xmask[gfirst];
+      int gt0_n = grp_off[gfirst], gt1_n = grp_off[gfirst + 1];
       for (int g = gfirst; g < g1; g += a.split) {
-        const uint32_t xl = (uint32_t)(xmask[g] & lowmask);
+        const uint64_t xm = xm_n;
+        const int gt0 = gt0_n, gt1 = gt1_n;
+        if (g + a.split < g1) {
+          xm_n = xmask[g + a.split];
+          gt0_n = grp_off[g + a.split];
+          gt1_n = grp_off[g + a.split + 1];
+        }
+        if (gt1 <= gt0) continue;  // a group whose terms all had zero coefficients
+        const uint32_t xl = (uint32_t)(xm & lowmask);
+        const uint32_t xa = (xl >> 5) << 3;
         int idx[ROWS];
         bool any_hit = false;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           // sub-bucket of 32 values: one LDS read, a bit test and a rank
+          // (bit-field extracts use the low 5 bits of t as offset and width);
+          // the rank is formed for every row and masked to -1 on a miss
           const uint32_t t = (uint32_t)sl[r] ^ xl;
-          const uint2 pr = ldir[t >> 5];
-          const uint32_t b = t & 31u;
-          const bool hit = valid[r] && ((pr.x >> b) & 1u);
-          idx[r] = hit ? (int)(pr.y + __popc(pr.x & ((1u << b) - 1u))) : -1;
+          const uint2 pr = *reinterpret_cast<const uint2*>(smem + (sa[r] ^ xa));
+          const uint32_t rank = pr.y + (uint32_t)__popc(__builtin_amdgcn_ubfe(pr.x, 0u, t));
+          const bool hit = valid[r] && __builtin_amdgcn_ubfe(pr.x, t, 1u) != 0u;
+          idx[r] = (int)(rank | (hit ? 0u : 0xFFFFFFFFu));
           any_hit |= hit;
         }
+        if constexpr (COMPACT) {
+          class_hit |= compact_group<VT, KT, ROWS>(idx, lane, lq, lrow, lpart, vq, zmask, coef,
+                                                   gt0, gt1, bucket_base, narrow);
+          continue;
+        }
         if (!__any(any_hit)) continue;
-        // (the dense path has registers to spare: gather first in both precisions)
+        class_hit = true;
+        // (the dense path has registers to spare: gather first in both precisions;
+        // a miss reads 0, and adding c * 0 leaves the partial unchanged)
         VT vv[ROWS];
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) vv[r] = vq[max(idx[r], 0)];
+        for (int r = 0; r < ROWS; ++r) vv[r] = gather0(vr, idx[r], vzero<VT>());
         RT cr[ROWS], ci[ROWS];
-        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, narrow, sl, cr, ci);
+        rows_coef<CT, KT, ROWS>(zmask, coef, gt0, gt1, bucket_base, narrow, sl, cr, ci);
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-          if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
-        }
+        for (int r = 0; r < ROWS; ++r) part_fma(pacc[r], cr[r], ci[r], vv[r]);
       }
+      if (class_hit) fold_class<VT, ROWS, COMPACT>(acc, pacc, lpart, lane);
       (void)qn;
     } else {
       __syncthreads();  // the previous class is done with the LDS tables
@@ -279,28 +502,45 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
       const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
       if (!wave_active) continue;  // the barriers above are still reached by every wave
       const VT* __restrict__ vq = v + qs;
+      const __amdgpu_buffer_rsrc_t vr = slice_rsrc(vq, qn);
+      // a miss (index -1) reads 0 from either source
+      auto gather = [&](int i) -> VT {
+        if constexpr (STAGE_V) return i >= 0 ? lv[i] : vzero<VT>();
+        return gather0(vr, i, vzero<VT>());
+      };
+      bool class_hit = false;
+      // The next group's X mask and term range are loaded one group ahead.
+      uint64_t xm_n = xmask[gfirst];
+      int gt0_n = grp_off[gfirst], gt1_n = grp_off[gfirst + 1];
       for (int g = gfirst; g < g1; g += a.split) {
-        const KT xl = (KT)(xmask[g] & lowmask);
+        const uint64_t xm = xm_n;
+        const int gt0 = gt0_n, gt1 = gt1_n;
+        if (g + a.split < g1) {
+          xm_n = xmask[g + a.split];
+          gt0_n = grp_off[g + a.split];
+          gt1_n = grp_off[g + a.split + 1];
+        }
+        if (gt1 <= gt0) continue;  // a group whose terms all had zero coefficients
+        const KT xl = (KT)(xm & lowmask);
         KT tl[ROWS];
-        int base[ROWS], n[ROWS];
-        bool live[ROWS];
+        int base[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          const uint32_t pr = lpair[(uint32_t)(tl[r] >> f)];
-          base[r] = (int)(pr & 0xFFFFu);
-          n[r] = (int)(pr >> 16) - base[r];
-          live[r] = valid[r] && n[r] > 0;
+          // start of the target's sub-bucket (an empty one at the slice's end
+          // starts on the last key, which cannot equal the target)
+          base[r] = min((int)(lpair[(uint32_t)(tl[r] >> f)] & 0xFFFFu), qn - 1);
         }
-        // Branch-free search for the last key <= target: after ceil(log2(n))
-        // halvings base is that key, so one equality test decides membership.
-        // base stays in [lo, qn], and lkeys[qn] exists, so every read is in range.
-        for (int st = 0; st < steps; ++st) {
+        // Branch-free search for the last key <= target by binary lifting over
+        // 2^steps >= largest sub-bucket positions, each probe clamped to the
+        // slice's last key: keys past the sub-bucket are larger than the
+        // target, so base ends on the target if it is present. Every read is
+        // in [0, qn).
+        for (int st = steps - 1; st >= 0; --st) {
 #pragma unroll
           for (int r = 0; r < ROWS; ++r) {
-            const int half = n[r] >> 1;
-            base[r] = lkeys[base[r] + half] <= tl[r] ? base[r] + half : base[r];
-            n[r] -= half;
+            const int probe = min(base[r] + (1 << st), qn - 1);
+            base[r] = lkeys[probe] <= tl[r] ? probe : base[r];
           }
         }
         int idx[ROWS];
@@ -308,27 +548,34 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           const KT kb = lkeys[base[r]];
-          idx[r] = (live[r] && kb == tl[r]) ? base[r] : -1;
+          idx[r] = (valid[r] && kb == tl[r]) ? base[r] : -1;
           any_hit |= idx[r] >= 0;
         }
+        if constexpr (COMPACT) {
+          class_hit |= compact_group<VT, KT, ROWS>(idx, lane, lq, lrow, lpart, STAGE_V ? lv : vq,
+                                                   zmask, coef, gt0, gt1, bucket_base, narrow);
+          continue;
+        }
         if (!__any(any_hit)) continue;
-        // Hits: gather (a miss reads the slice's first element, in range and
-        // shared by the wave) and one pass over the group's terms for all rows.
+        class_hit = true;
+        // Hits: gather (a miss reads 0, so its product leaves the partial
+        // unchanged) and one pass over the group's terms for all rows.
         // complex64 gathers first to hide the load behind the terms; complex128
         // gathers after them, which keeps 16 fewer VGPRs live (occupancy).
         VT vv[ROWS];
         if constexpr (kGatherFirst<VT>) {
 #pragma unroll
-          for (int r = 0; r < ROWS; ++r) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
+          for (int r = 0; r < ROWS; ++r) vv[r] = gather(idx[r]);
         }
         RT cr[ROWS], ci[ROWS];
-        rows_coef<CT, KT, ROWS>(zmask, coef, grp_off[g], grp_off[g + 1], bucket_base, narrow, sl, cr, ci);
+        rows_coef<CT, KT, ROWS>(zmask, coef, gt0, gt1, bucket_base, narrow, sl, cr, ci);
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-          if constexpr (!kGatherFirst<VT>) vv[r] = STAGE_V ? lv[max(idx[r], 0)] : vq[max(idx[r], 0)];
-          if (idx[r] >= 0) acc[r].fma(cr[r], ci[r], vv[r]);
+          if constexpr (!kGatherFirst<VT>) vv[r] = gather(idx[r]);
+          part_fma(pacc[r], cr[r], ci[r], vv[r]);
         }
       }
+      if (class_hit) fold_class<VT, ROWS, COMPACT>(acc, pacc, lpart, lane);
     }
   }
 
@@ -336,7 +583,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   VT* __restrict__ dst = gs == 0 ? out : reinterpret_cast<VT*>(a.partial) + (int64_t)(gs - 1) * a.n;
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    if (valid[r]) dst[row0 + tid + r * NT] = acc[r].result();
+    if (valid[r]) dst[row0 + wslot + r * kWave + lane] = acc[r].result();
   }
 }
 
@@ -356,10 +603,10 @@ split_reduce_kernel(int64_t n, int nparts, const VT* __restrict__ partial, VT* _
   }
 }
 
-template <typename VT, typename KT, int ROWS, bool SV, bool DENSE, int NT = kBlock>
+template <typename VT, typename KT, int ROWS, bool SV, bool DENSE, bool COMPACT, int NT = kBlock>
 int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds, hipStream_t s) {
   // (t.descs / t.item_inst are NULL for a single instance)
-  auto kern = bucket_join_kernel<VT, KT, ROWS, SV, DENSE, NT>;
+  auto kern = bucket_join_kernel<VT, KT, ROWS, SV, DENSE, COMPACT, NT>;
   if (lds > 65536) {
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -381,20 +628,41 @@ int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds,
   return SQDK_OK;
 }
 
-template <typename VT, typename KT, bool SV, bool DENSE = false>
-int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, size_t lds,
-              hipStream_t s) {
+template <typename VT, typename KT, bool SV, bool DENSE, bool COMPACT>
+int launch_mode(const MatvecTables<VT>& t, const MatvecParams& args, int rows, size_t lds,
+                hipStream_t s) {
   // 512-thread workgroups (2,048-row chunks) exist for 4 rows per thread only.
   if (args.threads == 512) {
     if (rows != 4) return fail(SQDK_EINVAL, "512-thread chunks need 4 rows per thread");
-    return launch_rows<VT, KT, 4, SV, DENSE, 512>(t, args, lds, s);
+    return launch_rows<VT, KT, 4, SV, DENSE, COMPACT, 512>(t, args, lds, s);
   }
   switch (rows) {
-    case 4: return launch_rows<VT, KT, 4, SV, DENSE>(t, args, lds, s);
-    case 8: return launch_rows<VT, KT, 8, SV, DENSE>(t, args, lds, s);
-    case 16: return launch_rows<VT, KT, 16, SV, DENSE>(t, args, lds, s);
+    case 4: return launch_rows<VT, KT, 4, SV, DENSE, COMPACT>(t, args, lds, s);
+    case 8: return launch_rows<VT, KT, 8, SV, DENSE, COMPACT>(t, args, lds, s);
+    case 16:
+      if constexpr (!COMPACT) return launch_rows<VT, KT, 16, SV, DENSE, false>(t, args, lds, s);
+      return fail(SQDK_EINVAL, "compacted hits need 4 or 8 rows per thread");
     default: return fail(SQDK_EINVAL, "rows_per_thread %d unsupported (4, 8 or 16)", rows);
   }
+}
+
+// LDS bytes of one workgroup: the lookup tables, plus the per-row areas of the
+// compacted mode.
+template <typename VT>
+size_t lds_bytes(size_t table_bytes, const MatvecParams& args, int rows, int key_bytes) {
+  if (!args.compact) return table_bytes;
+  return compact_layout(table_bytes, rows * args.threads, sizeof(VT), key_bytes).total;
+}
+
+template <typename VT, typename KT, bool SV, bool DENSE = false>
+int launch_kt(const MatvecTables<VT>& t, const MatvecParams& args, int rows, size_t table_bytes,
+              hipStream_t s) {
+  const size_t lds = lds_bytes<VT>(table_bytes, args, rows, (int)sizeof(KT));
+  if (lds > 160 * 1024)
+    return fail(SQDK_ERANGE, "LDS use of %zu bytes exceeds 160 KiB (tables %zu bytes%s)", lds,
+                table_bytes, args.compact ? ", compacted hits" : "");
+  if (args.compact) return launch_mode<VT, KT, SV, DENSE, true>(t, args, rows, lds, s);
+  return launch_mode<VT, KT, SV, DENSE, false>(t, args, rows, lds, s);
 }
 
 template <typename VT>
@@ -403,15 +671,10 @@ int launch_common(const MatvecTables<VT>& t, MatvecParams& args, int narrow_keys
   if (t.fbits) {
     // Dense lookup: only the (bitmap, offset) table of the target bucket is staged.
     if (stage_v) return fail(SQDK_EINVAL, "v staging is not available with the dense lookup");
-    if (((size_t)8 << args.dmax) > 160 * 1024)
-      return fail(SQDK_ERANGE, "dense lookup table of 2^%d entries exceeds 160 KiB of LDS", args.dmax);
     return launch_kt<VT, uint32_t, false, true>(t, args, rows, (size_t)8 << args.dmax, s);
   }
   const LdsLayout lay = lds_layout(args.dmax, args.max_bucket, narrow_keys ? 4 : 8,
                                    stage_v ? sizeof(VT) : 0);
-  if (lay.total > 160 * 1024)
-    return fail(SQDK_ERANGE, "LDS tables of %zu bytes exceed 160 KiB (largest bucket %d keys)",
-                lay.total, args.max_bucket);
   if (stage_v)
     return narrow_keys ? launch_kt<VT, uint32_t, true>(t, args, rows, lay.total, s)
                        : launch_kt<VT, uint64_t, true>(t, args, rows, lay.total, s);
@@ -421,8 +684,8 @@ int launch_common(const MatvecTables<VT>& t, MatvecParams& args, int narrow_keys
 
 template <typename VT>
 int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const void* v,
-              void* out, int rows, int xcd, int stage_v, int split, void* work,
-              hipStream_t s, const int32_t* skip, int32_t* counter) {
+              void* out, int rows, int xcd, int stage_v, int split, int compact,
+              void* work, hipStream_t s, const int32_t* skip, int32_t* counter) {
   MatvecTables<VT> t;
   t.keys = sp->keys;
   t.bstart = sp->bstart;
@@ -452,6 +715,7 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.dmax = sp->fine_bits;
   args.split = split;
   args.threads = sp->chunk_rows / rows;
+  args.compact = compact;
   args.n = sp->n;
   args.partial = work;
   args.single.bstart_base = 0;
@@ -467,7 +731,7 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
 
 template <typename VT>
 int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out, int rows,
-                 int xcd, int stage_v, int split, void* work, hipStream_t s,
+                 int xcd, int stage_v, int split, int compact, void* work, hipStream_t s,
                  const int32_t* skip, const int32_t* inst_skip, int32_t* counter) {
   MatvecTables<VT> t;
   t.keys = b->keys;
@@ -495,6 +759,7 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
   args.dmax = b->max_fine_bits;
   args.split = split;
   args.threads = b->chunk_rows / rows;
+  args.compact = compact;
   args.n = b->n;
   args.partial = work;
   memset(&args.single, 0, sizeof(args.single));
@@ -531,12 +796,13 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                    space->fine_bits <= space->bucket_bits,
                "fine_bits %d invalid", space->fine_bits);
   if (space->n_items == 0) return SQDK_OK;
-  int rows = 4, xcd = 1, stage_v = 0, split = 1;
+  int rows = 4, xcd = 1, stage_v = 0, split = 1, compact = 0;
   if (tune) {
     if (tune->rows_per_thread) rows = tune->rows_per_thread;
     if (tune->group_split > 1) split = tune->group_split;
     xcd = tune->xcd_remap;
     stage_v = tune->stage_v;
+    compact = tune->compact != 0;
   }
   SQDK_REQUIRE(split <= 1024, "group_split %d too large", split);
   SQDK_REQUIRE(rows * kBlock == space->chunk_rows || (rows == 4 && rows * 512 == space->chunk_rows),
@@ -548,13 +814,13 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                split);
   if (precision == SQDK_C128) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, xcd, stage_v, split, work, s,
-                              skip, counter);
+    return launch_vt<double2>(ham, space, ham->coef, v, out, rows, xcd, stage_v, split, compact,
+                              work, s, skip, counter);
   }
   if (precision == SQDK_C64) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, xcd, stage_v, split, work, s,
-                             skip, counter);
+    return launch_vt<float2>(ham, space, ham->coef32, v, out, rows, xcd, stage_v, split, compact,
+                             work, s, skip, counter);
   }
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }
@@ -575,12 +841,13 @@ int batch_matvec_impl(const sqdk_batch* b, int32_t precision, const void* v, voi
   SQDK_REQUIRE(b->max_bucket_bits >= 0 && b->max_bucket_bits <= 63, "max_bucket_bits %d invalid",
                b->max_bucket_bits);
   if (b->n_items == 0) return SQDK_OK;
-  int rows = 4, xcd = 1, stage_v = 0, split = 1;
+  int rows = 4, xcd = 1, stage_v = 0, split = 1, compact = 0;
   if (tune) {
     if (tune->rows_per_thread) rows = tune->rows_per_thread;
     if (tune->group_split > 1) split = tune->group_split;
     xcd = tune->xcd_remap;
     stage_v = tune->stage_v;
+    compact = tune->compact != 0;
   }
   SQDK_REQUIRE(split <= 1024, "group_split %d too large", split);
   SQDK_REQUIRE(rows * kBlock == b->chunk_rows || (rows == 4 && rows * 512 == b->chunk_rows),
@@ -591,11 +858,11 @@ int batch_matvec_impl(const sqdk_batch* b, int32_t precision, const void* v, voi
                "matvec workspace of %zu bytes, %zu needed for group_split %d", work_bytes, need,
                split);
   if (precision == SQDK_C128)
-    return launch_batch<double2>(b, b->coef, v, out, rows, xcd, stage_v, split, work, s, skip,
-                                 inst_skip, counter);
+    return launch_batch<double2>(b, b->coef, v, out, rows, xcd, stage_v, split, compact, work, s,
+                                 skip, inst_skip, counter);
   if (precision == SQDK_C64)
-    return launch_batch<float2>(b, b->coef32, v, out, rows, xcd, stage_v, split, work, s, skip,
-                                inst_skip, counter);
+    return launch_batch<float2>(b, b->coef32, v, out, rows, xcd, stage_v, split, compact, work,
+                                s, skip, inst_skip, counter);
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }
 

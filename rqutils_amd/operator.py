@@ -303,13 +303,19 @@ class PauliOperator:
         self._cstruct = None
         self._work = {}
 
-    def set_tune(self, group_split: int = 1, stage_v: int = 0, xcd_remap: int = 1):
-        """Launch options of the matvec (rows per thread are fixed by the subspace)."""
+    def set_tune(self, group_split: int = 1, stage_v: int = 0, xcd_remap: int = 1,
+                 compact: int = 0):
+        """Launch options of the matvec (rows per thread are fixed by the subspace).
+        ``compact``: apply hits through the per-wave LDS queue (4 or 8 rows per
+        thread); same bits as the default register mode."""
         if not 1 <= int(group_split) <= 1024:
             raise ValueError(f"group_split={group_split} outside 1..1024")
+        if compact and self.space.rows_per_thread not in (4, 8):
+            raise ValueError("compact hits need 4 or 8 rows per thread")
         self.tune.group_split = int(group_split)
         self.tune.stage_v = int(stage_v)
         self.tune.xcd_remap = int(xcd_remap)
+        self.tune.compact = int(bool(compact))
         self._work.clear()
 
     def work_bytes(self, precision: int) -> int:

@@ -47,7 +47,7 @@ class SQDProblem:
     """
 
     def __init__(self, hamiltonian, keys, num_qubits: Optional[int] = None, *, device=None,
-                 tune=None, group_split: int = 1, **space_kw):
+                 tune=None, group_split: int = 1, compact: int = 0, **space_kw):
         if not isinstance(hamiltonian, PauliSumXZ):
             hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True,
                                                    add_padding=True)
@@ -60,11 +60,12 @@ class SQDProblem:
                                   precision="c64" if tune == "auto" else tune)
             space_kw = {**space_kw, **self.tuned.space_kw()}
             group_split = self.tuned.group_split
+            compact = self.tuned.compact
         if not any(k in space_kw for k in ("dense", "bucket_cap", "bucket_bits", "fine_bits")):
             space_kw["dense"] = None
         self.space = Subspace(keys, nq, device=device, **space_kw)
         self.op = PauliOperator(hamiltonian, self.space)
-        self.op.set_tune(group_split=group_split)
+        self.op.set_tune(group_split=group_split, compact=compact)
 
     @property
     def n(self) -> int:
