@@ -84,7 +84,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heisenberg28",
-                    choices=["heisenberg28", "random40", "small16", "batch64", "netstart"])
+                    choices=["heisenberg28", "random40", "small16", "batch64", "netstart",
+                             "svsim"])
     ap.add_argument("--instances", type=int, default=64, help="instances of batch64")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solution block")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -100,9 +101,10 @@ def main():
                     help="dense lookup at this bucket width (0: autotune or search)")
     ap.add_argument("--no-net", action="store_true", help="skip the network-start block")
     ap.add_argument("--no-intake", action="store_true", help="skip the sample-intake block")
-    ap.add_argument("--net-train", type=int, default=16,
+    ap.add_argument("--no-svsim", action="store_true", help="skip the state-vector block")
+    ap.add_argument("--net-train", type=int, default=100,
                     help="training Hamiltonians of the network-start block")
-    ap.add_argument("--net-test", type=int, default=3, help="held-out Hamiltonians")
+    ap.add_argument("--net-test", type=int, default=25, help="held-out Hamiltonians")
     ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprof byte counters)")
@@ -120,6 +122,8 @@ def main():
     if args.workload == "batch64":
         bench_batch(args, dev)
         return
+    if args.workload == "svsim":
+        return bench_svsim(args, dev)
     if args.workload == "netstart":
         bench_netstart(args, dev)
         return
@@ -242,6 +246,8 @@ def main():
         record["time_to_solution"] = time_to_solution(prob, w, dev)
     if not args.no_intake:
         record["intake"] = intake_block(w, dev)
+    if not args.no_svsim and w.name == "heisenberg28":
+        record["svsim"] = svsim_block(dev, cpu=not args.no_cpu)
     if not args.no_net and w.name == "heisenberg28":
         record["network_start"] = {
             "heisenberg28_2^20": network_start(dev, "heisenberg28_2^20", args.net_train,
@@ -343,21 +349,24 @@ def intake_block(w, dev, repeats: int = 4, reps: int = 3):
             "cpu_sample": f"pack + np.unique of {sample.shape[0]} rows, x{rows / sample.shape[0]:.1f}"}
 
 
-NET_STARTS = ("gaussian", "min_diag", "network_bits", "network_bits_diag")
+NET_STARTS = ("gaussian", "min_diag", "network_bits", "network_bits_mix", "network_bits_diag",
+              "network_bits_diag_mix")
+NET_MIX = 0.3   # norm of the Gaussian part of the *_mix network starts
 
 
 def network_start(dev, family, n_train, n_test, steps):
     """LOBPCG (fused, FP32 vectors, 1e-6 relative residual) on held-out
-    Hamiltonians of a family from four starts: a seeded Gaussian, the one-hot
+    Hamiltonians of a family from six starts: a seeded Gaussian, the one-hot
     at the smallest diagonal entry, the start network on bits, and the start
-    network on bits plus the scaled diagonal energy. Both networks are trained
-    on the FP64 ground vectors of ``n_train`` other members of the family."""
+    network on bits plus two diagonal-energy features, each network alone and
+    with a Gaussian part of norm NET_MIX. Both networks are trained on the
+    FP64 ground vectors of ``n_train`` other members of the family."""
     import functools
     import multiprocessing as mp
     import torch
     from rqutils_amd import workloads as W
     from rqutils_amd.sqd import SQDProblem
-    from rqutils_amd.startnet import StartNet, diag_feature, gaussian_start, ground_targets
+    from rqutils_amd.startnet import StartNet, energy_features, gaussian_start, ground_targets
 
     if family == "heisenberg28_2^20":
         nq, make = 28, functools.partial(W.heisenberg28, n_states=1 << 20)
@@ -380,14 +389,14 @@ def network_start(dev, family, n_train, n_test, steps):
         tg, _, info = ground_targets(p)
         keys.append(p.space.keys)
         tgts.append(tg)
-        feats.append(diag_feature(p))
+        feats.append(energy_features(p))
         if (j + 1) % 25 == 0:
             log(f"[{family}] {j + 1}/{n_train} ground vectors")
     torch.cuda.synchronize()
     ground_s = time.time() - t
     log(f"[{family}] {n_train} FP64 ground vectors ({ground_s:.1f} s)")
     nets, train_s, final_loss = {}, {}, {}
-    for ne in (0, 1):
+    for ne in (0, 2):
         net = StartNet(nq, device=dev, seed=0, n_extra=ne)
         t = time.time()
         losses = net.train(keys, tgts, feats if ne else None, steps=steps, lr=3e-3,
@@ -410,10 +419,11 @@ def network_start(dev, family, n_train, n_test, steps):
                 x0 = gaussian_start(p.n, dev, w.seed)
             elif name == "min_diag":
                 x0 = p.vinit_min_diag()
-            elif name == "network_bits":
-                x0 = nets[name].start_vector(p.space.keys)
             else:
-                x0 = nets[name].start_vector(p.space.keys, diag_feature(p))
+                net_name = name.removesuffix("_mix")
+                mix = NET_MIX if name.endswith("_mix") else 0.0
+                extra = energy_features(p) if net_name == "network_bits_diag" else None
+                x0 = nets[net_name].start_vector(p.space.keys, extra, mix=mix, seed=w.seed)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             _, inf = p.solve(x0, mode="fused", precision="c64", criterion="relative", tol=1e-6,
@@ -446,6 +456,83 @@ def network_start(dev, family, n_train, n_test, steps):
     for k in NET_STARTS[1:]:
         out[k]["speedup_vs_gaussian"] = round(g / out[k]["seconds_mean"], 2)
     return out
+
+
+def svsim_block(dev, n=30, layers=10, cpu=True, cpu_qubits=18):
+    """State-vector simulation (rqutils/svsim.py) of the reference example's
+    GHZ circuit and of seeded random layers on n qubits in complex128, timed
+    with HIP events on the launch stream; the state (2^n amplitudes) is
+    resident before the timed region. HBM bytes per pass: one read and one
+    write of the state (32 B per amplitude)."""
+    import torch
+    from rqutils_amd import workloads as W
+    from rqutils_amd.svsim import apply, to_circuitxz
+    state = torch.zeros(1 << n, dtype=torch.complex128, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    out = {"num_qubits": n, "precision": "c128", "state_bytes": 16 << n}
+    for name, gates in (("ghz", W.ghz_circuit(n)), (f"layers{layers}", W.layered_circuit(n, layers))):
+        c = to_circuitxz(gates)
+        c.num_qubits = n
+        state.zero_()
+        state[0] = 1.0
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        runs = apply(state, c)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        traffic = runs * 32 * (1 << n)
+        rec = {"gates": len(gates), "passes": runs, "ms": round(ms, 3),
+               "gates_per_s": round(len(gates) / (ms * 1e-3), 1),
+               "amplitude_updates_per_s": float(len(gates) * (1 << n) / (ms * 1e-3)),
+               "hbm_gbs": round(traffic / (ms * 1e-3) / 1e9, 1),
+               "hbm_frac": round(traffic / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "per_gate_pass_bytes": 32 * (1 << n) * len(gates)}
+        if name == "ghz":
+            a0 = float(state[0].abs().item())
+            a1 = float(state[-1].abs().item())
+            rest = float(torch.linalg.vector_norm(state[1:-1]).item())
+            rec["check"] = {"abs_first": a0, "abs_last": a1, "norm_rest": rest,
+                            "ok": abs(a0 - 2 ** -0.5) < 1e-12 and abs(a1 - 2 ** -0.5) < 1e-12
+                            and rest < 1e-12}
+        log(f"svsim {name} {n}q: {len(gates)} gates in {runs} passes, {ms:.2f} ms "
+            f"({rec['hbm_gbs']} GB/s of state traffic)")
+        out[name] = rec
+    del state
+    if cpu:
+        from oracle import svsim_oracle as sv
+        g = W.ghz_circuit(cpu_qubits)
+        t0 = time.perf_counter()
+        sv.svsim(*sv.to_circuitxz(g, cpu_qubits))
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"amplitude_updates_per_s": len(g) * (1 << cpu_qubits) / dt,
+                               "cores": 1, "kind": "port",
+                               "sample": f"oracle GHZ circuit on {cpu_qubits} qubits "
+                                         f"({len(g)} gates, {dt:.1f} s)"}
+    return out
+
+
+def bench_svsim(args, dev):
+    """State-vector simulation alone: ``svsim_block`` at 30 qubits."""
+    res = svsim_block(dev, cpu=not args.no_cpu)
+    lay = res["layers10"]
+    record = {"metric": "svsim amplitude updates/s (gate x amplitude), complex128",
+              "value": lay["amplitude_updates_per_s"], "unit": "amplitude updates/s",
+              "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": lay["ms"],
+              "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+              "dtype": "c128", "data": "synthetic (GHZ and seeded random layers)",
+              "config": {"workload": "svsim", "num_qubits": res["num_qubits"],
+                         "parallelism": "single GPU"},
+              "roofline": {"bound": "hbm", "achieved": lay["hbm_gbs"], "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": lay["hbm_frac"], "traffic": None,
+                           "kernel": "svsim_run_kernel<double2,11>"},
+              "svsim": res}
+    if "cpu_baseline" in res:
+        record["cpu_baseline"] = {**res["cpu_baseline"], "value": res["cpu_baseline"][
+            "amplitude_updates_per_s"], "unit": "amplitude updates/s"}
+    print(json.dumps(record), flush=True)
 
 
 def bench_netstart(args, dev):

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 9
+#define SQDK_ABI_VERSION 10
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -393,6 +393,28 @@ int sqdk_colsum(int64_t n, int64_t cols, const float* x, float* out, void* strea
  * refreshing the bf16 copy w_bf16 (may be NULL). */
 int sqdk_adam(int64_t n, float* w, const float* g, float* m, float* v, uint16_t* w_bf16,
               float lr, float beta1, float beta2, float eps, int32_t step, void* stream);
+
+/* ---- state-vector simulation -------------------------------------------- */
+
+/* One gate of a rotation circuit in the symplectic form of CircuitXZ
+ * (rqutils/svsim.py:41-49): the state maps to cos psi + i sin Z^z X^x psi,
+ * out[i] = cos psi[i] + i sin (-1)^{popcount(i & z)} psi[i ^ x]. */
+typedef struct sqdk_gate {
+  uint64_t x;
+  uint64_t z;
+  double cos;
+  double sin;
+} sqdk_gate;
+
+/* Apply gates[0..n_gates) (host array) in order to the 2^num_qubits
+ * amplitudes `state` (device, complex128 for SQDK_C128 or complex64 for
+ * SQDK_C64) in place. Replaces do_svsim (rqutils/svsim.py:96-122); the
+ * initial state is the caller's (rqutils/svsim.py:103-104 builds the one-hot
+ * on the host side here). Consecutive gates whose X masks fit one LDS tile
+ * share one pass over the state; runs_out (may be NULL) receives the number
+ * of passes. num_qubits <= 40; every mask must lie below 2^num_qubits. */
+int sqdk_svsim(int32_t num_qubits, int32_t precision, void* state, const sqdk_gate* gates,
+               int64_t n_gates, int32_t* runs_out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -41,6 +41,10 @@ class SqdkHam(C.Structure):
         ("diag_t0", C.c_int32), ("diag_t1", C.c_int32), ("num_qubits", C.c_int32),
         ("bucket_bits", C.c_int32), ("reserved", C.c_int32),
     ]
+
+
+class SqdkGate(C.Structure):
+    _fields_ = [("x", C.c_uint64), ("z", C.c_uint64), ("cos", C.c_double), ("sin", C.c_double)]
 
 
 class SqdkInst(C.Structure):
@@ -159,6 +163,8 @@ SIGNATURES = {
     "sqdk_adam": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                             C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
                             C.c_void_p]),
+    "sqdk_svsim": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
+                             C.c_void_p, C.c_void_p]),
 }
 
 

@@ -1,0 +1,253 @@
+// State-vector simulation of rotation circuits (rqutils/svsim.py:96-122).
+//
+// A gate in symplectic form (X mask x, Z mask z, cos c, sin s) maps the state
+// to c psi + i s Z^z X^x psi:
+//   out[i] = c psi[i] + i s (-1)^{popcount(i & z)} psi[i ^ x],
+// the update of do_svsim (the cos term is skipped when c == 0, as there).
+//
+// The reference makes one pass over the 2^n amplitudes per gate. Here the
+// host splits the circuit into runs: consecutive gates whose X masks, above
+// the lowest kMinLow qubits, touch at most k - kMinLow qubits. A run's tile
+// qubits Q are those qubits plus the lowest qubits up to k in all; fixing
+// every other qubit selects one tile of 2^k amplitudes, which a workgroup
+// loads into LDS (contiguous runs of at least 2^kMinLow amplitudes, so loads
+// stay coalesced), takes through every gate of the run, and writes back in
+// place. Each run is then one read and one write of the state instead of one
+// per gate. A gate with X on more high qubits than a tile holds takes one
+// global pass of its own (pairs (i, i ^ x) per thread, in place).
+#include "internal.hpp"
+
+#include <vector>
+
+namespace sqdk {
+
+namespace {
+
+constexpr int kSvThreads = 256;
+constexpr int kMinLow = 6;          // contiguous low qubits in every tile
+constexpr int kMaxRunGates = 120;   // gates per launch (kernel argument space)
+
+struct SvGate {
+  uint32_t xl, zl;   // X and Z masks in tile-local bit positions
+  uint64_t zo;       // Z bits outside the tile: a per-tile sign
+  double c, s;
+};
+
+struct SvRun {
+  uint64_t qmask;    // tile qubits
+  int32_t n, k, low; // qubits, tile bits, contiguous low tile bits
+  int32_t ngates;
+  SvGate g[kMaxRunGates];
+};
+
+template <typename VT> struct SvReal;
+template <> struct SvReal<double2> { using type = double; };
+template <> struct SvReal<float2> { using type = float; };
+
+// Bits of t deposited, in order, at the set positions of mask.
+__device__ inline uint64_t deposit(uint64_t t, uint64_t mask) {
+  uint64_t out = 0;
+  while (mask) {
+    const uint64_t low = mask & (~mask + 1);
+    if (t & 1) out |= low;
+    t >>= 1;
+    mask &= mask - 1;
+  }
+  return out;
+}
+
+// out = c a + i s ph b (the cos term only when c != 0, as the reference).
+template <typename VT>
+__device__ inline VT rot(VT a, VT b, typename SvReal<VT>::type c, typename SvReal<VT>::type sp) {
+  VT o;
+  o.x = -sp * b.y;
+  o.y = sp * b.x;
+  if (c != 0) {
+    o.x = c * a.x + o.x;
+    o.y = c * a.y + o.y;
+  }
+  return o;
+}
+
+template <typename VT, int K>
+__global__ void __launch_bounds__(kSvThreads)
+svsim_run_kernel(VT* __restrict__ state, int64_t n_tiles, SvRun run) {
+  using RT = typename SvReal<VT>::type;
+  constexpr int kTile = 1 << K;
+  __shared__ VT tile[kTile];
+  __shared__ uint64_t hoff[1 << (K - kMinLow)];
+  const int tid = threadIdx.x;
+  const int low = run.low;
+  const uint32_t lowmask = (1u << low) - 1u;
+  const uint64_t nmask = run.n >= 64 ? ~0ull : ((1ull << run.n) - 1ull);
+  const uint64_t hmask = run.qmask & ~(uint64_t)lowmask;   // tile qubits above the low block
+  const uint64_t omask = nmask & ~run.qmask;              // qubits that pick the tile
+  for (int h = tid; h < (1 << (run.k - low)); h += kSvThreads) hoff[h] = deposit((uint64_t)h, hmask);
+  __syncthreads();
+  const int size = 1 << run.k;
+  for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint64_t base = deposit((uint64_t)t, omask);
+    for (int e = tid; e < size; e += kSvThreads)
+      tile[e] = state[base | hoff[e >> low] | (uint32_t)(e & lowmask)];
+    __syncthreads();
+    for (int gi = 0; gi < run.ngates; ++gi) {
+      const SvGate G = run.g[gi];
+      const RT c = (RT)G.c, s = (RT)G.s;
+      const int tpar = __popcll(base & G.zo) & 1;
+      if (G.xl == 0) {
+        for (int e = tid; e < size; e += kSvThreads) {
+          const RT sp = ((__popc(e & G.zl) ^ tpar) & 1) ? -s : s;
+          tile[e] = rot(tile[e], tile[e], c, sp);
+        }
+      } else {
+        const int b = __ffs(G.xl) - 1;   // lowest X bit: pairs (l, l ^ x) with bit b clear in l
+        const uint32_t below = (1u << b) - 1u;
+        for (int p = tid; p < size / 2; p += kSvThreads) {
+          const uint32_t l = (((uint32_t)p & ~below) << 1) | ((uint32_t)p & below);
+          const uint32_t m = l ^ G.xl;
+          const VT a = tile[l], bb = tile[m];
+          const RT sl = ((__popc(l & G.zl) ^ tpar) & 1) ? -s : s;
+          const RT sm = ((__popc(m & G.zl) ^ tpar) & 1) ? -s : s;
+          tile[l] = rot(a, bb, c, sl);
+          tile[m] = rot(bb, a, c, sm);
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < size; e += kSvThreads)
+      state[base | hoff[e >> low] | (uint32_t)(e & lowmask)] = tile[e];
+    __syncthreads();  // the next tile reuses the LDS buffer
+  }
+}
+
+// One gate straight in global memory: pairs (i, i ^ x), or single amplitudes
+// when x == 0.
+template <typename VT>
+__global__ void __launch_bounds__(kSvThreads)
+svsim_gate_kernel(VT* __restrict__ state, int n, uint64_t x, uint64_t z, double cd, double sd) {
+  using RT = typename SvReal<VT>::type;
+  const RT c = (RT)cd, s = (RT)sd;
+  const int64_t total = x ? (int64_t)1 << (n - 1) : (int64_t)1 << n;
+  const int b = x ? __ffsll((long long)x) - 1 : 0;
+  const uint64_t below = x ? ((1ull << b) - 1ull) : 0;
+  for (int64_t p = blockIdx.x * (int64_t)kSvThreads + threadIdx.x; p < total;
+       p += (int64_t)gridDim.x * kSvThreads) {
+    if (x == 0) {
+      const RT sp = (__popcll((uint64_t)p & z) & 1) ? -s : s;
+      state[p] = rot(state[p], state[p], c, sp);
+    } else {
+      const uint64_t l = (((uint64_t)p & ~below) << 1) | ((uint64_t)p & below);
+      const uint64_t m = l ^ x;
+      const VT a = state[l], bb = state[m];
+      const RT sl = (__popcll(l & z) & 1) ? -s : s;
+      const RT sm = (__popcll(m & z) & 1) ? -s : s;
+      state[l] = rot(a, bb, c, sl);
+      state[m] = rot(bb, a, c, sm);
+    }
+  }
+}
+
+// Position of each set bit of qmask, ascending: tile bit j sits at qubit pos[j].
+int tile_positions(uint64_t qmask, int* pos) {
+  int j = 0;
+  for (int q = 0; q < 64; ++q)
+    if ((qmask >> q) & 1ull) pos[j++] = q;
+  return j;
+}
+
+uint32_t to_local(uint64_t m, const int* pos, int k) {
+  uint32_t out = 0;
+  for (int j = 0; j < k; ++j)
+    if ((m >> pos[j]) & 1ull) out |= 1u << j;
+  return out;
+}
+
+template <typename VT, int K>
+int launch_runs(int n, VT* state, const sqdk_gate* gates, int64_t n_gates, int32_t* runs_out,
+                hipStream_t s) {
+  const int k = n < K ? n : K;
+  const int min_low = k < kMinLow ? k : kMinLow;
+  const uint64_t lowfix = (1ull << min_low) - 1ull;
+  const uint64_t nmask = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+  const int64_t n_tiles = (int64_t)1 << (n - k);
+  const unsigned grid = (unsigned)std::min<int64_t>(n_tiles, 4096);
+  int runs = 0;
+  int64_t i = 0;
+  while (i < n_gates) {
+    const uint64_t hx0 = gates[i].x & ~lowfix;
+    if (__builtin_popcountll(hx0) > k - min_low) {
+      // X on more high qubits than a tile holds: one global pass.
+      const int64_t total = gates[i].x ? (int64_t)1 << (n - 1) : (int64_t)1 << n;
+      const unsigned g = (unsigned)std::min<int64_t>(ceil_div(total, kSvThreads), 8192);
+      svsim_gate_kernel<VT><<<g, kSvThreads, 0, s>>>(state, n, gates[i].x, gates[i].z,
+                                                      gates[i].cos, gates[i].sin);
+      SQDK_LAUNCHED("svsim_gate_kernel");
+      ++runs;
+      ++i;
+      continue;
+    }
+    // Greedy run: add gates while the union of their high X bits fits.
+    uint64_t h = hx0;
+    int64_t j = i + 1;
+    while (j < n_gates && j - i < kMaxRunGates) {
+      const uint64_t hn = h | (gates[j].x & ~lowfix);
+      if (__builtin_popcountll(hn) > k - min_low) break;
+      h = hn;
+      ++j;
+    }
+    // Tile qubits: the run's high X bits, then the lowest other qubits.
+    uint64_t q = h;
+    for (int b = 0; b < n && __builtin_popcountll(q) < k; ++b) q |= 1ull << b;
+    SvRun run;
+    run.qmask = q & nmask;
+    run.n = n;
+    run.k = k;
+    int low = 0;
+    while (low < k && ((q >> low) & 1ull)) ++low;
+    run.low = low;
+    int pos[64];
+    tile_positions(run.qmask, pos);
+    run.ngates = (int)(j - i);
+    for (int64_t t = i; t < j; ++t) {
+      SvGate& g = run.g[t - i];
+      g.xl = to_local(gates[t].x, pos, k);
+      g.zl = to_local(gates[t].z & run.qmask, pos, k);
+      g.zo = gates[t].z & ~run.qmask;
+      g.c = gates[t].cos;
+      g.s = gates[t].sin;
+    }
+    svsim_run_kernel<VT, K><<<grid, kSvThreads, 0, s>>>(state, n_tiles, run);
+    SQDK_LAUNCHED("svsim_run_kernel");
+    ++runs;
+    i = j;
+  }
+  if (runs_out) *runs_out = runs;
+  return SQDK_OK;
+}
+
+}  // namespace
+
+}  // namespace sqdk
+
+extern "C" int sqdk_svsim(int32_t num_qubits, int32_t precision, void* state,
+                          const sqdk_gate* gates, int64_t n_gates, int32_t* runs_out,
+                          void* stream) {
+  using namespace sqdk;
+  SQDK_REQUIRE(num_qubits >= 1 && num_qubits <= 40, "num_qubits %d outside 1..40", num_qubits);
+  SQDK_REQUIRE(state != nullptr, "NULL state");
+  SQDK_REQUIRE(n_gates >= 0 && (n_gates == 0 || gates), "bad gate list");
+  const uint64_t nmask = (1ull << num_qubits) - 1ull;
+  for (int64_t i = 0; i < n_gates; ++i) {
+    SQDK_REQUIRE(((gates[i].x | gates[i].z) & ~nmask) == 0,
+                 "gate %lld acts on a qubit >= %d", (long long)i, num_qubits);
+  }
+  if (runs_out) *runs_out = 0;
+  if (n_gates == 0) return SQDK_OK;
+  if (precision == SQDK_C128)
+    return launch_runs<double2, 11>(num_qubits, static_cast<double2*>(state), gates, n_gates,
+                                    runs_out, as_stream(stream));
+  if (precision == SQDK_C64)
+    return launch_runs<float2, 12>(num_qubits, static_cast<float2*>(state), gates, n_gates,
+                                   runs_out, as_stream(stream));
+  return fail(SQDK_EINVAL, "unknown precision %d", precision);
+}

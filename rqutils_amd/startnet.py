@@ -21,7 +21,7 @@ import torch
 from . import _lib
 
 __all__ = ["StartNet", "targets_from_vector", "ground_targets", "gaussian_start",
-           "diag_feature"]
+           "diag_feature", "energy_features"]
 
 
 def _p(t):
@@ -70,6 +70,17 @@ def diag_feature(prob) -> torch.Tensor:
     sd = float(d.std().item()) if d.numel() > 1 else 1.0
     f = (d - d.min()) / (sd if sd > 0 else 1.0)
     return f.to(torch.float32).reshape(-1, 1).contiguous()
+
+
+def energy_features(prob) -> torch.Tensor:
+    """Per-state features (n x 2, FP32): the scaled diagonal energy f of
+    ``diag_feature`` and -log10(f + 1e-6) / 6, which is 1 at the subspace's
+    lowest-energy configuration and at most about 0.2 wherever f >= 0.01, so
+    that configurations within a small fraction of a standard deviation of
+    the minimum are told apart from it."""
+    f = diag_feature(prob).to(torch.float64)
+    g = -torch.log10(f + 1e-6) / 6.0
+    return torch.cat([f, g], dim=1).to(torch.float32).contiguous()
 
 
 class StartNet:
@@ -295,10 +306,22 @@ class StartNet:
         return float((ok * w).sum() / w.sum())
 
     def start_vector(self, keys: torch.Tensor, extra: torch.Tensor | None = None,
-                     chunk: int = 1 << 18) -> torch.Tensor:
-        """Normalised complex128 start vector exp(y0) sign(y1) over the keys."""
+                     chunk: int = 1 << 18, mix: float = 0.0, seed: int = 0) -> torch.Tensor:
+        """Normalised complex128 start vector exp(y0) sign(y1) over the keys.
+
+        ``mix`` > 0 adds a seeded Gaussian vector of norm ``mix`` (the network's
+        part keeps norm sqrt(1 - mix^2)): the start then overlaps every
+        eigenvector at least as a Gaussian start does, so a network that
+        concentrates its start on the wrong few states cannot leave LOBPCG near
+        an excited state."""
+        if not 0.0 <= mix < 1.0:
+            raise ValueError("mix must lie in [0, 1)")
         y = self.predict(keys, extra, chunk).to(torch.float64)
         a = y[:, 0] - y[:, 0].max()
         x = torch.exp(a) * torch.where(y[:, 1] > 0, 1.0, -1.0)
-        x = x / torch.linalg.vector_norm(x)
-        return x.to(torch.complex128)
+        x = (x / torch.linalg.vector_norm(x)).to(torch.complex128)
+        if mix > 0.0:
+            g = gaussian_start(x.numel(), x.device, seed)
+            x = math.sqrt(1.0 - mix * mix) * x + mix * g / torch.linalg.vector_norm(g)
+            x = x / torch.linalg.vector_norm(x)
+        return x

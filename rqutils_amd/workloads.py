@@ -269,5 +269,35 @@ def batch_instance(seed: int) -> Workload:
                     {"terms": terms, "sector_ones": n // 2})
 
 
+def ghz_circuit(n: int) -> list:
+    """GHZ preparation (h on qubit 0, then cx from each qubit to the next, as in
+    the reference's example, examples/svsim.py:55-58) written with rx, rz and
+    rzz: h = rz(pi/2) rx(pi/2) rz(pi/2) and cx(c, t) = h_t cz(c, t) h_t with the
+    reference's cz = rzz(pi/2) rz_c(-pi/2) rz_t(-pi/2) (rqutils/svsim.py:148-153),
+    each up to a global phase. Ends in (|0...0> + |1...1>)/sqrt(2) up to a
+    global phase."""
+    h = lambda q: [("rz", q, np.pi / 2), ("rx", q, np.pi / 2), ("rz", q, np.pi / 2)]
+    gates = h(0)
+    for c in range(n - 1):
+        t = c + 1
+        gates += h(t)
+        gates += [("rzz", [c, t], np.pi / 2), ("rz", c, -np.pi / 2), ("rz", t, -np.pi / 2)]
+        gates += h(t)
+    return gates
+
+
+def layered_circuit(n: int, layers: int, seed: int = 0) -> list:
+    """Seeded random layers: rx and rz on every qubit, then rzz on neighbours."""
+    rng = np.random.default_rng(seed)
+    gates = []
+    for _ in range(layers):
+        for q in range(n):
+            gates.append(("rx", q, float(rng.uniform(-np.pi, np.pi))))
+            gates.append(("rz", q, float(rng.uniform(-np.pi, np.pi))))
+        for q in range(n - 1):
+            gates.append(("rzz", [q, q + 1], float(rng.uniform(-np.pi, np.pi))))
+    return gates
+
+
 def make(name: str, **kw) -> Workload:
     return {"small16": small16, "heisenberg28": heisenberg28, "random40": random40}[name](**kw)

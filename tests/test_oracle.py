@@ -272,3 +272,73 @@ def test_mlp_encode_bits():
         bits = format(k, "040b")[::-1]
         assert list(x[i, :40]) == [(-1.0 if b == "1" else 1.0) for b in bits]
         assert not x[i, 40:].any()
+
+
+# ---- state-vector simulator (rqutils/svsim.py) -----------------------------------
+
+def _random_gate(rng, n):
+    name = ["x", "y", "z", "rx", "ry", "rz", "rzz"][rng.integers(0, 7)]
+    k = 2 if name == "rzz" else int(rng.integers(1, min(3, n) + 1))
+    qubits = sorted(rng.choice(n, size=min(k, n), replace=False).tolist())
+    ang = float(rng.uniform(-np.pi, np.pi))
+    return (name, qubits, ang) if name.startswith("r") else (name, qubits)
+
+
+def test_svsim_each_gate_matches_dense():
+    """Every gate (one and several qubits) against its dense Kronecker matrix:
+    pins qubit order, the output-index sign and the reference's phases."""
+    from oracle import svsim_oracle as sv
+    rng = np.random.default_rng(3)
+    n = 4
+    for name in sv.GATE_SPECS:
+        for qubits in ([2], [0, 3], [1, 2, 3]):
+            if name == "rzz" and len(qubits) != 2:
+                continue
+            ang = 0.7
+            gate = (name, qubits, ang) if name.startswith("r") else (name, qubits)
+            psi0 = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+            got = sv.svsim(*sv.to_circuitxz([gate], n), initial_state=psi0)
+            want = sv.gate_matrix_dense(name, qubits, ang, n) @ psi0
+            assert np.allclose(got, want, atol=1e-13), (name, qubits)
+
+
+def test_svsim_circuit_matches_product_of_dense_gates():
+    from oracle import svsim_oracle as sv
+    rng = np.random.default_rng(7)
+    n = 5
+    gates = [_random_gate(rng, n) for _ in range(40)]
+    psi = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    want = psi.copy()
+    for g in gates:
+        want = sv.gate_matrix_dense(g[0], g[1], g[2] if len(g) > 2 else None, n) @ want
+    got = sv.svsim(*sv.to_circuitxz(gates, n), initial_state=psi)
+    assert np.allclose(got, want, atol=1e-12)
+
+
+def test_svsim_conventions():
+    """rx/rz/rzz/x/y/z keep the norm; the reference's ry is
+    cos(a/2) I + sin(a/2) Y (not unitary), x and z carry a -i; integer starts
+    are one-hot; num_qubits defaults to the largest qubit + 1; unknown
+    names raise."""
+    from oracle import svsim_oracle as sv
+    x, z, c, s, n = sv.to_circuitxz([("rx", 0, 0.3), ("rz", 2, 1.1)])
+    assert n == 3 and list(x) == [1, 0] and list(z) == [0, 4]
+    assert np.isclose(c[0], np.cos(-0.15)) and np.isclose(s[0], np.sin(-0.15))
+    one = sv.svsim(*sv.to_circuitxz([("x", 1)], 2), initial_state=0)
+    assert np.allclose(one, [0, 0, -1j, 0])
+    psi = np.array([1.0, 0.0], dtype=complex)
+    ry = sv.svsim(*sv.to_circuitxz([("ry", 0, 0.8)], 1), initial_state=psi)
+    assert np.allclose(ry, [np.cos(0.4), 1j * np.sin(0.4)])   # cos I + sin Y on |0>
+    pm = np.array([1.0, 1.0], dtype=complex) / np.sqrt(2)
+    ry_pm = sv.svsim(*sv.to_circuitxz([("ry", 0, 0.8)], 1), initial_state=pm)
+    assert abs(np.linalg.norm(ry_pm) - 1.0) < 1e-12            # |+> keeps its norm ...
+    pi = np.array([1.0, 1j], dtype=complex) / np.sqrt(2)
+    ry_pi = sv.svsim(*sv.to_circuitxz([("ry", 0, 0.8)], 1), initial_state=pi)
+    assert abs(np.linalg.norm(ry_pi) - 1.0) > 0.1              # ... an eigenvector of Y does not
+    rng = np.random.default_rng(0)
+    gates = [g for g in (_random_gate(rng, 6) for _ in range(60)) if g[0] != "ry"]
+    v = rng.normal(size=64) + 1j * rng.normal(size=64)
+    v /= np.linalg.norm(v)
+    assert abs(np.linalg.norm(sv.svsim(*sv.to_circuitxz(gates, 6), initial_state=v)) - 1) < 1e-12
+    with pytest.raises(ValueError):
+        sv.to_circuitxz([("cz", [0, 1])])
