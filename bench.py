@@ -568,9 +568,9 @@ def bench_batch(args, dev):
     t = time.time()
     best, table = autotune_batch(insts, device=dev)
     log(f"batch autotune {table} -> cap={best[0]} rows={best[1]} split={best[2]} "
-        f"compact={best[4]} ({time.time() - t:.1f} s)")
+        f"compact={best[4]} dense={best[5]} ({time.time() - t:.1f} s)")
     b = SQDBatch(insts, device=dev, bucket_cap=best[0], rows_per_thread=best[1],
-                 group_split=best[2], compact=best[4])
+                 group_split=best[2], compact=best[4], dense=best[5])
     # algorithmic bytes, summed over instances
     s_v = 8
     b_mv = 0
@@ -611,7 +611,8 @@ def bench_batch(args, dev):
     errs = [abs(a.eigval - r.eigval) / abs(r.eigval) for a, r in zip(res64.infos, ref.infos)]
     # sequential baseline: the same fused solver, one instance per call
     probs = [SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=dev, bucket_cap=best[0],
-                        rows_per_thread=best[1], group_split=best[2], compact=best[4])
+                        rows_per_thread=best[1], group_split=best[2], compact=best[4],
+                        dense=best[5])
              for w in ws]
     starts = [b.segment(x0, i).contiguous() for i in range(len(ws))]
     torch.cuda.synchronize()
@@ -638,7 +639,7 @@ def bench_batch(args, dev):
         "config": {"workload": f"batch{len(ws)}", "instances": len(ws), "states": b.n,
                    "hits": hits_total, "bucket_cap": best[0], "rows_per_thread": best[1],
                    "group_split": best[2], "compact_hits": bool(best[4]),
-                   "parallelism": "single GPU"},
+                   "dense_lookup_instances": b.n_dense, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "bucket_join_kernel (batched)",

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 10
+#define SQDK_ABI_VERSION 11
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -105,6 +105,8 @@ typedef struct sqdk_inst {
   int32_t diag_t0, diag_t1; /* term range of its X = 0 group */
   int32_t bucket_bits;
   int32_t fine_bits;
+  int64_t fbits_base;       /* first word of its dense-lookup bitmaps in the batch's fbits
+                               (fine_bits == bucket_bits - 5), or -1: search lookup */
 } sqdk_inst;
 
 /* Many independent instances (each its own Hamiltonian and subspace) laid
@@ -132,9 +134,14 @@ typedef struct sqdk_batch {
   int32_t n_inst;
   int32_t n_items;
   int32_t chunk_rows;        /* rows per chunk: 256 x rows_per_thread */
-  int32_t max_bucket;        /* largest bucket over the batch */
-  int32_t max_fine_bits;
+  int32_t max_bucket;        /* largest bucket over the search-lookup instances (>= 1) */
+  int32_t max_fine_bits;     /* largest fine_bits over the search-lookup instances */
   int32_t max_bucket_bits;   /* <= 32 selects 32-bit LDS keys */
+  const uint32_t* fbits;     /* concatenated dense-lookup bitmaps, or NULL: every instance
+                                searches. With it, one matvec is two launches over the
+                                items, one per lookup, each skipping the other's instances */
+  int32_t dense_fine_bits;   /* largest fine_bits over the dense-lookup instances */
+  int32_t reserved;
 } sqdk_batch;
 
 /* Launch tuning of the bucket-join matvec (0 = library default). */

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -50,7 +50,7 @@ class SqdkGate(C.Structure):
 class SqdkInst(C.Structure):
     _fields_ = [("bstart_base", C.c_int64), ("fdir_base", C.c_int64), ("class_base", C.c_int32),
                 ("n_classes", C.c_int32), ("diag_t0", C.c_int32), ("diag_t1", C.c_int32),
-                ("bucket_bits", C.c_int32), ("fine_bits", C.c_int32)]
+                ("bucket_bits", C.c_int32), ("fine_bits", C.c_int32), ("fbits_base", C.c_int64)]
 
 
 class SqdkBatch(C.Structure):
@@ -62,7 +62,8 @@ class SqdkBatch(C.Structure):
         ("class_off", C.c_void_p), ("inst", C.c_void_p),
         ("n", C.c_int64), ("n_inst", C.c_int32), ("n_items", C.c_int32),
         ("chunk_rows", C.c_int32), ("max_bucket", C.c_int32), ("max_fine_bits", C.c_int32),
-        ("max_bucket_bits", C.c_int32),
+        ("max_bucket_bits", C.c_int32), ("fbits", C.c_void_p), ("dense_fine_bits", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
 

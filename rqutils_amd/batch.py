@@ -39,7 +39,11 @@ class SQDBatch:
     """A batch of (hamiltonian, keys, num_qubits) instances on one GPU."""
 
     def __init__(self, instances, *, device=None, bucket_cap: int = DEFAULT_BUCKET_CAP,
-                 rows_per_thread: int = 4, group_split: int = 1, compact: int = 0):
+                 rows_per_thread: int = 4, group_split: int = 1, compact: int = 0,
+                 dense: bool | None = False):
+        """``dense``: False searches every instance's targets; None gives each
+        instance whose subspace suits it (``operator.dense_fits``) the dense
+        bitmap lookup, and one matvec is then two launches (dense, search)."""
         dev = torch.device(device if device is not None else "cuda")
         if dev.index is None:
             dev = torch.device("cuda", torch.cuda.current_device())
@@ -51,7 +55,7 @@ class SQDBatch:
         self.spaces, self.ops = [], []
         for ham, keys, nq in instances:
             sp = Subspace(keys, nq, device=dev, bucket_cap=bucket_cap,
-                          rows_per_thread=rows_per_thread)
+                          rows_per_thread=rows_per_thread, dense=dense)
             self.spaces.append(sp)
             self.ops.append(PauliOperator(ham, sp))
         self.rows_per_thread = rows_per_thread
@@ -61,7 +65,7 @@ class SQDBatch:
 
     def _concat(self):
         dev = self.device
-        keys, bstart, fdir, items, item_inst = [], [], [], [], []
+        keys, bstart, fdir, items, item_inst, fbits = [], [], [], [], [], []
         xmask, grp_off, class_hi, class_off = [], [], [], []
         gz, gc, gc32, dz, dc, dc32 = [], [], [], [], [], []
         descs = (_lib.SqdkInst * len(self.spaces))()
@@ -70,8 +74,8 @@ class SQDBatch:
         # Term layout: every instance's group terms first (so that group ranges
         # stay contiguous across instances), then every instance's X = 0 terms.
         total_group_terms = sum(op.n_terms - op.diag_terms for op in self.ops)
-        b_off = f_off = g_off = gt_off = dt_off = c_off = 0
-        max_bucket = max_fine = max_bits = 0
+        b_off = f_off = g_off = gt_off = dt_off = c_off = fb_off = 0
+        max_bucket = max_fine = max_bits = dense_fine = 0
         for i, (sp, op) in enumerate(zip(self.spaces, self.ops)):
             r0 = row_base[-1]
             keys.append(sp.keys)
@@ -99,6 +103,15 @@ class SQDBatch:
             d.diag_t1 = total_group_terms + dt_off + nd
             d.bucket_bits = sp.bucket_bits
             d.fine_bits = sp.fine_bits
+            if sp.dense:
+                d.fbits_base = fb_off
+                fbits.append(sp.fbits)
+                fb_off += sp.fbits.numel()
+                dense_fine = max(dense_fine, sp.fine_bits)
+            else:
+                d.fbits_base = -1           # search lookup
+                max_bucket = max(max_bucket, sp.max_bucket)
+                max_fine = max(max_fine, sp.fine_bits)
             b_off += sp.bstart.numel()
             f_off += sp.fdir.numel()
             g_off += op.n_groups
@@ -107,8 +120,6 @@ class SQDBatch:
             c_off += op.n_classes
             row_base.append(r0 + sp.n)
             item_base.append(item_base[-1] + sp.n_items)
-            max_bucket = max(max_bucket, sp.max_bucket)
-            max_fine = max(max_fine, sp.fine_bits)
             max_bits = max(max_bits, sp.bucket_bits)
         zmask = gz + dz
         coef = gc + dc
@@ -161,9 +172,13 @@ class SQDBatch:
         s.n_inst = self.n_inst
         s.n_items = self.n_items
         s.chunk_rows = 256 * self.rows_per_thread
-        s.max_bucket = max_bucket
+        s.max_bucket = max(1, max_bucket)
         s.max_fine_bits = max_fine
         s.max_bucket_bits = max_bits
+        self.fbits = torch.cat(fbits) if fbits else None
+        s.fbits = self.fbits.data_ptr() if fbits else None
+        s.dense_fine_bits = dense_fine
+        self.n_dense = len(fbits)
         self.cstruct = s
 
     # -- views -------------------------------------------------------------------
@@ -241,34 +256,36 @@ class SQDBatch:
 
 
 def autotune_batch(instances, *, device=None, caps=(2048, 8192), rows=(4,), splits=(1, 2, 4),
-                   compact=(0, 1), reps: int = 3):
-    """Fastest (bucket cap, rows per thread, group split, compacted hits) of the
-    batched matvec; returns ((cap, rows, split, ms, compact), table)."""
+                   compact=(0, 1), dense=(False, None), reps: int = 3):
+    """Fastest (bucket cap, rows per thread, group split, compacted hits, dense
+    lookup) of the batched matvec; returns ((cap, rows, split, ms, compact,
+    dense), table). ``dense`` None lets every suitable instance use the dense
+    lookup."""
     best = None
     table = []
-    for cap in caps:
-        for r in rows:
-            b = SQDBatch(instances, device=device, bucket_cap=cap, rows_per_thread=r)
-            g = torch.Generator(device="cpu").manual_seed(0)
-            v = torch.complex(torch.randn(b.n, generator=g), torch.randn(b.n, generator=g)).to(
-                torch.complex64).to(b.device)
-            for cm in compact:
-                for s in splits:
-                    try:
-                        b.set_tune(group_split=s, compact=cm)
-                        out = b.matvec(v)
-                    except (ValueError, RuntimeError):
-                        continue        # LDS budget exceeded for this shape
-                    e0 = torch.cuda.Event(enable_timing=True)
-                    e1 = torch.cuda.Event(enable_timing=True)
-                    e0.record()
-                    for _ in range(reps):
-                        b.matvec(v, out)
-                    e1.record()
-                    e1.synchronize()
-                    ms = e0.elapsed_time(e1) / reps
-                    table.append((cap, r, s, round(ms, 4), cm))
-                    if best is None or ms < best[3]:
-                        best = (cap, r, s, ms, cm)
-            del b
+    shapes = [(cap, r, dn) for cap in caps for r in rows for dn in dense]
+    for cap, r, dn in shapes:
+        b = SQDBatch(instances, device=device, bucket_cap=cap, rows_per_thread=r, dense=dn)
+        g = torch.Generator(device="cpu").manual_seed(0)
+        v = torch.complex(torch.randn(b.n, generator=g), torch.randn(b.n, generator=g)).to(
+            torch.complex64).to(b.device)
+        for cm in compact:
+            for s in splits:
+                try:
+                    b.set_tune(group_split=s, compact=cm)
+                    out = b.matvec(v)
+                except (ValueError, RuntimeError):
+                    continue        # LDS budget exceeded for this shape
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(reps):
+                    b.matvec(v, out)
+                e1.record()
+                e1.synchronize()
+                ms = e0.elapsed_time(e1) / reps
+                table.append((cap, r, s, round(ms, 4), cm, dn))
+                if best is None or ms < best[3]:
+                    best = (cap, r, s, ms, cm, dn)
+        del b
     return best, table

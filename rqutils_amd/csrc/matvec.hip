@@ -327,6 +327,9 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   const int inst_id = item_inst ? (int)item_inst[w] : 0;
   if (a.inst_skip && a.inst_skip[inst_id] != 0) return;
   const sqdk_inst D = descs ? descs[inst_id] : a.single;
+  // A mixed batch runs one launch per lookup: each skips the other's instances.
+  if (descs && (D.fbits_base >= 0) != DENSE) return;
+  const uint32_t* __restrict__ fb = DENSE ? fbits + D.fbits_base : fbits;
   const uint32_t* __restrict__ bst = bstart + D.bstart_base;
   const uint16_t* __restrict__ fdr = fdir + D.fdir_base;
 
@@ -422,7 +425,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     if constexpr (DENSE) {
       __syncthreads();  // the previous class is done with the LDS table
       const uint64_t fbase = q << d;
-      for (int e = tid; e < ndir; e += NT) ldir[e] = make_uint2(fbits[fbase + e], fdr[fbase + e]);
+      for (int e = tid; e < ndir; e += NT) ldir[e] = make_uint2(fb[fbase + e], fdr[fbase + e]);
       __syncthreads();
       if (!wave_active) continue;  // the barriers above are still reached by every wave
       const VT* __restrict__ vq = v + qs;
@@ -618,13 +621,18 @@ int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds,
                                            t.zmask, t.coef, t.class_hi, t.class_off, t.v, t.out,
                                            t.descs, t.item_inst, args);
   SQDK_LAUNCHED("bucket_join_kernel");
-  if (args.split > 1) {
-    const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(args.n, kBlock), 4096));
-    split_reduce_kernel<VT><<<g, kBlock, 0, s>>>(args.n, args.split - 1,
-                                                 reinterpret_cast<const VT*>(args.partial), t.out,
-                                                 args.skip);
-    SQDK_LAUNCHED("split_reduce_kernel");
-  }
+  return SQDK_OK;
+}
+
+// After every launch of a matvec: out += the other shares' partial sums.
+template <typename VT>
+int reduce_split(const MatvecTables<VT>& t, const MatvecParams& args, hipStream_t s) {
+  if (args.split <= 1) return SQDK_OK;
+  const unsigned g = (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(args.n, kBlock), 4096));
+  split_reduce_kernel<VT><<<g, kBlock, 0, s>>>(args.n, args.split - 1,
+                                               reinterpret_cast<const VT*>(args.partial), t.out,
+                                               args.skip);
+  SQDK_LAUNCHED("split_reduce_kernel");
   return SQDK_OK;
 }
 
@@ -726,7 +734,9 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.single.diag_t1 = h->diag_t1;
   args.single.bucket_bits = sp->bucket_bits;
   args.single.fine_bits = sp->fine_bits;
-  return launch_common<VT>(t, args, sp->bucket_bits <= 32, rows, stage_v, s);
+  args.single.fbits_base = 0;
+  SQDK_TRY(launch_common<VT>(t, args, sp->bucket_bits <= 32, rows, stage_v, s));
+  return reduce_split<VT>(t, args, s);
 }
 
 template <typename VT>
@@ -763,7 +773,17 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
   args.n = b->n;
   args.partial = work;
   memset(&args.single, 0, sizeof(args.single));
-  return launch_common<VT>(t, args, b->max_bucket_bits <= 32, rows, stage_v, s);
+  if (b->fbits) {
+    // dense-lookup instances first (32-bit LDS keys), then the searched ones
+    MatvecTables<VT> td = t;
+    td.fbits = b->fbits;
+    MatvecParams ad = args;
+    ad.dmax = b->dense_fine_bits;
+    ad.counter = nullptr;
+    SQDK_TRY(launch_common<VT>(td, ad, 1, rows, 0, s));
+  }
+  SQDK_TRY(launch_common<VT>(t, args, b->max_bucket_bits <= 32, rows, stage_v, s));
+  return reduce_split<VT>(t, args, s);
 }
 
 }  // namespace
@@ -840,6 +860,8 @@ int batch_matvec_impl(const sqdk_batch* b, int32_t precision, const void* v, voi
                b->max_fine_bits);
   SQDK_REQUIRE(b->max_bucket_bits >= 0 && b->max_bucket_bits <= 63, "max_bucket_bits %d invalid",
                b->max_bucket_bits);
+  SQDK_REQUIRE(!b->fbits || (b->dense_fine_bits >= 0 && b->dense_fine_bits <= 15),
+               "dense_fine_bits %d invalid", b->dense_fine_bits);
   if (b->n_items == 0) return SQDK_OK;
   int rows = 4, xcd = 1, stage_v = 0, split = 1, compact = 0;
   if (tune) {

@@ -82,3 +82,52 @@ def test_batch_graph_and_direct_agree(gpu):
     for a, c in zip(r1.infos, r2.infos):
         assert a.niter == c.niter and a.eigval == c.eigval
     assert torch.equal(r1.vectors, r2.vectors)
+
+
+def _mixed_instances():
+    """Two instances whose subspaces suit the dense lookup (>= 2^16 keys in a
+    2^17 or 2^18 key space) and three small searched ones."""
+    probs = [small_problem(300, n=17, terms=50, nstates=70000),
+             small_problem(301, n=12, terms=40, nstates=900),
+             small_problem(302, n=18, terms=60, nstates=80000),
+             small_problem(303, n=10, terms=30, nstates=500),
+             small_problem(304, n=13, terms=45, nstates=2000)]
+    return probs
+
+
+@pytest.mark.parametrize("split", [1, 2])
+def test_batch_mixed_dense_and_search_equals_single(gpu, split):
+    """A batch whose suitable instances take the dense lookup runs two
+    launches per matvec; every instance still gets exactly its own matvec."""
+    from rqutils_amd.batch import SQDBatch
+    from rqutils_amd.operator import PauliOperator, Subspace
+    probs = _mixed_instances()
+    b = SQDBatch([((p.labels, p.coeffs), p.keys, p.n) for p in probs], bucket_cap=1024,
+                 rows_per_thread=4, group_split=split, dense=None)
+    assert b.n_dense == 2 and [sp.dense for sp in b.spaces] == [True, False, True, False, False]
+    rng = np.random.default_rng(split)
+    v = _rand(rng, b.n)
+    for dtype in (torch.complex128, torch.complex64):
+        vt = torch.from_numpy(v).to(dtype).cuda()
+        out = b.matvec(vt)
+        for i, p in enumerate(probs):
+            sp = Subspace(p.keys, p.n, bucket_cap=1024, rows_per_thread=4, dense=None)
+            op = PauliOperator((p.labels, p.coeffs), sp)
+            op.set_tune(group_split=split)
+            ref = op.matvec(b.segment(vt, i).contiguous())
+            assert torch.equal(b.segment(out, i), ref), (i, dtype)
+            if dtype == torch.complex128 and i in (0, 1):
+                want = p.matvec(b.segment(vt, i).cpu().numpy())
+                seg = b.segment(out, i).cpu().numpy()
+                assert np.linalg.norm(seg - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def test_batch_mixed_solve(gpu):
+    from rqutils_amd.batch import SQDBatch
+    probs = _mixed_instances()
+    b = SQDBatch([((p.labels, p.coeffs), p.keys, p.n) for p in probs], dense=None)
+    x0 = torch.from_numpy(_rand(np.random.default_rng(0), b.n)).cuda()
+    res = b.solve(x0, precision="c128", tol=1e-9, maxiter=4000, use_graph=True)
+    for p, info in zip(probs, res.infos):
+        ev = p.ground()
+        assert info.converged and abs(info.eigval - ev) < 1e-8 * abs(ev), (p.n, info, ev)
