@@ -527,7 +527,7 @@ def bench_svsim(args, dev):
                          "parallelism": "single GPU"},
               "roofline": {"bound": "hbm", "achieved": lay["hbm_gbs"], "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": lay["hbm_frac"], "traffic": None,
-                           "kernel": "svsim_run_kernel<double2,11>"},
+                           "kernel": "svsim_reg_kernel<double2,11,3>"},
               "svsim": res}
     if "cpu_baseline" in res:
         record["cpu_baseline"] = {**res["cpu_baseline"], "value": res["cpu_baseline"][

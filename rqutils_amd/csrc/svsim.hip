@@ -13,8 +13,11 @@
 // loads into LDS (contiguous runs of at least 2^kMinLow amplitudes, so loads
 // stay coalesced), takes through every gate of the run, and writes back in
 // place. Each run is then one read and one write of the state instead of one
-// per gate. A gate with X on more high qubits than a tile holds takes one
-// global pass of its own (pairs (i, i ^ x) per thread, in place).
+// per gate. Within a full tile the amplitudes sit in registers (8 or 16 per
+// thread) and most gates never touch LDS (svsim_reg_kernel); tiles of fewer
+// qubits apply every gate in LDS (svsim_run_kernel). A gate with X on more
+// high qubits than a tile holds takes one global pass of its own (pairs
+// (i, i ^ x) per thread, in place).
 #include "internal.hpp"
 
 #include <vector>
@@ -147,6 +150,154 @@ svsim_gate_kernel(VT* __restrict__ state, int n, uint64_t x, uint64_t z, double 
   }
 }
 
+// ---- register-resident runs --------------------------------------------------
+//
+// The tile's 2^K amplitudes are spread over the 256 threads, 2^R each: a
+// layout picks R local bit positions (the register bits); thread t holds the
+// amplitudes whose other local bits spell t. Z-only gates and single-qubit X
+// gates on a register bit then need no LDS traffic at all: the sign of an
+// amplitude is parity(t & zt) ^ parity(e & zr) ^ the tile's outside parity,
+// the second term uniform across the workgroup. The registers go through LDS
+// only when a gate's X bit is not a register bit (a new layout) or a gate
+// flips several tile bits (applied in LDS as in svsim_run_kernel).
+
+enum SvOpKind : uint32_t { kOpDiag = 0, kOpPair = 1, kOpLds = 2, kOpLoad = 3, kOpStore = 4 };
+
+struct SvOp {
+  uint32_t xl, zl;   // tile-local X and Z masks (LDS gates)
+  uint64_t zo;       // Z bits outside the tile
+  double c, s;
+  uint32_t kind;
+  uint8_t j;         // pair: the register bit X flips
+  uint8_t zr;        // register ops: Z bits on the register positions (element coordinates)
+  uint16_t zt;       // register ops: Z bits on the thread positions (thread-id coordinates)
+  uint8_t rpos[4];   // load: local position of each register bit, ascending
+  uint32_t pad;
+};
+
+constexpr int kMaxOps = 80;   // ops per launch (kernel argument space)
+
+struct SvRunReg {
+  uint64_t qmask;
+  int32_t n, k, low, nops;
+  SvOp op[kMaxOps];
+};
+
+template <typename VT, int R>
+__device__ inline void reg_rotate_all(VT (&reg)[1 << R], typename SvReal<VT>::type c,
+                                      typename SvReal<VT>::type st, uint32_t zr) {
+#pragma unroll
+  for (int e = 0; e < (1 << R); ++e) {
+    const auto sp = (__popc((uint32_t)e & zr) & 1) ? -st : st;
+    reg[e] = rot(reg[e], reg[e], c, sp);
+  }
+}
+
+template <typename VT, int R, int B>
+__device__ inline void reg_pairs(VT (&reg)[1 << R], typename SvReal<VT>::type c,
+                                 typename SvReal<VT>::type st, uint32_t zr) {
+#pragma unroll
+  for (int e = 0; e < (1 << R); ++e) {
+    if (e & B) continue;
+    const int f = e | B;
+    const auto sl = (__popc((uint32_t)e & zr) & 1) ? -st : st;
+    const auto sm = (__popc((uint32_t)f & zr) & 1) ? -st : st;
+    const VT a = reg[e], b = reg[f];
+    reg[e] = rot(a, b, c, sl);
+    reg[f] = rot(b, a, c, sm);
+  }
+}
+
+template <typename VT, int K, int R>
+__global__ void __launch_bounds__(kSvThreads)
+svsim_reg_kernel(VT* __restrict__ state, int64_t n_tiles, SvRunReg run) {
+  using RT = typename SvReal<VT>::type;
+  constexpr int kTile = 1 << K;
+  constexpr int E = 1 << R;
+  static_assert(kTile == kSvThreads * E, "a tile is 256 threads x 2^R amplitudes");
+  __shared__ VT tile[kTile];
+  __shared__ uint64_t hoff[1 << (K - kMinLow)];
+  const int tid = threadIdx.x;
+  const int low = run.low;
+  const uint32_t lowmask = (1u << low) - 1u;
+  const uint64_t nmask = run.n >= 64 ? ~0ull : ((1ull << run.n) - 1ull);
+  const uint64_t hmask = run.qmask & ~(uint64_t)lowmask;
+  const uint64_t omask = nmask & ~run.qmask;
+  for (int h = tid; h < (1 << (K - low)); h += kSvThreads) hoff[h] = deposit((uint64_t)h, hmask);
+  __syncthreads();
+  VT reg[E];
+  for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const uint64_t base = deposit((uint64_t)t, omask);
+    for (int e = tid; e < kTile; e += kSvThreads)
+      tile[e] = state[base | hoff[e >> low] | (uint32_t)(e & lowmask)];
+    uint32_t tb = 0;                  // thread part of the local index in the current layout
+    uint32_t rp[4] = {0, 0, 0, 0};    // register bit positions of the current layout
+    for (int oi = 0; oi < run.nops; ++oi) {
+      const SvOp& O = run.op[oi];
+      const RT c = (RT)O.c, s = (RT)O.s;
+      if (O.kind == kOpLoad) {
+        __syncthreads();
+        uint32_t v = (uint32_t)tid;
+#pragma unroll
+        for (int i = 0; i < R; ++i) {  // insert a zero bit at each register position
+          rp[i] = O.rpos[i];
+          v = ((v >> rp[i]) << (rp[i] + 1)) | (v & ((1u << rp[i]) - 1u));
+        }
+        tb = v;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          uint32_t d = 0;
+#pragma unroll
+          for (int i = 0; i < R; ++i) d |= (uint32_t)((e >> i) & 1) << rp[i];
+          reg[e] = tile[tb | d];
+        }
+      } else if (O.kind == kOpStore) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          uint32_t d = 0;
+#pragma unroll
+          for (int i = 0; i < R; ++i) d |= (uint32_t)((e >> i) & 1) << rp[i];
+          tile[tb | d] = reg[e];
+        }
+      } else if (O.kind == kOpLds) {
+        __syncthreads();
+        const int tpar = __popcll(base & O.zo) & 1;
+        const int b = __ffs(O.xl) - 1;
+        const uint32_t below = (1u << b) - 1u;
+        for (int p = tid; p < kTile / 2; p += kSvThreads) {
+          const uint32_t l = (((uint32_t)p & ~below) << 1) | ((uint32_t)p & below);
+          const uint32_t m = l ^ O.xl;
+          const VT a = tile[l], bb = tile[m];
+          const RT sl = ((__popc(l & O.zl) ^ tpar) & 1) ? -s : s;
+          const RT sm = ((__popc(m & O.zl) ^ tpar) & 1) ? -s : s;
+          tile[l] = rot(a, bb, c, sl);
+          tile[m] = rot(bb, a, c, sm);
+        }
+      } else {
+        // register gate: the thread's sign, then the element's (uniform) parity
+        const int pt = (__popc((uint32_t)tid & O.zt) ^ __popcll(base & O.zo)) & 1;
+        const RT st = pt ? -s : s;
+        if (O.kind == kOpDiag) {
+          reg_rotate_all<VT, R>(reg, c, st, O.zr);
+        } else {
+          switch (O.j) {
+            case 0: reg_pairs<VT, R, 1>(reg, c, st, O.zr); break;
+            case 1: reg_pairs<VT, R, 2>(reg, c, st, O.zr); break;
+            case 2: reg_pairs<VT, R, 4>(reg, c, st, O.zr); break;
+            default:
+              if constexpr (R > 3) reg_pairs<VT, R, 8>(reg, c, st, O.zr);
+              break;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    for (int e = tid; e < kTile; e += kSvThreads)
+      state[base | hoff[e >> low] | (uint32_t)(e & lowmask)] = tile[e];
+    __syncthreads();  // the next tile reuses the LDS buffer
+  }
+}
+
 // Position of each set bit of qmask, ascending: tile bit j sits at qubit pos[j].
 int tile_positions(uint64_t qmask, int* pos) {
   int j = 0;
@@ -162,7 +313,105 @@ uint32_t to_local(uint64_t m, const int* pos, int k) {
   return out;
 }
 
-template <typename VT, int K>
+// Bits of m at the positions pos[0..cnt), packed in that order.
+uint32_t extract_bits(uint32_t m, const int* pos, int cnt) {
+  uint32_t out = 0;
+  for (int i = 0; i < cnt; ++i)
+    if ((m >> pos[i]) & 1u) out |= 1u << i;
+  return out;
+}
+
+// Plans one run (tile qubits qmask, full tiles of 2^K) as register ops and
+// launches it, one launch per kMaxOps ops; every launch is one pass over the
+// state. A layout takes the X bit the gate needs, then the next gates' single
+// X bits, then the highest tile bits (the low bits stay thread bits, which
+// keeps the LDS accesses of a layout change contiguous across lanes).
+template <typename VT, int K, int R>
+int launch_reg_run(VT* state, int n, int low, uint64_t qmask, const int* pos,
+                   const sqdk_gate* g, int64_t ng, int64_t n_tiles, unsigned grid, int& passes,
+                   hipStream_t s) {
+  static_assert(K - R == 8, "256 threads spell the thread bits");
+  SvRunReg run;
+  run.qmask = qmask;
+  run.n = n;
+  run.k = K;
+  run.low = low;
+  run.nops = 0;
+  std::vector<uint32_t> xl(ng), zl(ng);
+  for (int64_t t = 0; t < ng; ++t) {
+    xl[t] = to_local(g[t].x, pos, K);
+    zl[t] = to_local(g[t].z & qmask, pos, K);
+  }
+  int rset[4] = {0, 0, 0, 0};   // register bit positions of the current layout, ascending
+  int tset[K];                  // the thread bit positions
+  bool have = false;
+  auto push = [&](const SvOp& op) { run.op[run.nops++] = op; };
+  auto store = [&]() {
+    SvOp op{};
+    op.kind = kOpStore;
+    push(op);
+    have = false;
+  };
+  auto flush = [&]() -> int {
+    if (have) store();
+    if (run.nops == 0) return SQDK_OK;
+    svsim_reg_kernel<VT, K, R><<<grid, kSvThreads, 0, s>>>(state, n_tiles, run);
+    SQDK_LAUNCHED("svsim_reg_kernel");
+    ++passes;
+    run.nops = 0;
+    return SQDK_OK;
+  };
+  auto choose = [&](int b, int64_t t) {
+    uint32_t set = b >= 0 ? 1u << b : 0u;
+    for (int64_t u = t + 1; u < ng && u < t + 64 && __builtin_popcount(set) < R; ++u)
+      if (__builtin_popcount(xl[u]) == 1) set |= xl[u];
+    for (int p = K - 1; p >= 0 && __builtin_popcount(set) < R; --p) set |= 1u << p;
+    int ri = 0, ti = 0;
+    for (int p = 0; p < K; ++p) {
+      if ((set >> p) & 1u) rset[ri++] = p;
+      else tset[ti++] = p;
+    }
+    SvOp op{};
+    op.kind = kOpLoad;
+    for (int i = 0; i < R; ++i) op.rpos[i] = (uint8_t)rset[i];
+    push(op);
+    have = true;
+  };
+  for (int64_t t = 0; t < ng; ++t) {
+    if (run.nops + 4 > kMaxOps) SQDK_TRY(flush());   // room for store, load, op, store
+    const int bits = __builtin_popcount(xl[t]);
+    SvOp op{};
+    op.xl = xl[t];
+    op.zl = zl[t];
+    op.zo = g[t].z & ~qmask;
+    op.c = g[t].cos;
+    op.s = g[t].sin;
+    if (bits <= 1) {
+      const int b = bits ? __builtin_ctz(xl[t]) : -1;
+      bool in = have;
+      if (in && b >= 0) {
+        in = false;
+        for (int i = 0; i < R; ++i) in = in || rset[i] == b;
+      }
+      if (!in) {
+        if (have) store();
+        choose(b, t);
+      }
+      op.kind = bits ? kOpPair : kOpDiag;
+      for (int i = 0; i < R; ++i)
+        if (rset[i] == b) op.j = (uint8_t)i;
+      op.zr = (uint8_t)extract_bits(zl[t], rset, R);
+      op.zt = (uint16_t)extract_bits(zl[t], tset, K - R);
+    } else {
+      if (have) store();
+      op.kind = kOpLds;
+    }
+    push(op);
+  }
+  return flush();
+}
+
+template <typename VT, int K, int R>
 int launch_runs(int n, VT* state, const sqdk_gate* gates, int64_t n_gates, int32_t* runs_out,
                 hipStream_t s) {
   const int k = n < K ? n : K;
@@ -207,6 +456,13 @@ int launch_runs(int n, VT* state, const sqdk_gate* gates, int64_t n_gates, int32
     run.low = low;
     int pos[64];
     tile_positions(run.qmask, pos);
+    if (k == K) {
+      const int rc = launch_reg_run<VT, K, R>(state, n, low, run.qmask, pos, gates + i, j - i,
+                                              n_tiles, grid, runs, s);
+      if (rc != SQDK_OK) return rc;
+      i = j;
+      continue;
+    }
     run.ngates = (int)(j - i);
     for (int64_t t = i; t < j; ++t) {
       SvGate& g = run.g[t - i];
@@ -244,10 +500,10 @@ extern "C" int sqdk_svsim(int32_t num_qubits, int32_t precision, void* state,
   if (runs_out) *runs_out = 0;
   if (n_gates == 0) return SQDK_OK;
   if (precision == SQDK_C128)
-    return launch_runs<double2, 11>(num_qubits, static_cast<double2*>(state), gates, n_gates,
-                                    runs_out, as_stream(stream));
+    return launch_runs<double2, 11, 3>(num_qubits, static_cast<double2*>(state), gates, n_gates,
+                                       runs_out, as_stream(stream));
   if (precision == SQDK_C64)
-    return launch_runs<float2, 12>(num_qubits, static_cast<float2*>(state), gates, n_gates,
-                                   runs_out, as_stream(stream));
+    return launch_runs<float2, 12, 4>(num_qubits, static_cast<float2*>(state), gates, n_gates,
+                                      runs_out, as_stream(stream));
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }

@@ -70,10 +70,11 @@ def test_svsim_runs_split_and_global_passes(gpu):
     ref = sv.svsim(*sv.to_circuitxz(gates, n), initial_state=psi)
     got, runs = _run(gates, n, psi)
     assert _rel(got, ref) < 1e-12
-    # 18 rx: 12 high qubits in runs of at most 5 -> 3 runs (the last one also
-    # takes the low gates up to the 120-gate launch limit); 300 low gates ->
-    # 3 launches in all; the wide x alone; the last three gates one more run
-    assert 6 <= runs <= 10, runs
+    # 18 rx: 12 high qubits in runs of at most 5 -> 3 runs; the 300 low gates
+    # (many of them multi-qubit X gates applied in LDS, with register layout
+    # changes around them) split into launches of at most 80 ops; the wide x
+    # alone; the last three gates one more run. Far fewer passes than gates.
+    assert 6 <= runs <= 16, runs
 
 
 def test_svsim_one_hot_and_norm(gpu):
