@@ -457,7 +457,9 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
           const uint32_t rank = pr.y + (uint32_t)__popc(__builtin_amdgcn_ubfe(pr.x, 0u, t));
           const bool hit = valid[r] && __builtin_amdgcn_ubfe(pr.x, t, 1u) != 0u;
           idx[r] = (int)(rank | (hit ? 0u : 0xFFFFFFFFu));
-          any_hit |= hit;
+          // the wave's decision reads idx, so both words of the entry come in one
+          // 64-bit LDS read before it (the rank is not deferred to the hit path)
+          any_hit |= idx[r] >= 0;
         }
         if constexpr (COMPACT) {
           class_hit |= compact_group<VT, KT, ROWS>(idx, lane, lq, lrow, lpart, vq, zmask, coef,
