@@ -363,3 +363,50 @@ def test_matvec_512_thread_chunks_match_256(gpu, dense, split):
     assert _rel(outs[1], p.matvec(v)) < 1e-12
     with pytest.raises(ValueError):
         Subspace(p.keys, p.n, rows_per_thread=8, block_threads=512)
+
+
+@pytest.mark.parametrize("dense_lookup,compact", [(False, 0), (True, 0), (False, 1), (True, 1)])
+def test_zero_sum_terms_and_empty_group(gpu, dense_lookup, compact):
+    """Duplicate strings that cancel: the X=110 group (XXI, YYZ) loses every
+    term, so its range is empty and the kernel skips it; the X=100 group keeps
+    XII and XIZ around a cancelled XZI (the reference's get_diagonal would stop
+    at that zero, quirk Q3; the intended operator keeps XIZ). Checked against
+    the dense Kronecker sum, which has no notion of groups. The dense lookup
+    needs 32-value sub-buckets, so the strings act on the low 3 of 6 qubits."""
+    from rqutils_amd.operator import PauliOperator, Subspace
+    base = ["XXI", "XXI", "YYZ", "YYZ", "ZIZ", "XII", "XZI", "XZI", "XIZ", "IXX"]
+    coeffs = [1.0, -1.0, 0.3, -0.3, 0.5, 1.0, 0.5, -0.5, 2.0, 0.7]
+    n = 6
+    labels = ["III" + lab for lab in base]
+    keys = np.arange(2 ** n, dtype=np.uint64)
+    kw = dict(dense=True, bucket_bits=5) if dense_lookup else dict(bucket_bits=3)
+    sp = Subspace(keys, n, rows_per_thread=4, **kw)
+    op = PauliOperator((labels, coeffs), sp)
+    op.set_tune(compact=compact)
+    grp = op.grp_off.cpu().numpy()
+    assert np.any(grp[1:] == grp[:-1]), "expected an empty group"
+    v = _rand_vec(np.random.default_rng(4), keys.size)
+    got = op.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+    assert _rel(got, dense.dense_hamiltonian(labels, coeffs) @ v) < 1e-12
+
+
+def test_ground_locg_maxiter_and_exact_eigenvector(gpu):
+    """maxiter=1 stops after one loop iteration (rqutils/ground_locg.py:366-372);
+    an exact eigenvector start has a zero residual, which the norm guards
+    (ground_locg.py:276, 388) must turn into a clean stop, not NaN."""
+    from rqutils_amd.ground_locg import ground_locg
+    rng = np.random.default_rng(8)
+    n = 40
+    a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    a = a + a.conj().T
+    w, u = np.linalg.eigh(a)
+    at = torch.from_numpy(a).cuda()
+    x0 = torch.from_numpy(rng.normal(size=n) + 0j).cuda()
+    ev, vec, it = ground_locg(at, x0, maxiter=1)
+    assert it == 1 and np.isfinite(ev)
+    for mode in ("reference", "fused"):
+        ev, vec, it = ground_locg(at, torch.from_numpy(u[:, 0].copy()).cuda(), maxiter=50,
+                                  mode=mode)
+        vv = vec.cpu().numpy()
+        assert np.all(np.isfinite(vv)) and abs(ev - w[0]) < 1e-10 * abs(w[0]), (mode, ev, it)
+        assert abs(abs(np.vdot(u[:, 0], vv)) / np.linalg.norm(vv) - 1.0) < 1e-10

@@ -152,3 +152,14 @@ def test_workload_generators_small():
     assert walls.mean() < np.count_nonzero(np.diff(ub, axis=1), axis=1).mean()
     b = W.batch_instance(5)
     assert 20 <= b.num_qubits <= 32 and 2 ** 12 - 1 <= b.n <= 2 ** 20 + 1
+
+
+def test_sqd_argument_errors():
+    """Host-side errors of the reference's sqd() (rqutils/sqd.py:222-225):
+    states_size below the number of rows, and an unknown cache_level."""
+    from rqutils_amd.sqd import sqd
+    states = np.array([[0, 1], [1, 0]], dtype=np.uint8)
+    with pytest.raises(ValueError):
+        sqd((["XX", "ZZ"], [1.0, 0.5]), states, states_size=1)
+    with pytest.raises(ValueError):
+        sqd((["XX", "ZZ"], [1.0, 0.5]), states, cache_level=(2, 0))

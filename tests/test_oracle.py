@@ -342,3 +342,53 @@ def test_svsim_conventions():
     assert abs(np.linalg.norm(sv.svsim(*sv.to_circuitxz(gates, 6), initial_state=v)) - 1) < 1e-12
     with pytest.raises(ValueError):
         sv.to_circuitxz([("cz", [0, 1])])
+
+
+# ---- property tests (hypothesis) --------------------------------------------------
+
+from hypothesis import given, settings, strategies as st  # noqa: E402
+
+
+@st.composite
+def _pauli_problem(draw):
+    n = draw(st.integers(1, 5))
+    terms = draw(st.integers(1, 8))
+    labels = [draw(st.text("IXYZ", min_size=n, max_size=n)) for _ in range(terms)]
+    coeffs = [draw(st.floats(-2, 2, allow_nan=False, allow_infinity=False)) for _ in range(terms)]
+    keys = sorted(draw(st.sets(st.integers(0, 2 ** n - 1), min_size=1, max_size=2 ** n)))
+    return labels, coeffs, np.asarray(keys, np.uint64), n
+
+
+@settings(max_examples=60, deadline=None)
+@given(_pauli_problem())
+def test_oracle_matvec_is_projected_dense_sum(problem):
+    """For any strings (duplicates, zeros, all-identity included) and any
+    subspace, the oracle's sort-join matvec equals P H P from Kronecker
+    products, and the projected operator is Hermitian."""
+    labels, coeffs, keys, n = problem
+    if all(c == 0 for c in coeffs):
+        return
+    p = OracleProblem(labels, coeffs, keys, n)
+    v = np.random.default_rng(len(keys)).normal(size=keys.size) + 0j
+    want = dense.projected(dense.dense_hamiltonian(labels, coeffs), keys.astype(np.int64)) @ v
+    assert np.allclose(p.matvec(v), want, atol=1e-12)
+    m = dense.projected(dense.dense_hamiltonian(labels, coeffs), keys.astype(np.int64))
+    assert np.allclose(m, m.conj().T)
+
+
+@settings(max_examples=60, deadline=None)
+@given(st.integers(1, 10).flatmap(lambda n: st.tuples(
+    st.just(n), st.sets(st.integers(0, 2 ** n - 1), min_size=1, max_size=2 ** n),
+    st.integers(0, 2 ** n - 1))))
+def test_oracle_xsource_is_a_dictionary_lookup(case):
+    """get_xsource (the sort-join of rqutils/sqd.py:476-500) is, for every
+    subspace and X mask, the index of s ^ x in the sorted subspace or -1."""
+    from rqutils_amd.operator import keys_to_states
+    n, keyset, x = case
+    keys = np.asarray(sorted(keyset), np.uint64)
+    packed = so.pack_states(keys_to_states(keys, n))
+    xs = so.pack_states(keys_to_states(np.asarray([x], np.uint64), n))[0]
+    got = so.get_xsource(xs, packed)
+    pos = {int(k): i for i, k in enumerate(keys)}
+    want = np.array([pos.get(int(k) ^ x, -1) for k in keys])
+    assert np.array_equal(got, want)
