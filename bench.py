@@ -308,9 +308,16 @@ def time_to_solution(prob, w, dev):
 def intake_block(w, dev, repeats: int = 4, reps: int = 3):
     """Sample intake on the device: ``repeats * N`` state rows (every key of
     the workload at least once, the rest drawn with replacement, shuffled)
-    resident in HBM as uint8 [rows, n]; timed: pack into keys, radix sort and
-    unique. The result must be the workload's keys bit for bit. CPU baseline:
-    numpy (pack + np.unique) on a 2^22-row sample, scaled by the row count."""
+    resident in HBM as uint8 [rows, n]; timed: pack into keys and unique
+    (occupancy bitmap for keys of at most 28 bits, radix sort above). The
+    result must be the workload's keys bit for bit. CPU baseline: numpy
+    (pack + np.unique) on a 2^22-row sample, scaled by the row count.
+    Algorithmic bytes: the rows read and their keys written (n + 8 per row),
+    then for the bitmap the grouping by slice (24 per row: histogram read,
+    scatter read and write; none up to 20 bits), the grouped keys read (8 per
+    row), the 2^n / 8-byte bitmap written, counted and read, and 8 per
+    distinct key written; for the radix sort 24 per row per 8-bit pass and 24
+    for the unique."""
     import torch
     from rqutils_amd.intake import pack_states, unique_keys, unpack_keys
     from rqutils_amd.operator import states_to_keys
@@ -340,10 +347,21 @@ def intake_block(w, dev, repeats: int = 4, reps: int = 3):
     t0 = time.perf_counter()
     np.unique(states_to_keys(sample))
     cpu_s = (time.perf_counter() - t0) * rows / sample.shape[0]
+    bitmap = nq <= 28 and ((1 << nq) // 8 <= 4 << 20 or (1 << nq) // 8 <= 32 * rows)
+    if bitmap:
+        algo = (rows * (nq + 8 + (24 if nq > 20 else 0) + 8) + 3 * max((1 << nq) // 8, 4)
+                + 8 * u.numel())
+    else:
+        algo = rows * (nq + 8 + 24 * -(-nq // 8) + 24)
+    gbs = algo / (ms * 1e-3) / 1e9
     log(f"intake: {rows} rows -> {u.numel()} distinct in {ms:.2f} ms on the GPU "
-        f"({rows / (ms * 1e-3) / 1e9:.2f} G rows/s), bit-exact={ok}; numpy {cpu_s:.2f} s (scaled)")
+        f"({rows / (ms * 1e-3) / 1e9:.2f} G rows/s, {'bitmap' if bitmap else 'radix sort'}, "
+        f"{gbs:.0f} GB/s algorithmic), bit-exact={ok}; numpy {cpu_s:.2f} s (scaled)")
     return {"rows": rows, "distinct": int(u.numel()), "ms": round(ms, 3),
             "rows_per_s": round(rows / (ms * 1e-3), 1), "bit_exact_vs_workload_keys": ok,
+            "method": "bitmap" if bitmap else "radix_sort",
+            "algorithmic_bytes": int(algo), "achieved_gbs": round(gbs, 1),
+            "hbm_frac": round(gbs / 8000.0, 3),
             "state_bytes": rows * nq,
             "cpu_numpy_s_scaled": round(cpu_s, 3),
             "cpu_sample": f"pack + np.unique of {sample.shape[0]} rows, x{rows / sample.shape[0]:.1f}"}

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 11
+#define SQDK_ABI_VERSION 12
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -193,11 +193,14 @@ int sqdk_intake_pack(const uint8_t* states, int64_t n_rows, int32_t num_qubits,
                      int64_t row_stride, uint64_t* keys, void* stream);
 
 /* Sorted distinct keys: out[0..*n_unique) (out holds n entries; *n_unique is
- * written on the device). A stable LSD radix sort over the low key_bits bits
- * (keys must be below 2^key_bits) and a compaction; replaces the lexsort +
- * unique of uniquify_states (sqd.py:414-445), whose fill rows are not
- * materialised. Workspace: sqdk_intake_unique_workspace. */
-int sqdk_intake_unique_workspace(int64_t n, size_t* bytes);
+ * written on the device, -1 if some key is not below 2^key_bits). Replaces the
+ * lexsort + unique of uniquify_states (sqd.py:414-445), whose fill rows are
+ * not materialised. Up to 36-bit keys, when 2^key_bits / 8 bytes are at most
+ * 64 MiB or 32 bytes per key, an occupancy bitmap gives the result without a
+ * sort; otherwise a stable LSD radix sort over the low key_bits bits and a
+ * compaction. Workspace: sqdk_intake_unique_workspace (it depends on both n
+ * and key_bits). */
+int sqdk_intake_unique_workspace(int64_t n, int32_t key_bits, size_t* bytes);
 int sqdk_intake_unique(const uint64_t* keys, int64_t n, int32_t key_bits, uint64_t* out,
                        int64_t* n_unique, void* work, size_t work_bytes, void* stream);
 

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -109,7 +109,8 @@ SIGNATURES = {
                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "sqdk_intake_pack": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_int64, C.c_void_p,
                                    C.c_void_p]),
-    "sqdk_intake_unique_workspace": (C.c_int, [C.c_int64, C.POINTER(C.c_size_t)]),
+    "sqdk_intake_unique_workspace": (C.c_int, [C.c_int64, C.c_int32,
+                                                C.POINTER(C.c_size_t)]),
     "sqdk_intake_unique": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_size_t, C.c_void_p]),
     "sqdk_intake_unpack": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p,

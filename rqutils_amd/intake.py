@@ -3,11 +3,12 @@
 The reference packs sampled state rows behind a pad bit, sorts them with a
 multi-key ``lax.sort`` and removes duplicates (``uniquify_states``), filling
 the tail with 0xFF rows; ``sqd`` unpacks the result into ``basis_states``.
-Here rows become uint64 keys on the device (``pack_states``), a stable radix
-sort plus compaction gives the sorted distinct keys (``unique_keys``;
-the fill rows are not materialised: ``states_size`` only enters the stopping
-rule), and ``unpack_keys`` turns keys back into uint8 rows. All three run in
-the HIP library (``sqdk_intake_*``) and are bit-exact.
+Here rows become uint64 keys on the device (``pack_states``), an occupancy
+bitmap (keys of at most 36 bits) or a stable radix sort plus compaction gives
+the sorted distinct keys (``unique_keys``; the fill rows are not
+materialised: ``states_size`` only enters the stopping rule), and
+``unpack_keys`` turns keys back into uint8 rows. All three run in the HIP
+library (``sqdk_intake_*``) and are bit-exact.
 """
 from __future__ import annotations
 
@@ -53,7 +54,9 @@ def pack_states(states, device=None) -> torch.Tensor:
 
 
 def unique_keys(keys: torch.Tensor, key_bits: int = 64) -> torch.Tensor:
-    """Sorted distinct keys (every key below ``2**key_bits``)."""
+    """Sorted distinct keys (every key below ``2**key_bits``, else
+    ``ValueError``). Narrow key spaces go through an occupancy bitmap, wide
+    ones through a radix sort; both give the same keys."""
     dev = keys.device
     n = keys.numel()
     if n == 0:
@@ -61,14 +64,18 @@ def unique_keys(keys: torch.Tensor, key_bits: int = 64) -> torch.Tensor:
     keys = keys.contiguous()
     L = _lib.lib()
     nb = C.c_size_t(0)
-    _lib.check(L.sqdk_intake_unique_workspace(n, C.byref(nb)), "sqdk_intake_unique_workspace")
+    _lib.check(L.sqdk_intake_unique_workspace(n, int(key_bits), C.byref(nb)),
+               "sqdk_intake_unique_workspace")
     work = torch.empty(nb.value, dtype=torch.uint8, device=dev)
     out = torch.empty(n, dtype=torch.int64, device=dev)
     count = torch.empty(1, dtype=torch.int64, device=dev)
     _lib.check(L.sqdk_intake_unique(keys.data_ptr(), n, int(key_bits), out.data_ptr(),
                                     count.data_ptr(), work.data_ptr(), nb.value,
                                     _lib.stream_ptr(dev)), "sqdk_intake_unique")
-    return out[: int(count.item())]
+    m = int(count.item())
+    if m < 0:
+        raise ValueError(f"a key is not below 2**{key_bits}")
+    return out[:m]
 
 
 def uniquify_states(states, device=None) -> torch.Tensor:
