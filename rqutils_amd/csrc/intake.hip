@@ -295,40 +295,74 @@ radix_scatter_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out
   }
 }
 
-__device__ inline bool first_of_run(const uint64_t* __restrict__ k, int64_t i) {
-  return i == 0 || k[i] != k[i - 1];
+// Unique over sorted keys, with the scatter's wave-striped layout: wave w of
+// tile t takes keys t * 4096 + w * 1024 + j * 64 + lane, j = 0..15 (coalesced).
+// A key starts a run when it is the first or differs from its predecessor.
+__device__ inline void run_starts(const uint64_t* __restrict__ k, int64_t n, int64_t base,
+                                 int lane, uint64_t (&key)[kSortItems],
+                                 bool (&first)[kSortItems]) {
+  uint64_t prev[kSortItems];
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t i = min(base + j * 64 + lane, n - 1);
+    const int64_t ip = i > 0 ? i - 1 : 0;
+    key[j] = k[i];
+    prev[j] = k[ip];
+  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) {
+    const int64_t i = base + j * 64 + lane;
+    first[j] = i < n && (i == 0 || key[j] != prev[j]);
+  }
 }
 
-// tile_cnt[t] = distinct-key starts in tile t of the sorted keys.
+// tile_cnt[t] = run starts in tile t of the sorted keys.
 __global__ void __launch_bounds__(kSortThreads)
 unique_count_kernel(const uint64_t* __restrict__ k, int64_t n, uint32_t* __restrict__ tile_cnt) {
-  __shared__ uint32_t ws[kSortThreads / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSortItems;
+  __shared__ uint32_t ws[kSortWaves + 1];
+  const int lane = threadIdx.x & 63;
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)(threadIdx.x >> 6) * (64 * kSortItems);
+  uint64_t key[kSortItems];
+  bool first[kSortItems];
+  run_starts(k, n, base, lane, key, first);
   uint32_t s = 0;
-  for (int j = 0; j < kSortItems; ++j) {
-    const int64_t i = base + j;
-    if (i < n && first_of_run(k, i)) ++s;
-  }
+#pragma unroll
+  for (int j = 0; j < kSortItems; ++j) s += first[j] ? 1u : 0u;
   uint32_t tot;
   block_exclusive_scan(s, ws, tot);
   if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
 }
 
+// Writes the run starts of tile t from out[tile_off[t]] on, in order: wave
+// totals are scanned over the tile, then each step's starts are placed by a
+// ballot and mbcnt.
 __global__ void __launch_bounds__(kSortThreads)
 unique_write_kernel(const uint64_t* __restrict__ k, int64_t n,
                     const uint32_t* __restrict__ tile_off, uint64_t* __restrict__ out) {
-  __shared__ uint32_t ws[kSortThreads / 64 + 1];
-  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kSortItems;
-  uint32_t s = 0;
+  __shared__ uint32_t wsum[kSortWaves];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)w * (64 * kSortItems);
+  uint64_t key[kSortItems];
+  bool first[kSortItems];
+  run_starts(k, n, base, lane, key, first);
+  uint64_t bal[kSortItems];
+  uint32_t t = 0;
+#pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    const int64_t i = base + j;
-    if (i < n && first_of_run(k, i)) ++s;
+    bal[j] = __ballot(first[j]);
+    t += (uint32_t)__popcll(bal[j]);
   }
-  uint32_t tot;
-  uint32_t pos = tile_off[blockIdx.x] + block_exclusive_scan(s, ws, tot);
+  if (lane == 0) wsum[w] = t;
+  __syncthreads();
+  uint32_t pos = tile_off[blockIdx.x];
+  for (int v = 0; v < w; ++v) pos += wsum[v];
+#pragma unroll
   for (int j = 0; j < kSortItems; ++j) {
-    const int64_t i = base + j;
-    if (i < n && first_of_run(k, i)) out[pos++] = k[i];
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi(
+        (uint32_t)(bal[j] >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bal[j], 0u));
+    if (first[j]) out[pos + below] = key[j];
+    pos += (uint32_t)__popcll(bal[j]);
   }
 }
 
