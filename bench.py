@@ -269,6 +269,7 @@ def time_to_solution(prob, w, dev):
     res = {}
     runs = [("fused_c64", dict(mode="fused", precision="c64")),
             ("fused_c128", dict(mode="fused", precision="c128")),
+            ("fused_c64_diag_precond", dict(mode="fused", precision="c64", precond="diag")),
             ("reference_c128", dict(mode="reference", precision="c128"))]
     for name, kw in runs:
         torch.cuda.synchronize()
@@ -302,6 +303,8 @@ def time_to_solution(prob, w, dev):
     res["fp64_reference_energy"] = e_ref
     base = res["reference_c128"]["seconds"]
     res["speedup_fused_c64_vs_plain_fp64"] = round(base / res["fused_c64"]["seconds"], 2)
+    res["speedup_fused_c64_diag_precond_vs_plain_fp64"] = round(
+        base / res["fused_c64_diag_precond"]["seconds"], 2)
     return res
 
 

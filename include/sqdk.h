@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 12
+#define SQDK_ABI_VERSION 13
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -293,6 +293,15 @@ enum sqdk_locg_criterion {
   SQDK_CRIT_RELATIVE = 1   /* |A x - theta x| <= tol * |theta| */
 };
 
+/* Preconditioner of the FUSED mode (the reference's iteration has none). */
+enum sqdk_locg_precond {
+  SQDK_PRECOND_NONE = 0,   /* search direction = residual R */
+  SQDK_PRECOND_DIAG = 1    /* R / max(d - min(theta, min d), floor), d = diag(H) from the
+                              X = 0 terms, floor = max(0.02 (mean d - min d), 1e-3 |theta|):
+                              Jacobi with the shift kept at or below min d; needs the
+                              built-in operator */
+};
+
 typedef struct sqdk_locg_opts {
   int32_t mode;            /* sqdk_locg_mode */
   int32_t precision;       /* SQDK_C128 or SQDK_C64 */
@@ -303,7 +312,7 @@ typedef struct sqdk_locg_opts {
   int32_t check_every;     /* host convergence poll period in iterations (>=1) */
   int32_t refresh_every;   /* FUSED: recompute A x and A p explicitly every k iterations */
   int32_t use_graph;       /* capture check_every iterations as one HIP graph */
-  int32_t reserved;
+  int32_t precond;         /* sqdk_locg_precond (FUSED only) */
 } sqdk_locg_opts;
 
 typedef struct sqdk_locg_result {

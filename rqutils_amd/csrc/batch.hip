@@ -494,6 +494,7 @@ int sqdk_batch_locg_solve(const sqdk_batch* batch, const sqdk_tune* tune, void* 
                           sqdk_locg_result* results, void* stream) {
   SQDK_REQUIRE(batch && x && opts && work && results, "NULL pointer argument");
   SQDK_REQUIRE(opts->mode == SQDK_LOCG_FUSED, "batched solves use the fused mode");
+  SQDK_REQUIRE(opts->precond == SQDK_PRECOND_NONE, "batched solves have no preconditioner");
   SQDK_REQUIRE(opts->criterion == SQDK_CRIT_REFERENCE || opts->criterion == SQDK_CRIT_RELATIVE,
                "unknown criterion %d", opts->criterion);
   SQDK_REQUIRE(opts->maxiter >= 0, "maxiter %d < 0", opts->maxiter);

@@ -44,16 +44,35 @@ int lincomb_impl(int64_t n, int precision, void* out, int k, const void* const* 
                  const int32_t* skip = nullptr);
 
 // Gram and projected-operator sums of the fused LOBPCG over
-// V = (X, R, P), AV = (AX, AR, AP): 13 complex sums written to out.
+// V = (X, W, P), AV = (AX, AW, AP): 13 complex sums, then |res|^2 (res ==
+// NULL: |W|^2), written to out.
 int gram_impl(int64_t n, int precision, const void* const* v, const void* const* av,
-              double* out, double* partials, hipStream_t s, const int32_t* skip = nullptr);
+              const void* res, double* out, double* partials, hipStream_t s,
+              const int32_t* skip = nullptr);
 
+// Diagonal preconditioner of the fused LOBPCG (theta = the Ritz value; st =
+// (min diag, 0.02 (mean diag - min diag)) in a slot): R scaled by
+// 1 / max(diag - min(theta, min diag), max(st.y, 1e-3 |theta|)).
+//
 // Fused LOBPCG update; coefficients at slots[c_base ...]:
-//   X' = c0 X + c1 R + c2 P, AX' = c0 AX + c1 AR + c2 AP,
-//   P' = d1 R + d2 P, AP' = d1 AR + d2 AP, R' = AX' - theta X'.
-int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, const void* AR,
-                      void* P, void* AP, const double* slots, int c_base, hipStream_t s,
+//   X' = c0 X + c1 W + c2 P, AX' = c0 AX + c1 AW + c2 AP,
+//   P' = d1 W + d2 P, AP' = d1 AW + d2 AP, R' = AX' - theta' X',
+// and with diag given W' = the preconditioned R' (the statistics at
+// slots[stat_slot]; W may alias R when diag is NULL).
+int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, void* W,
+                      const void* AW, void* P, void* AP, const double* diag,
+                      const double* slots, int c_base, int stat_slot, hipStream_t s,
                       const int32_t* skip = nullptr);
+
+// W = the preconditioned R at theta = slots[theta_slot].x.
+int precond_impl(int64_t n, int precision, const void* R, const double* diag,
+                 const double* slots, int theta_slot, int stat_slot, void* W, hipStream_t s,
+                 const int32_t* skip = nullptr);
+
+// out (a complex slot) = (min diag, 0.02 (mean diag - min diag)); partials
+// holds 2 * max_blocks doubles.
+int diag_stats_impl(int64_t n, const double* diag, double* partials, int max_blocks,
+                    double* out, hipStream_t s);
 
 int convert_impl(int64_t n, int from_p, const void* in, int to_p, void* out, hipStream_t s);
 

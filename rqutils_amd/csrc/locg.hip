@@ -31,13 +31,14 @@ enum Slot : int {
   kOne = 0,
   kZero = 1,
   kD = 2,        // dot results D0.. (16 pairs)
-  kGram = 20,    // 13 fused Gram sums
+  kGram = 20,    // 14 fused Gram sums
   kC = 40,       // lincomb coefficients C0..C7
   kUpd = 50,     // fused update coefficients (6)
   kTheta = 60,   // .x theta
   kKappa = 61,   // kappa[0..2]
   kInfo = 64,    // .x |r|, .y |Ax|
   kInfo2 = 65,   // .x reltol / tol*|theta|, .y last criterion value
+  kDiagStat = 66, // preconditioner: .x min diag, .y 0.02 (mean diag - min diag)
   kNumSlots = 80
 };
 enum Flag : int { kConv = 0, kNiter = 1, kPValid = 2, kStatus = 3, kMatvecs = 4, kNumFlags = 16 };
@@ -184,13 +185,13 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       // A non-finite sum (NaN or Inf in the operator or the vectors) ends the
       // solve with status 3 instead of iterating on it.
       bool finite = isfinite(theta);
-      for (int i = 0; i < 13; ++i) finite = finite && isfinite(g[i].x) && isfinite(g[i].y);
+      for (int i = 0; i < 14; ++i) finite = finite && isfinite(g[i].x) && isfinite(g[i].y);
       if (!finite) {
         F[kStatus] = 3;
         F[kConv] = 1;
         break;
       }
-      const double rn = sqrt(fmax(g[1].x, 0.0));
+      const double rn = sqrt(fmax(g[13].x, 0.0));   // |R| (the residual itself)
       const double axn = sqrt(fmax(g[12].x, 0.0));
       if (criterion_met(prm, rn, axn, theta, S)) {
         F[kConv] = 1;
@@ -224,6 +225,17 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
     default:
       break;
   }
+}
+
+// diag[i] = Re sum_k c_k (-1)^{popcount(z_k & key_i)} over the X = 0 terms.
+__global__ void __launch_bounds__(256)
+diag_real_kernel(const uint64_t* __restrict__ zmask, const double2* __restrict__ coef, int t0,
+                 int t1, const uint64_t* __restrict__ keys, int64_t n, double* __restrict__ diag) {
+  const int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x;
+  if (i >= n) return;
+  double cr, ci;
+  group_coef<double2>(zmask, coef, t0, t1, keys[i], cr, ci);
+  diag[i] = cr;
 }
 
 __global__ void init_slots_kernel(double2* S, int32_t* F) {
@@ -263,6 +275,7 @@ struct Solver {
   int32_t* F;
   double* partials;
   char* vec[7];
+  double* diag = nullptr;   // FUSED with the diagonal preconditioner
   void* mv_work = nullptr;
   size_t mv_bytes = 0;
   ScalarParams prm;
@@ -408,14 +421,24 @@ int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_h
   return SQDK_OK;
 }
 
+// FUSED: one matvec (A W) per iteration over the basis (X, W, P), where W is
+// the residual R itself or, with the diagonal preconditioner, R scaled by
+// 1 / max(diag - min(theta, min diag), floor) (Jacobi preconditioned LOBPCG,
+// the shift kept at or below the smallest diagonal entry).
 int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host) {
   char* X = static_cast<char*>(x);
   char* R = sv.vec[0];
   char* P = sv.vec[1];
   char* AX = sv.vec[2];
-  char* AR = sv.vec[3];
+  char* AW = sv.vec[3];
   char* AP = sv.vec[4];
+  char* W = sv.diag ? sv.vec[5] : R;
   const size_t vb = (size_t)sv.n * elem_bytes(sv.prec);
+  auto precondition = [&](bool skip) -> int {
+    if (!sv.diag) return SQDK_OK;
+    return precond_impl(sv.n, sv.prec, R, sv.diag, reinterpret_cast<const double*>(sv.S),
+                        kTheta, kDiagStat, W, sv.s, skip ? sv.F + kConv : nullptr);
+  };
   SQDK_HIP(hipMemsetAsync(P, 0, vb, sv.s));
   SQDK_HIP(hipMemsetAsync(AP, 0, vb, sv.s));
   SQDK_TRY(sv.dots({{X, X}}, false));
@@ -425,20 +448,22 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
   SQDK_TRY(sv.dots({{X, AX}}, false));
   SQDK_TRY(sv.scalar(kStepRho, false));
   SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, false));
+  SQDK_TRY(precondition(false));
 
-  const void* V[3] = {X, R, P};
-  const void* AV[3] = {AX, AR, AP};
+  const void* V[3] = {X, W, P};
+  const void* AV[3] = {AX, AW, AP};
+  const void* res = sv.diag ? R : nullptr;
   int done = 0;
   int since_refresh = 0;
   while (true) {
     const int chunk = std::min(o.check_every, o.maxiter - done);
     for (int it = 0; it < chunk; ++it) {
-      SQDK_TRY(sv.apply(R, AR, true));
-      SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, reinterpret_cast<double*>(sv.S + kGram),
+      SQDK_TRY(sv.apply(W, AW, true));
+      SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, res, reinterpret_cast<double*>(sv.S + kGram),
                          sv.partials, sv.s, sv.F + kConv));
       SQDK_TRY(sv.scalar(kStepFusedRR, true));
-      SQDK_TRY(fused_update_impl(sv.n, sv.prec, X, AX, R, AR, P, AP,
-                                 reinterpret_cast<const double*>(sv.S), kUpd, sv.s,
+      SQDK_TRY(fused_update_impl(sv.n, sv.prec, X, AX, R, W, AW, P, AP, sv.diag,
+                                 reinterpret_cast<const double*>(sv.S), kUpd, kDiagStat, sv.s,
                                  sv.F + kConv));
       if (o.refresh_every > 0 && ++since_refresh >= o.refresh_every) {
         since_refresh = 0;
@@ -447,6 +472,7 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
         SQDK_TRY(sv.dots({{X, X}, {X, AX}}, true));
         SQDK_TRY(sv.scalar(kStepRefreshRho, true));
         SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, true));
+        SQDK_TRY(precondition(true));
       }
       if (sv.op->fn && it + 1 < chunk) break;
     }
@@ -465,6 +491,7 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
       if (flags_host[kConv] || done >= o.maxiter || flags_host[kStatus] >= 2) break;
       // Not converged after all: A P is refreshed too and the loop goes on.
       SQDK_TRY(sv.apply(P, AP, false));
+      SQDK_TRY(precondition(false));
       since_refresh = 0;
     }
   }
@@ -488,7 +515,8 @@ int sqdk_locg_workspace(const sqdk_op* op, int64_t n, const sqdk_locg_opts* opts
   const size_t vb = align256((size_t)n * elem_bytes(opts->precision));
   size_t mv = 0;
   if (op && !op->fn && op->space) mv = align256(matvec_work_bytes(op->space, op->tune, opts->precision));
-  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) + 7 * vb + mv;
+  const size_t dg = opts->precond ? align256((size_t)n * sizeof(double)) : 0;
+  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) + 7 * vb + mv + dg;
   return SQDK_OK;
 }
 
@@ -505,6 +533,11 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
     SQDK_REQUIRE(op->space->n == n, "operator size %lld != vector size %lld",
                  (long long)op->space->n, (long long)n);
   }
+  SQDK_REQUIRE(opts->precond == SQDK_PRECOND_NONE || opts->precond == SQDK_PRECOND_DIAG,
+               "unknown preconditioner %d", opts->precond);
+  SQDK_REQUIRE(opts->precond == SQDK_PRECOND_NONE ||
+                   (opts->mode == SQDK_LOCG_FUSED && !op->fn),
+               "the diagonal preconditioner needs the FUSED mode and the built-in operator");
   size_t need = 0;
   SQDK_TRY(sqdk_locg_workspace(op, n, opts, &need));
   SQDK_REQUIRE(work_bytes >= need, "workspace of %zu bytes, %zu needed", work_bytes, need);
@@ -529,6 +562,14 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   if (!op->fn) {
     sv.mv_bytes = matvec_work_bytes(op->space, op->tune, o.precision);
     sv.mv_work = sv.mv_bytes ? w + 7 * vb : nullptr;
+    if (o.precond == SQDK_PRECOND_DIAG) {
+      sv.diag = reinterpret_cast<double*>(w + 7 * vb + align256(sv.mv_bytes));
+      const sqdk_ham* h = op->ham;
+      diag_real_kernel<<<(unsigned)ceil_div(n, 256), 256, 0, sv.s>>>(
+          h->zmask, reinterpret_cast<const double2*>(h->coef), h->diag_t0, h->diag_t1,
+          op->space->keys, n, sv.diag);
+      SQDK_LAUNCHED("diag_real_kernel");
+    }
   }
   sv.prm.tol = o.tol;
   sv.prm.nlen = (double)(o.n_for_tol > 0 ? o.n_for_tol : n);
@@ -537,6 +578,9 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
 
   init_slots_kernel<<<1, 128, 0, sv.s>>>(sv.S, sv.F);
   SQDK_LAUNCHED("init_slots_kernel");
+  if (sv.diag)
+    SQDK_TRY(diag_stats_impl(n, sv.diag, sv.partials, kDotBlocks,
+                             reinterpret_cast<double*>(sv.S + kDiagStat), sv.s));
   int32_t flags[kNumFlags] = {0};
   int rc = o.mode == SQDK_LOCG_REFERENCE ? run_reference(sv, x, o, flags)
                                          : run_fused(sv, x, o, flags);

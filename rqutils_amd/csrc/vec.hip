@@ -52,7 +52,7 @@ dots_final_kernel(int nblocks, const double* __restrict__ partials, double* __re
   if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
 
-constexpr int kGramSums = 13;
+constexpr int kGramSums = 14;
 
 // Sums over V = (X, R, P), AV = (AX, AR, AP):
 //  0 XX 1 RR 2 PP 3 XR 4 XP 5 RP          (Gram, conj(first) * second)
@@ -63,7 +63,8 @@ __global__ void __launch_bounds__(kBlock)
 gram_partial_kernel(int64_t n, const VT* __restrict__ X, const VT* __restrict__ R,
                     const VT* __restrict__ P, const VT* __restrict__ AX,
                     const VT* __restrict__ AR, const VT* __restrict__ AP,
-                    double* __restrict__ partials, const int32_t* __restrict__ skip) {
+                    const VT* __restrict__ RES, double* __restrict__ partials,
+                    const int32_t* __restrict__ skip) {
   if (skip && skip[0] != 0) return;
   __shared__ double red[2 * kGramSums * (kBlock / kWave)];
   double acc[2 * kGramSums];
@@ -73,9 +74,10 @@ gram_partial_kernel(int64_t n, const VT* __restrict__ X, const VT* __restrict__ 
        i += (int64_t)gridDim.x * kBlock) {
     const double2 x = Vec<VT>::load(X, i), r = Vec<VT>::load(R, i), p = Vec<VT>::load(P, i);
     const double2 ax = Vec<VT>::load(AX, i), ar = Vec<VT>::load(AR, i), ap = Vec<VT>::load(AP, i);
+    const double2 res = RES ? Vec<VT>::load(RES, i) : r;
     const double2 prods[kGramSums] = {cmulc(x, x),  cmulc(r, r),  cmulc(p, p),   cmulc(x, r),  cmulc(x, p),
                                       cmulc(r, p),  cmulc(x, ax), cmulc(r, ar),  cmulc(p, ap), cmulc(x, ar),
-                                      cmulc(x, ap), cmulc(r, ap), cmulc(ax, ax)};
+                                      cmulc(x, ap), cmulc(r, ap), cmulc(ax, ax), cmulc(res, res)};
 #pragma unroll
     for (int k = 0; k < kGramSums; ++k) {
       acc[2 * k] += prods[k].x;
@@ -112,29 +114,113 @@ lincomb_kernel(int64_t n, int k, LinIn li, const double2* __restrict__ slots, VT
   }
 }
 
+// Diagonal (Jacobi) preconditioner for the lowest eigenpair at Ritz value
+// theta: 1 / max(d - sigma, floor) with the shift sigma = min(theta, min d)
+// (so d - sigma >= 0: the scaling stays positive while theta is still above
+// the smallest diagonal entries, and it favours those) and floor =
+// max(st.y, 1e-3 |theta|), st.y = 0.02 (mean d - min d) (a small fraction of
+// the diagonal's spread; st.x = min d).
+__device__ inline double precond_scale(double d, double theta, double2 st) {
+  const double sigma = fmin(theta, st.x);
+  return 1.0 / fmax(d - sigma, fmax(fmax(st.y, 1e-3 * fabs(theta)), 1e-300));
+}
+
+// W (the search direction in the basis) may be R itself (no preconditioner,
+// diag == NULL) or a separate vector that receives diag-preconditioned R'.
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
-fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* __restrict__ R,
-                    const VT* __restrict__ AR, VT* __restrict__ P, VT* __restrict__ AP,
-                    const double2* __restrict__ c, const int32_t* __restrict__ skip) {
+fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, VT* W,
+                    const VT* __restrict__ AW, VT* __restrict__ P, VT* __restrict__ AP,
+                    const double* __restrict__ diag, const double2* __restrict__ c,
+                    const double2* __restrict__ dstat, const int32_t* __restrict__ skip) {
   if (skip && skip[0] != 0) return;
   // c[0..2] = c0, c1, c2; c[3..4] = d1, d2; c[5].x = theta
   const double2 c0 = c[0], c1 = c[1], c2 = c[2], d1 = c[3], d2 = c[4];
   const double theta = c[5].x;
+  const double2 st = diag ? *dstat : make_double2(0.0, 0.0);
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
-    const double2 x = Vec<VT>::load(X, i), r = Vec<VT>::load(R, i), p = Vec<VT>::load(P, i);
-    const double2 ax = Vec<VT>::load(AX, i), ar = Vec<VT>::load(AR, i), ap = Vec<VT>::load(AP, i);
-    const double2 xn = cadd(cadd(cmul(c0, x), cmul(c1, r)), cmul(c2, p));
-    const double2 axn = cadd(cadd(cmul(c0, ax), cmul(c1, ar)), cmul(c2, ap));
-    const double2 pn = cadd(cmul(d1, r), cmul(d2, p));
-    const double2 apn = cadd(cmul(d1, ar), cmul(d2, ap));
+    const double2 x = Vec<VT>::load(X, i), w = Vec<VT>::load(W, i), p = Vec<VT>::load(P, i);
+    const double2 ax = Vec<VT>::load(AX, i), aw = Vec<VT>::load(AW, i), ap = Vec<VT>::load(AP, i);
+    const double2 xn = cadd(cadd(cmul(c0, x), cmul(c1, w)), cmul(c2, p));
+    const double2 axn = cadd(cadd(cmul(c0, ax), cmul(c1, aw)), cmul(c2, ap));
+    const double2 pn = cadd(cmul(d1, w), cmul(d2, p));
+    const double2 apn = cadd(cmul(d1, aw), cmul(d2, ap));
     const double2 rn = csub(axn, cscale(xn, theta));
     Vec<VT>::store(X, i, xn);
     Vec<VT>::store(AX, i, axn);
     Vec<VT>::store(P, i, pn);
     Vec<VT>::store(AP, i, apn);
     Vec<VT>::store(R, i, rn);
+    if (diag) Vec<VT>::store(W, i, cscale(rn, precond_scale(diag[i], theta, st)));
+  }
+}
+
+// W = R * precond_scale(diag, theta), theta = slots[theta_slot].x, the
+// diagonal's statistics at slots[stat_slot].
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+precond_kernel(int64_t n, const VT* __restrict__ R, const double* __restrict__ diag,
+               const double2* __restrict__ slots, int theta_slot, int stat_slot,
+               VT* __restrict__ W, const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  const double theta = slots[theta_slot].x;
+  const double2 st = slots[stat_slot];
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock)
+    Vec<VT>::store(W, i, cscale(Vec<VT>::load(R, i), precond_scale(diag[i], theta, st)));
+}
+
+// Statistics of the diagonal for the preconditioner: per-block min and sum,
+// then one block writes out = (min, 0.02 (mean - min)).
+__global__ void __launch_bounds__(kBlock)
+diag_stats_partial_kernel(int64_t n, const double* __restrict__ diag,
+                          double* __restrict__ partials) {
+  __shared__ double smin[kBlock], ssum[kBlock];
+  double mn = INFINITY, sm = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    mn = fmin(mn, diag[i]);
+    sm += diag[i];
+  }
+  smin[threadIdx.x] = mn;
+  ssum[threadIdx.x] = sm;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + h]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = smin[0];
+    partials[2 * blockIdx.x + 1] = ssum[0];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+diag_stats_final_kernel(int64_t n, int blocks, const double* __restrict__ partials,
+                        double2* __restrict__ out) {
+  __shared__ double smin[kBlock], ssum[kBlock];
+  double mn = INFINITY, sm = 0.0;
+  for (int b = threadIdx.x; b < blocks; b += kBlock) {
+    mn = fmin(mn, partials[2 * b]);
+    sm += partials[2 * b + 1];
+  }
+  smin[threadIdx.x] = mn;
+  ssum[threadIdx.x] = sm;
+  __syncthreads();
+  for (int h = kBlock / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + h]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double dmin = smin[0];
+    *out = make_double2(dmin, 0.02 * (ssum[0] / (double)n - dmin));
   }
 }
 
@@ -174,16 +260,19 @@ int dots_impl(int64_t n, int precision, int npairs, const PairList& pl, double* 
 }
 
 int gram_impl(int64_t n, int precision, const void* const* v, const void* const* av,
-              double* out, double* partials, hipStream_t s, const int32_t* skip) {
+              const void* res, double* out, double* partials, hipStream_t s,
+              const int32_t* skip) {
   const unsigned g = dot_grid(n);
   if (precision == SQDK_C128)
     gram_partial_kernel<double2><<<g, kBlock, 0, s>>>(
         n, (const double2*)v[0], (const double2*)v[1], (const double2*)v[2],
-        (const double2*)av[0], (const double2*)av[1], (const double2*)av[2], partials, skip);
+        (const double2*)av[0], (const double2*)av[1], (const double2*)av[2],
+        (const double2*)res, partials, skip);
   else
     gram_partial_kernel<float2><<<g, kBlock, 0, s>>>(
         n, (const float2*)v[0], (const float2*)v[1], (const float2*)v[2],
-        (const float2*)av[0], (const float2*)av[1], (const float2*)av[2], partials, skip);
+        (const float2*)av[0], (const float2*)av[1], (const float2*)av[2],
+        (const float2*)res, partials, skip);
   SQDK_LAUNCHED("gram_partial_kernel");
   dots_final_kernel<<<2 * kGramSums, kBlock, 0, s>>>((int)g, partials, out, skip);
   SQDK_LAUNCHED("dots_final_kernel");
@@ -208,20 +297,49 @@ int lincomb_impl(int64_t n, int precision, void* out, int k, const void* const* 
   return SQDK_OK;
 }
 
-int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, const void* AR,
-                      void* P, void* AP, const double* slots, int c_base, hipStream_t s,
+int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, void* W,
+                      const void* AW, void* P, void* AP, const double* diag,
+                      const double* slots, int c_base, int stat_slot, hipStream_t s,
                       const int32_t* skip) {
   const unsigned g = stream_grid(n);
   const double2* c = reinterpret_cast<const double2*>(slots) + c_base;
+  const double2* st = reinterpret_cast<const double2*>(slots) + stat_slot;
   if (precision == SQDK_C128)
     fused_update_kernel<double2><<<g, kBlock, 0, s>>>(n, (double2*)X, (double2*)AX, (double2*)R,
-                                                      (const double2*)AR, (double2*)P,
-                                                      (double2*)AP, c, skip);
+                                                      (double2*)W, (const double2*)AW,
+                                                      (double2*)P, (double2*)AP, diag, c, st,
+                                                      skip);
   else
     fused_update_kernel<float2><<<g, kBlock, 0, s>>>(n, (float2*)X, (float2*)AX, (float2*)R,
-                                                     (const float2*)AR, (float2*)P,
-                                                     (float2*)AP, c, skip);
+                                                     (float2*)W, (const float2*)AW, (float2*)P,
+                                                     (float2*)AP, diag, c, st, skip);
   SQDK_LAUNCHED("fused_update_kernel");
+  return SQDK_OK;
+}
+
+int precond_impl(int64_t n, int precision, const void* R, const double* diag,
+                 const double* slots, int theta_slot, int stat_slot, void* W, hipStream_t s,
+                 const int32_t* skip) {
+  const unsigned g = stream_grid(n);
+  const double2* sl = reinterpret_cast<const double2*>(slots);
+  if (precision == SQDK_C128)
+    precond_kernel<double2><<<g, kBlock, 0, s>>>(n, (const double2*)R, diag, sl, theta_slot,
+                                                 stat_slot, (double2*)W, skip);
+  else
+    precond_kernel<float2><<<g, kBlock, 0, s>>>(n, (const float2*)R, diag, sl, theta_slot,
+                                                stat_slot, (float2*)W, skip);
+  SQDK_LAUNCHED("precond_kernel");
+  return SQDK_OK;
+}
+
+int diag_stats_impl(int64_t n, const double* diag, double* partials, int max_blocks,
+                    double* out, hipStream_t s) {
+  const int g = (int)std::max<int64_t>(1, std::min<int64_t>(ceil_div(n, kBlock), max_blocks));
+  diag_stats_partial_kernel<<<g, kBlock, 0, s>>>(n, diag, partials);
+  SQDK_LAUNCHED("diag_stats_partial_kernel");
+  diag_stats_final_kernel<<<1, kBlock, 0, s>>>(n, g, partials,
+                                               reinterpret_cast<double2*>(out));
+  SQDK_LAUNCHED("diag_stats_final_kernel");
   return SQDK_OK;
 }
 

@@ -43,6 +43,7 @@ STATUS_NONFINITE = 3      # LocgInfo.status: NaN or Inf in the Gram sums
 _MODES = {"reference": _lib.LOCG_REFERENCE, "fused": _lib.LOCG_FUSED}
 _CRITERIA = {"reference": _lib.CRIT_REFERENCE, "relative": _lib.CRIT_RELATIVE}
 _PRECISIONS = {"c128": _lib.C128, "c64": _lib.C64}
+_PRECONDS = {None: 0, "none": 0, "diag": 1}     # sqdk_locg_precond
 
 
 @dataclass
@@ -108,11 +109,15 @@ class _Callback:
 def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = None,
           criterion: str = "relative", tol: float = 1e-6, maxiter: int = 1000,
           n_for_tol: int | None = None, check_every: int = 4, refresh_every: int = 16,
-          real: bool = False) -> tuple[torch.Tensor, LocgInfo]:
+          real: bool = False, precond: str | None = None) -> tuple[torch.Tensor, LocgInfo]:
     """Lowest eigenpair of ``op`` from start vector ``x0`` (device tensor).
 
     ``op`` is a PauliOperator or a callable on device tensors. Returns the
     normalised eigenvector (complex, solver precision) and :class:`LocgInfo`.
+    ``precond="diag"`` (fused mode, PauliOperator only) searches along the
+    residual scaled by ``1 / max(d - min(theta, min d), floor)``, ``d`` the
+    diagonal from the operator's X = 0 terms and ``floor = max(0.02 (mean d -
+    min d), 1e-3 |theta|)``.
     """
     L = _lib.lib()
     if mode not in _MODES:
@@ -134,6 +139,11 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     opts.check_every = int(check_every)
     opts.refresh_every = int(refresh_every)
     opts.use_graph = 0
+    if precond not in _PRECONDS:
+        raise ValueError(f"precond must be one of {sorted(_PRECONDS, key=str)}")
+    opts.precond = _PRECONDS[precond]
+    if opts.precond and (mode != "fused" or not isinstance(op, PauliOperator)):
+        raise ValueError("precond='diag' needs mode='fused' and a PauliOperator")
     cop = _lib.SqdkOp()
     keep = []
     cb = None
@@ -177,7 +187,7 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
                 vspace=None, log_level: int = logging.WARNING, *, mode: str = "reference",
                 precision: str | None = None, criterion: str = "reference",
                 check_every: int | None = None, refresh_every: int = 16,
-                device=None):
+                device=None, precond: str | None = None):
     r"""Single-vector LOBPCG (rqutils/ground_locg.py:128-157).
 
     Args:
@@ -191,6 +201,9 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
             vector dtype (ground_locg.py:208-209).
         vspace: ``(dimension, dtype)`` for an integer ``xinit``.
         log_level: Verbosity (DEBUG logs the result).
+        precond: ``"diag"`` for the Jacobi-preconditioned fused iteration
+            (``mode="fused"`` with a ``PauliOperator``; the reference has no
+            preconditioner).
 
     Returns:
         ``(eigval, eigvec, niter)``: the Rayleigh-Ritz value as a float, the
@@ -255,7 +268,7 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
         check_every = 4
     vec, info = solve(op, x0, mode=mode, precision=precision, criterion=criterion, tol=tol,
                       maxiter=maxiter, check_every=check_every, refresh_every=refresh_every,
-                      real=real)
+                      real=real, precond=precond)
     if log_level <= logging.DEBUG:
         LOG.debug("LOBPCG: eigval %s after %d iterations (converged=%s, |r|=%g)",
                   info.eigval, info.niter, info.converged, info.resid_norm)
