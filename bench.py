@@ -101,6 +101,7 @@ def main():
                     help="dense lookup at this bucket width (0: autotune or search)")
     ap.add_argument("--no-net", action="store_true", help="skip the network-start block")
     ap.add_argument("--no-intake", action="store_true", help="skip the sample-intake block")
+    ap.add_argument("--no-cache", action="store_true", help="skip the cached hit table")
     ap.add_argument("--no-svsim", action="store_true", help="skip the state-vector block")
     ap.add_argument("--net-train", type=int, default=100,
                     help="training Hamiltonians of the network-start block")
@@ -242,8 +243,10 @@ def main():
         "wall_s_timed": round(wall, 3),
     }
 
+    if not args.no_cache:
+        record["hit_table"] = hit_table_block(prob, v, out, o128, hits, args.steps, dev)
     if not args.no_solve:
-        record["time_to_solution"] = time_to_solution(prob, w, dev)
+        record["time_to_solution"] = time_to_solution(prob, w, dev, cached=not args.no_cache)
     if not args.no_intake:
         record["intake"] = intake_block(w, dev)
     if not args.no_svsim and w.name == "heisenberg28":
@@ -260,7 +263,57 @@ def main():
     print(json.dumps(record), flush=True)
 
 
-def time_to_solution(prob, w, dev):
+def hit_table_block(prob, v, out_join, o128_join, hits, steps, dev):
+    """The cached hit table: build time and size in each precision, then the
+    product timed with HIP events like the bucket join above, and its
+    difference from the join's product. Algorithmic bytes per product: the
+    hits' entries (4 + s_c each; the padding is not read), the diagonal, v,
+    out and the row length (3 s_v + 4 per row), one s_v gather per hit and 16
+    per slice."""
+    import torch
+    from rqutils_amd import _lib
+    op = prob.op
+    op.use_cache = True
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for prec, vec, ref in ((_lib.C64, v, out_join), (_lib.C128, v.to(torch.complex128), o128_join)):
+        name = "c64" if prec == _lib.C64 else "c128"
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        op.build_cache(prec)
+        torch.cuda.synchronize()
+        build_s = time.perf_counter() - t0
+        info = op.cache_info[prec]
+        o = torch.empty_like(vec)
+        prob.matvec(vec, o)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(steps):
+            prob.matvec(vec, o)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / steps
+        s_c = 16 if prec == _lib.C128 else 8
+        algo = hits * (4 + s_c) + prob.n * (3 * s_c + 4) + hits * s_c + info["slices"] * 16
+        diff = (o.to(torch.complex128) - ref.to(torch.complex128)).norm().item() / \
+            ref.to(torch.complex128).norm().item()
+        gbs = algo / (ms * 1e-3) / 1e9
+        res[name] = {"build_s": round(build_s, 4), "entries": info["entries"],
+                     "padding": round(info["entries"] / max(hits, 1), 3),
+                     "table_bytes": info["bytes"], "ms": round(ms, 4),
+                     "matvecs_per_s": round(1e3 / ms, 1), "algorithmic_bytes": algo,
+                     "achieved_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                     "rel_diff_vs_bucket_join": diff}
+        log(f"hit table {name}: built in {build_s:.3f} s, {info['entries']} entries "
+            f"({info['bytes'] / 1e9:.2f} GB, {info['entries'] / max(hits, 1):.2f}x the hits); "
+            f"product {ms:.3f} ms ({gbs:.0f} GB/s algorithmic), rel diff vs join {diff:.1e}")
+    op.use_cache = False
+    return res
+
+
+def time_to_solution(prob, w, dev, cached=False):
     import torch
     n = w.n
     g = torch.Generator(device="cpu").manual_seed(77)
@@ -271,7 +324,13 @@ def time_to_solution(prob, w, dev):
             ("fused_c128", dict(mode="fused", precision="c128")),
             ("fused_c64_diag_precond", dict(mode="fused", precision="c64", precond="diag")),
             ("reference_c128", dict(mode="reference", precision="c128"))]
+    if cached:
+        runs += [("cached_fused_c64", dict(mode="fused", precision="c64")),
+                 ("cached_fused_c64_diag_precond",
+                  dict(mode="fused", precision="c64", precond="diag")),
+                 ("cached_reference_c128", dict(mode="reference", precision="c128"))]
     for name, kw in runs:
+        prob.op.use_cache = name.startswith("cached_")
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         vec, info = prob.solve(x0, criterion="relative", tol=1e-6, maxiter=3000, **kw)
@@ -282,6 +341,7 @@ def time_to_solution(prob, w, dev):
                      "resid_rel": info.resid_norm / abs(info.eigval)}
         log(f"{name}: {dt:.2f} s, {info.niter} it, {info.matvecs} matvecs, E={info.eigval:.12f}, "
             f"r={info.resid_norm / abs(info.eigval):.2e}")
+    prob.op.use_cache = False
     # the reference's default start: one-hot at the smallest diagonal entry
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -305,6 +365,14 @@ def time_to_solution(prob, w, dev):
     res["speedup_fused_c64_vs_plain_fp64"] = round(base / res["fused_c64"]["seconds"], 2)
     res["speedup_fused_c64_diag_precond_vs_plain_fp64"] = round(
         base / res["fused_c64_diag_precond"]["seconds"], 2)
+    if cached:
+        # (the tables were built in the hit_table block, which reports their build time)
+        cb = res["cached_reference_c128"]["seconds"]
+        res["speedup_cached_fused_c64_diag_precond_vs_cached_plain_fp64"] = round(
+            cb / res["cached_fused_c64_diag_precond"]["seconds"], 2)
+        res["speedup_cached_fused_c64_diag_precond_vs_plain_fp64"] = round(
+            base / res["cached_fused_c64_diag_precond"]["seconds"], 2)
+    prob.op.use_cache = False
     return res
 
 
@@ -371,7 +439,7 @@ def intake_block(w, dev, repeats: int = 4, reps: int = 3):
 
 
 NET_STARTS = ("gaussian", "min_diag", "network_bits", "network_bits_mix", "network_bits_diag",
-              "network_bits_diag_mix")
+              "network_bits_diag_mix", "gaussian_precond", "network_bits_diag_precond")
 NET_MIX = 0.3   # norm of the Gaussian part of the *_mix network starts
 
 
@@ -380,8 +448,10 @@ def network_start(dev, family, n_train, n_test, steps):
     Hamiltonians of a family from six starts: a seeded Gaussian, the one-hot
     at the smallest diagonal entry, the start network on bits, and the start
     network on bits plus two diagonal-energy features, each network alone and
-    with a Gaussian part of norm NET_MIX. Both networks are trained on the
-    FP64 ground vectors of ``n_train`` other members of the family."""
+    with a Gaussian part of norm NET_MIX; then the Gaussian and the feature
+    network's starts again with the diagonal preconditioner (``*_precond``).
+    Both networks are trained on the FP64 ground vectors of ``n_train`` other
+    members of the family."""
     import functools
     import multiprocessing as mp
     import torch
@@ -434,21 +504,22 @@ def network_start(dev, family, n_train, n_test, steps):
         _, psi, info = ground_targets(p)
         e_ref = info.eigval
         for name in NET_STARTS:
+            base = name.removesuffix("_precond")
             torch.cuda.synchronize()
             t0 = time.perf_counter()
-            if name == "gaussian":
+            if base == "gaussian":
                 x0 = gaussian_start(p.n, dev, w.seed)
-            elif name == "min_diag":
+            elif base == "min_diag":
                 x0 = p.vinit_min_diag()
             else:
-                net_name = name.removesuffix("_mix")
-                mix = NET_MIX if name.endswith("_mix") else 0.0
+                net_name = base.removesuffix("_mix")
+                mix = NET_MIX if base.endswith("_mix") else 0.0
                 extra = energy_features(p) if net_name == "network_bits_diag" else None
                 x0 = nets[net_name].start_vector(p.space.keys, extra, mix=mix, seed=w.seed)
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             _, inf = p.solve(x0, mode="fused", precision="c64", criterion="relative", tol=1e-6,
-                             maxiter=3000)
+                             maxiter=3000, precond="diag" if base != name else None)
             torch.cuda.synchronize()
             t2 = time.perf_counter()
             r = per[name]
@@ -476,6 +547,8 @@ def network_start(dev, family, n_train, n_test, steps):
     g = out["gaussian"]["seconds_mean"]
     for k in NET_STARTS[1:]:
         out[k]["speedup_vs_gaussian"] = round(g / out[k]["seconds_mean"], 2)
+    out["network_bits_diag_precond"]["speedup_vs_gaussian_precond"] = round(
+        out["gaussian_precond"]["seconds_mean"] / out["network_bits_diag_precond"]["seconds_mean"], 2)
     return out
 
 

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 13
+#define SQDK_ABI_VERSION 14
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -277,6 +277,47 @@ int sqdk_dots_workspace(int64_t n, int32_t npairs, size_t* bytes);
 int sqdk_convert(int64_t n, int32_t from_precision, const void* in,
                  int32_t to_precision, void* out, void* stream);
 
+/* ---- cached hit table (the sparse form of the reference's source cache) ----
+ *
+ * The reference can cache each group's source indices, 4 bytes per (row,
+ * group) (cache_level[0] == 1, rqutils/sqd.py:353-371, 448-503). Here only the
+ * hits are kept: for every row its (source row, group coefficient C_g(s_i))
+ * pairs, found once by the bucket join, plus the X = 0 group as a diagonal.
+ * Rows are cut into 64-row slices (the join's row slots); a slice stores its
+ * entries column-major (entry k of its l-th row at base + 64 k + l) up to its
+ * largest hit count, shorter rows padded with column 0xFFFFFFFF and
+ * coefficient 0; row_len says where each row's hits end. Building:
+ * sqdk_cache_slices (host count of slices), sqdk_cache_count (device: slice
+ * rows and the entry offsets, the last one the entry count), allocation by the
+ * caller, sqdk_cache_fill. */
+typedef struct sqdk_cache {
+  int64_t n;                    /* rows (the subspace size) */
+  int64_t n_slices;
+  const uint32_t* slice_rows;   /* [2 n_slices]: first row, rows in the slice (0-64) */
+  const uint64_t* slice_base;   /* [n_slices + 1]: first entry; the last = entries */
+  const uint32_t* col;          /* [entries] source row, 0xFFFFFFFF = padding */
+  const void* coef;             /* [entries] complex coefficient (cache precision) */
+  const void* diag;             /* [n] complex X = 0 coefficient of each row */
+  const uint32_t* row_len;      /* [n] hits of each row (its entries before the padding) */
+  int32_t precision;            /* SQDK_C128 or SQDK_C64 */
+  int32_t reserved;
+} sqdk_cache;
+
+/* Slices of the subspace: n_items x chunk_rows / 64 (chunks of 1024, 2048 or
+ * 4096 rows). */
+int sqdk_cache_slices(const sqdk_space* space, int64_t* n_slices);
+/* slice_rows [2 n_slices] and slice_base [n_slices + 1] (device). */
+int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space, uint32_t* slice_rows,
+                     uint64_t* slice_base, void* stream);
+/* col / coef [slice_base[n_slices]], diag [n] (device; coef and diag complex of
+ * precision) and row_len [n]. */
+int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
+                    const uint64_t* slice_base, uint32_t* col, void* coef, void* diag,
+                    uint32_t* row_len, void* stream);
+/* out = H v through the table (v, out: n complex of the cache's precision);
+ * sums in FP64, each row's in group order; a row reads its own hits only. */
+int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* out, void* stream);
+
 /* ---- single-vector LOBPCG ---------------------------------------------- */
 
 /* External matvec: write A*in into out (both n complex elements of the
@@ -328,13 +369,16 @@ typedef struct sqdk_locg_result {
 } sqdk_locg_result;
 
 /* The operator: the built-in Pauli-sum matvec (ham + space + tune) when fn is
- * NULL, else the external callback. */
+ * NULL, else the external callback. With cache set (and fn NULL) the products
+ * use the cached hit table instead of the bucket join (its precision must be
+ * the solve's). */
 typedef struct sqdk_op {
   const sqdk_ham* ham;
   const sqdk_space* space;
   const sqdk_tune* tune;
   sqdk_matvec_fn fn;
   void* ctx;
+  const struct sqdk_cache* cache;
 } sqdk_op;
 
 /* Device workspace (bytes) for a solve of n elements with this operator. */

@@ -74,6 +74,10 @@ int precond_impl(int64_t n, int precision, const void* R, const double* diag,
 int diag_stats_impl(int64_t n, const double* diag, double* partials, int max_blocks,
                     double* out, hipStream_t s);
 
+// out = the cached hit table's product (precision must be the cache's).
+int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void* out,
+                    hipStream_t s, const int32_t* skip = nullptr, int32_t* counter = nullptr);
+
 int convert_impl(int64_t n, int from_p, const void* in, int to_p, void* out, hipStream_t s);
 
 inline size_t elem_bytes(int precision) { return precision == SQDK_C128 ? 16 : 8; }

@@ -288,6 +288,9 @@ struct Solver {
       if (rc != 0) return fail(SQDK_ECALLBACK, "matvec callback returned %d", rc);
       return SQDK_OK;
     }
+    if (op->cache)
+      return hit_matvec_impl(op->cache, prec, in, out, s, skip ? F + kConv : nullptr,
+                             F + kMatvecs);
     return matvec_impl(op->ham, op->space, prec, in, out, op->tune, mv_work, mv_bytes, s,
                        skip ? F + kConv : nullptr, F + kMatvecs);
   }
@@ -533,6 +536,9 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
     SQDK_REQUIRE(op->space->n == n, "operator size %lld != vector size %lld",
                  (long long)op->space->n, (long long)n);
   }
+  SQDK_REQUIRE(!op->cache || op->fn || (op->cache->precision == opts->precision &&
+                                        op->cache->n == n),
+               "cached hit table does not match the solve (precision or size)");
   SQDK_REQUIRE(opts->precond == SQDK_PRECOND_NONE || opts->precond == SQDK_PRECOND_DIAG,
                "unknown preconditioner %d", opts->precond);
   SQDK_REQUIRE(opts->precond == SQDK_PRECOND_NONE ||

@@ -155,6 +155,10 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
         cop.ham = C.pointer(h)
         cop.space = C.pointer(s)
         cop.tune = C.pointer(t)
+        cs = op.cache_struct(prec)
+        if cs is not None:
+            keep.append(cs)
+            cop.cache = C.pointer(cs)
     elif not callable(op):
         raise ValueError("op must be a PauliOperator or a callable")
     nbytes = C.c_size_t(0)

@@ -302,6 +302,11 @@ class PauliOperator:
         self.tune = _lib.SqdkTune(space.rows_per_thread, 1, 1, 0)
         self._cstruct = None
         self._work = {}
+        self.use_cache = False     # products through the cached hit table (build_cache)
+        self.cache_budget = None   # largest table (bytes) built on first use; None: any
+        self._caches = {}          # precision -> (SqdkCache, tensors...)
+        self._cache_too_big = set()
+        self.cache_info = {}
 
     def set_tune(self, group_split: int = 1, stage_v: int = 0, xcd_remap: int = 1,
                  compact: int = 0):
@@ -405,6 +410,62 @@ class PauliOperator:
             self._cstruct = (self.c_struct(), self.space.c_struct(), self.tune)
         return self._cstruct
 
+    def build_cache(self, precision: int, max_bytes: int | None = None) -> bool:
+        """Build the hit table of this operator in ``precision`` (``_lib.C128``
+        or ``_lib.C64``): every row's (source row, group coefficient) pairs,
+        found once by the bucket join, in 64-row column-major slices, plus the
+        diagonal and each row's hit count. Later products read the table instead of searching
+        (``use_cache``). Returns False, building nothing, when the table would
+        take more than ``max_bytes``. The sparse counterpart of the reference's
+        source-index cache (rqutils/sqd.py:353-371)."""
+        if precision in self._caches:
+            return True
+        L = _lib.lib()
+        dev = self.space.device
+        h, s, _ = self.structs()
+        ns = C.c_int64(0)
+        _lib.check(L.sqdk_cache_slices(C.byref(s), C.byref(ns)), "sqdk_cache_slices")
+        ns = ns.value
+        srows = torch.empty(2 * ns, dtype=torch.int32, device=dev)
+        sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+        stream = _lib.stream_ptr(dev)
+        _lib.check(L.sqdk_cache_count(C.byref(h), C.byref(s), srows.data_ptr(), sbase.data_ptr(),
+                                      stream), "sqdk_cache_count")
+        entries = int(sbase[-1].item())
+        cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
+        esz = 16 if precision == _lib.C128 else 8
+        nbytes = entries * (4 + esz) + self.space.n * (esz + 4) + ns * 16
+        if max_bytes is not None and nbytes > max_bytes:
+            return False
+        col = torch.empty(max(entries, 1), dtype=torch.int32, device=dev)
+        coef = torch.empty(max(entries, 1), dtype=cdt, device=dev)
+        diag = torch.empty(self.space.n, dtype=cdt, device=dev)
+        rlen = torch.empty(self.space.n, dtype=torch.int32, device=dev)
+        _lib.check(L.sqdk_cache_fill(C.byref(h), C.byref(s), precision, sbase.data_ptr(),
+                                     col.data_ptr(), coef.data_ptr(), diag.data_ptr(),
+                                     rlen.data_ptr(), stream), "sqdk_cache_fill")
+        cs = _lib.SqdkCache(self.space.n, ns, srows.data_ptr(), sbase.data_ptr(), col.data_ptr(),
+                            coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, 0)
+        self._caches[precision] = (cs, srows, sbase, col, coef, diag, rlen)
+        self.cache_info[precision] = {"slices": ns, "entries": entries, "bytes": nbytes}
+        return True
+
+    def cache_struct(self, precision: int):
+        """The SqdkCache of ``precision`` when products go through the table
+        (building it on first use, within ``cache_budget``), else None (the
+        bucket join)."""
+        if not self.use_cache or precision in self._cache_too_big:
+            return None
+        if precision not in self._caches and not self.build_cache(precision, self.cache_budget):
+            self._cache_too_big.add(precision)
+            return None
+        return self._caches[precision][0]
+
+    def drop_cache(self):
+        self._caches = {}
+        self._cache_too_big = set()
+        self.cache_info = {}
+
     def matvec(self, v: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """``H v`` for a complex128 or complex64 device vector of length ``n``."""
         if v.dtype == torch.complex128:
@@ -418,6 +479,12 @@ class PauliOperator:
         v = v.contiguous()
         if out is None:
             out = torch.empty_like(v)
+        cs = self.cache_struct(prec)
+        if cs is not None:
+            _lib.check(_lib.lib().sqdk_cache_matvec(C.byref(cs), v.data_ptr(), out.data_ptr(),
+                                                    _lib.stream_ptr(self.space.device)),
+                       "sqdk_cache_matvec")
+            return out
         h, s, t = self.structs()
         work = self._workspace(prec)
         _lib.check(_lib.lib().sqdk_matvec(C.byref(h), C.byref(s), prec, v.data_ptr(),

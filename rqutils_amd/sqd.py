@@ -44,10 +44,13 @@ class SQDProblem:
     ``PauliSumXZ`` or ``(labels, coeffs)``. Exposes ``matvec`` and ``solve``.
     Without explicit directory arguments the subspace takes the dense lookup
     where it fits (``operator.dense_fits``); ``tune="auto"`` times both kinds.
+    ``cache``: False (every product runs the bucket join), True (products read
+    the hit table, built on first use in each precision) or "auto" (the table
+    when it takes at most a quarter of the free device memory).
     """
 
     def __init__(self, hamiltonian, keys, num_qubits: Optional[int] = None, *, device=None,
-                 tune=None, group_split: int = 1, compact: int = 0, **space_kw):
+                 tune=None, group_split: int = 1, compact: int = 0, cache=False, **space_kw):
         if not isinstance(hamiltonian, PauliSumXZ):
             hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True,
                                                    add_padding=True)
@@ -66,6 +69,12 @@ class SQDProblem:
         self.space = Subspace(keys, nq, device=device, **space_kw)
         self.op = PauliOperator(hamiltonian, self.space)
         self.op.set_tune(group_split=group_split, compact=compact)
+        if cache not in (False, True, "auto"):
+            raise ValueError("cache must be False, True or 'auto'")
+        self.op.use_cache = bool(cache)
+        if cache == "auto":
+            free, _ = torch.cuda.mem_get_info(self.space.device)
+            self.op.cache_budget = free // 4
 
     @property
     def n(self) -> int:
@@ -108,8 +117,12 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
             the length term of the convergence threshold. Must be at least
             ``len(states)``.
         return_eigvec: Return the eigenvector and the basis states as well.
-        cache_level: Accepted for compatibility; the device matvec caches no
-            source indices or diagonals, so every level gives the same result.
+        cache_level: ``cache_level[0] == 1`` (the default) keeps the hits of
+            every group in a device table when it fits in a quarter of the
+            free memory (the sparse form of the reference's source-index
+            cache); 0 searches them at every product. The diagonal level has
+            no effect (the diagonal is recomputed or cached with the hits).
+            Every level gives the same operator.
         mode / precision / maxiter / tol: solver controls (defaults reproduce
             the reference: the reference iteration in FP64 with ``tol`` = eps).
 
@@ -129,7 +142,8 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
     # pack + sort + unique on the device (the distinct rows of uniquify_states)
     keys = uniquify_states(states, device)
     start = time.time()
-    prob = SQDProblem(hamiltonian, keys, num_qubits, device=device)
+    prob = SQDProblem(hamiltonian, keys, num_qubits, device=device,
+                      cache="auto" if cache_level[0] == 1 else False)
     real = not prob.op.is_complex
     dtype = torch.complex128 if precision == "c128" else torch.complex64
     x0 = prob.vinit_min_diag(dtype)

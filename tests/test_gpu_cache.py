@@ -1,0 +1,155 @@
+"""Cached hit table (sqdk_cache_*): the hits of every group found once and
+applied by a sliced sparse product. Its entries must be exactly the oracle's
+off-diagonal matrix elements (sort-join construction), its diagonal the X = 0
+group's, and its products must equal the bucket join's to the precision's
+rounding, for the search and the dense lookup, every chunk size, both
+precisions, Hamiltonians without off-diagonal hits or without a diagonal, and
+inside the LOBPCG."""
+import numpy as np
+import pytest
+import torch
+
+from rqutils_amd import _lib
+from tests.helpers import OracleProblem, small_problem
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _problem(p, **kw):
+    from rqutils_amd.sqd import SQDProblem
+    return SQDProblem((p.labels, p.coeffs), p.keys, p.n, **kw)
+
+
+def _table_to_csr(op, prec, n):
+    """Host CSR of the hit table (padding dropped), its diagonal and its hit
+    count; checks the layout (padding only after a row's hits, lanes without a
+    row all padding, row lengths)."""
+    import scipy.sparse as sps
+    cs, srows, sbase, col, coef, diag, rlen = op._caches[prec]
+    srows = srows.cpu().numpy().view(np.uint32).reshape(-1, 2)
+    sbase = sbase.cpu().numpy()
+    col = col.cpu().numpy().view(np.uint32)
+    coef = coef.cpu().numpy()
+    rlen = rlen.cpu().numpy()
+    rows, cols, vals = [], [], []
+    for sid in range(srows.shape[0]):
+        first, nrow = int(srows[sid, 0]), int(srows[sid, 1])
+        b0, b1 = int(sbase[sid]), int(sbase[sid + 1])
+        assert (b1 - b0) % 64 == 0 and nrow <= 64
+        width = (b1 - b0) // 64
+        if width == 0:
+            continue
+        blk_c = col[b0:b1].reshape(width, 64)
+        blk_v = coef[b0:b1].reshape(width, 64)
+        for lane in range(64):
+            real = blk_c[:, lane] != 0xFFFFFFFF
+            if lane >= nrow:
+                assert not real.any()
+                continue
+            k = int(real.sum())
+            assert rlen[first + lane] == k
+            assert real[:k].all() and not real[k:].any()
+            assert np.all(blk_v[k:, lane] == 0)
+            rows += [first + lane] * k
+            cols += blk_c[:k, lane].tolist()
+            vals += blk_v[:k, lane].tolist()
+    m = sps.csr_matrix((np.asarray(vals, np.complex128), (rows, cols)), shape=(n, n))
+    return m, diag.cpu().numpy().astype(np.complex128), len(rows)
+
+
+@pytest.mark.parametrize("seed,dense_lookup,rows", [(0, False, 4), (1, True, 4), (2, False, 8),
+                                                    (3, False, 16), (4, True, 8)])
+def test_table_entries_match_oracle(gpu, seed, dense_lookup, rows):
+    p = small_problem(seed, n=12, terms=90, nstates=1500)
+    kw = dict(rows_per_thread=rows)
+    if dense_lookup:
+        kw.update(dense=True, bucket_bits=8)
+    prob = _problem(p, cache=True, **kw)
+    assert prob.op.build_cache(_lib.C128)
+    import scipy.sparse as sps
+    m, diag, real_entries = _table_to_csr(prob.op, _lib.C128, p.keys.size)
+    ref = p.csr().astype(np.complex128).tocsr()
+    off = ref - sps.diags(ref.diagonal())
+    assert abs(m - off).max() < 1e-14 * max(1.0, abs(off).max())
+    assert np.allclose(diag, ref.diagonal(), rtol=0, atol=1e-13)
+    # exactly one entry per (row, off-diagonal group) hit of the oracle's sort join
+    offg = [g for g in range(p.x.shape[0]) if np.any(p.x[g] != 0)]
+    hits = int(sum((p.xsources[g] >= 0).sum() for g in offg))
+    assert real_entries == hits
+    assert prob.op.cache_info[_lib.C128]["entries"] % 64 == 0
+
+
+@pytest.mark.parametrize("dense_lookup", [False, True])
+@pytest.mark.parametrize("prec,tol", [("c128", 1e-13), ("c64", 3e-7)])
+def test_cached_matvec_matches_join(gpu, dense_lookup, prec, tol):
+    p = small_problem(11, n=12, terms=150, nstates=2500)
+    kw = dict(dense=True, bucket_bits=8) if dense_lookup else {}
+    join = _problem(p, **kw)
+    cached = _problem(p, cache=True, **kw)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    dt = torch.complex128 if prec == "c128" else torch.complex64
+    vt = torch.from_numpy(v).to("cuda", dt)
+    a = join.matvec(vt).cpu().numpy().astype(np.complex128)
+    b = cached.matvec(vt).cpu().numpy().astype(np.complex128)
+    ref = p.matvec(vt.cpu().numpy().astype(np.complex128))
+    assert _rel(b, ref) < tol
+    assert _rel(b, a) < 2 * tol
+
+
+def test_cache_diagonal_only_and_offdiagonal_only(gpu):
+    rng = np.random.default_rng(5)
+    keys = np.sort(rng.choice(2 ** 10, size=600, replace=False)).astype(np.uint64)
+    v = rng.normal(size=600) + 1j * rng.normal(size=600)
+    for labels in (["ZZIIIIIIII", "IZIIZIIIII", "IIIIIIIIIZ"], ["XXIIIIIIII", "IYYIIIIIII"]):
+        p = OracleProblem(labels, rng.normal(size=len(labels)), keys, 10)
+        prob = _problem(p, cache=True)
+        got = prob.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+        assert _rel(got, p.matvec(v)) < 1e-13
+
+
+def test_cached_solve_same_energy(gpu):
+    from rqutils_amd import workloads as W
+    w = W.heisenberg28(seed=2, n_states=1 << 14, n_random=400, num_qubits=20)
+    from rqutils_amd.sqd import SQDProblem
+    g = torch.Generator(device="cpu").manual_seed(1)
+    x0 = torch.complex(torch.randn(w.n, generator=g, dtype=torch.float64),
+                       torch.randn(w.n, generator=g, dtype=torch.float64)).cuda()
+    res = {}
+    for cache in (False, True):
+        prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, cache=cache)
+        for mode, prec, tol in (("fused", "c64", 1e-6), ("reference", "c128", 1e-10)):
+            _, info = prob.solve(x0, mode=mode, precision=prec, criterion="relative", tol=tol,
+                                 maxiter=3000)
+            assert info.converged
+            res[(cache, mode)] = info
+    for mode in ("fused", "reference"):
+        a, b = res[(False, mode)], res[(True, mode)]
+        assert abs(a.eigval - b.eigval) / abs(a.eigval) < (1e-7 if mode == "fused" else 1e-12)
+        assert abs(a.niter - b.niter) <= 3
+
+
+def test_sqd_cache_levels_agree(gpu):
+    from rqutils_amd.sqd import sqd
+    rng = np.random.default_rng(8)
+    n = 10
+    labels = ["".join("IXYZ"[i] for i in row) for row in rng.integers(0, 4, (60, n))]
+    coeffs = rng.uniform(-1, 1, 60)
+    states = rng.integers(0, 2, (900, n)).astype(np.uint8)
+    e0 = sqd((labels, coeffs), states, cache_level=(0, 0), return_eigvec=False)
+    e1 = sqd((labels, coeffs), states, cache_level=(1, 0), return_eigvec=False)
+    assert abs(e0 - e1) < 1e-10 * abs(e0)
+
+
+def test_cache_budget_falls_back(gpu):
+    p = small_problem(6, n=11, terms=80, nstates=900)
+    prob = _problem(p, cache=True)
+    prob.op.cache_budget = 16         # far too small: products keep the bucket join
+    v = torch.randn(p.keys.size, dtype=torch.complex128, device="cuda")
+    got = prob.matvec(v).cpu().numpy()
+    assert not prob.op._caches
+    assert _rel(got, p.matvec(v.cpu().numpy())) < 1e-12
