@@ -300,7 +300,8 @@ typedef struct sqdk_cache {
   const void* diag;             /* [n] complex X = 0 coefficient of each row */
   const uint32_t* row_len;      /* [n] hits of each row (its entries before the padding) */
   int32_t precision;            /* SQDK_C128 or SQDK_C64 */
-  int32_t reserved;
+  int32_t slices_per_wave;      /* product: 1 (or 0) where rows have many hits, 4 where few
+                                   (the builder's choice: 4 at most 8 entries per row) */
 } sqdk_cache;
 
 /* Slices of the subspace: n_items x chunk_rows / 64 (chunks of 1024, 2048 or

@@ -96,7 +96,7 @@ class SqdkCache(C.Structure):
     _fields_ = [("n", C.c_int64), ("n_slices", C.c_int64), ("slice_rows", C.c_void_p),
                 ("slice_base", C.c_void_p), ("col", C.c_void_p), ("coef", C.c_void_p),
                 ("diag", C.c_void_p), ("row_len", C.c_void_p), ("precision", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("slices_per_wave", C.c_int32)]
 
 
 class SqdkOp(C.Structure):

@@ -1031,10 +1031,17 @@ template <typename VT> __device__ inline VT from_d2(double2 a);
 template <> __device__ inline float2 from_d2<float2>(double2 a) { return make_float2((float)a.x, (float)a.y); }
 template <> __device__ inline double2 from_d2<double2>(double2 a) { return a; }
 
-// out = diag v + the hit table's product, one wave per slice, FP64 sums of
-// exact products (complex64 operands are widened before multiplying). A lane
-// reads only its row's hits (rlen), so the padding costs no memory traffic.
-template <typename VT>
+// out = diag v + the hit table's product. A wave takes S slices and walks
+// their rows' entries side by side, so that their chains of dependent loads
+// (slice header, row, entries, gathers) overlap: S = 4 pays where rows have
+// few hits (the product is then the per-row streams, latency-bound at one
+// slice per wave), S = 1 where the gathers dominate (more waves in flight).
+// Every load is issued unconditionally from a valid address (a lane past its
+// row's hits re-reads its last entry, a row without hits entry 0) and its
+// product is masked, which keeps the loads free of branches. FP64 sums of
+// exact products (complex64 operands are widened before multiplying), each
+// row's in group order.
+template <typename VT, int S>
 __global__ void __launch_bounds__(kBlock)
 hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
                   const uint64_t* __restrict__ sbase, const uint32_t* __restrict__ col,
@@ -1045,43 +1052,89 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
   if (skip && skip[0] != 0) return;
   if (counter && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(counter, 1);
   const int lane = threadIdx.x & (kWave - 1);
-  const int64_t sid = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  if (sid >= n_slices) return;
-  const uint32_t first = srows[2 * sid], rows = srows[2 * sid + 1];
-  if ((uint32_t)lane >= rows) return;
-  const int64_t row = (int64_t)first + lane;
-  const uint64_t b0 = sbase[sid] + lane;
-  const int len = (int)rlen[row];
-  const double2 dd = to_d2(diag[row]), xd = to_d2(v[row]);
-  double sr = dd.x * xd.x - dd.y * xd.y;
-  double si = dd.x * xd.y + dd.y * xd.x;
-  constexpr int U = 4;
-  int k = 0;
-  for (; k + U <= len; k += U) {
-    uint32_t j[U];
-    double2 c[U], x[U];
+  const int64_t sid0 = ((int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * S;
+  if (sid0 >= n_slices) return;
+  if constexpr (S == 1) {
+    // one slice: each lane walks its own row's hits, four entries per step
+    const uint32_t first = srows[2 * sid0], rows = srows[2 * sid0 + 1];
+    if ((uint32_t)lane >= rows) return;
+    const int64_t row = (int64_t)first + lane;
+    const uint64_t b0 = sbase[sid0] + lane;
+    const int len = (int)rlen[row];
+    const double2 dd = to_d2(diag[row]), xd = to_d2(v[row]);
+    double sr = dd.x * xd.x - dd.y * xd.y;
+    double si = dd.x * xd.y + dd.y * xd.x;
+    constexpr int U = 4;
+    int k = 0;
+    for (; k + U <= len; k += U) {
+      uint32_t j[U];
+      double2 c[U], x[U];
 #pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const uint64_t e = b0 + (uint64_t)(k + u) * kWave;
+      for (int u = 0; u < U; ++u) {
+        const uint64_t e = b0 + (uint64_t)(k + u) * kWave;
+        j[u] = col[e];
+        c[u] = to_d2(hcoef[e]);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = to_d2(v[j[u]]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        sr = fma(c[u].x, x[u].x, fma(-c[u].y, x[u].y, sr));
+        si = fma(c[u].x, x[u].y, fma(c[u].y, x[u].x, si));
+      }
+    }
+    for (; k < len; ++k) {
+      const uint64_t e = b0 + (uint64_t)k * kWave;
+      const double2 c = to_d2(hcoef[e]);
+      const double2 x = to_d2(v[col[e]]);
+      sr = fma(c.x, x.x, fma(-c.y, x.y, sr));
+      si = fma(c.x, x.y, fma(c.y, x.x, si));
+    }
+    out[row] = from_d2<VT>(make_double2(sr, si));
+    return;
+  }
+  bool valid[S];
+  int64_t row[S];
+  uint64_t b0[S];
+  int len[S];
+  double sr[S], si[S];
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    const int64_t sid = min(sid0 + u, n_slices - 1);
+    const uint32_t first = srows[2 * sid], rows = srows[2 * sid + 1];
+    valid[u] = sid0 + u < n_slices && (uint32_t)lane < rows;
+    row[u] = valid[u] ? (int64_t)first + lane : 0;
+    b0[u] = sbase[sid] + lane;
+  }
+  int kmax = 0;
+#pragma unroll
+  for (int u = 0; u < S; ++u) {
+    len[u] = valid[u] ? (int)rlen[row[u]] : 0;
+    kmax = max(kmax, len[u]);
+    const double2 dd = to_d2(diag[row[u]]), xd = to_d2(v[row[u]]);
+    sr[u] = valid[u] ? dd.x * xd.x - dd.y * xd.y : 0.0;
+    si[u] = valid[u] ? dd.x * xd.y + dd.y * xd.x : 0.0;
+  }
+  for (int k = 0; k < kmax; ++k) {
+    uint32_t j[S];
+    double2 c[S];
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+      const uint64_t e = len[u] > 0 ? b0[u] + (uint64_t)min(k, len[u] - 1) * kWave : 0;
       j[u] = col[e];
       c[u] = to_d2(hcoef[e]);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) x[u] = to_d2(v[j[u]]);
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      sr = fma(c[u].x, x[u].x, fma(-c[u].y, x[u].y, sr));
-      si = fma(c[u].x, x[u].y, fma(c[u].y, x[u].x, si));
+    for (int u = 0; u < S; ++u) {
+      double2 x = to_d2(v[len[u] > 0 ? j[u] : (uint32_t)row[u]]);
+      if (k >= len[u]) x = make_double2(0.0, 0.0);
+      sr[u] = fma(c[u].x, x.x, fma(-c[u].y, x.y, sr[u]));
+      si[u] = fma(c[u].x, x.y, fma(c[u].y, x.x, si[u]));
     }
   }
-  for (; k < len; ++k) {
-    const uint64_t e = b0 + (uint64_t)k * kWave;
-    const double2 c = to_d2(hcoef[e]);
-    const double2 x = to_d2(v[col[e]]);
-    sr = fma(c.x, x.x, fma(-c.y, x.y, sr));
-    si = fma(c.x, x.y, fma(c.y, x.x, si));
-  }
-  out[row] = from_d2<VT>(make_double2(sr, si));
+#pragma unroll
+  for (int u = 0; u < S; ++u)
+    if (valid[u]) out[row[u]] = from_d2<VT>(make_double2(sr[u], si[u]));
 }
 
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
@@ -1219,22 +1272,28 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
   SQDK_REQUIRE(c->n >= 1 && c->n < (1ll << 31) && c->n_slices >= 1, "cache sizes out of range");
   SQDK_REQUIRE(c->slice_rows && c->slice_base && c->diag && c->row_len,
                "cache has NULL tables");
-  const int64_t grid = ceil_div(c->n_slices, kBlock / kWave);
+  SQDK_REQUIRE(c->slices_per_wave == 0 || c->slices_per_wave == 1 || c->slices_per_wave == 4,
+               "slices_per_wave %d not 1 or 4", c->slices_per_wave);
+  const int spw = c->slices_per_wave == 4 ? 4 : 1;
+  const int64_t grid = ceil_div(c->n_slices, (kBlock / kWave) * spw);
   SQDK_REQUIRE(grid < (1ll << 31), "too many slices");
-  if (precision == SQDK_C128)
-    hit_matvec_kernel<double2><<<(unsigned)grid, kBlock, 0, s>>>(
+  auto launch = [&](auto kern, auto vt) {
+    using VT = decltype(vt);
+    kern<<<(unsigned)grid, kBlock, 0, s>>>(
         c->n_slices, c->slice_rows, c->slice_base, c->col,
-        reinterpret_cast<const double2*>(c->coef), reinterpret_cast<const double2*>(c->diag),
-        c->row_len, reinterpret_cast<const double2*>(v), reinterpret_cast<double2*>(out), skip,
-        counter);
-  else if (precision == SQDK_C64)
-    hit_matvec_kernel<float2><<<(unsigned)grid, kBlock, 0, s>>>(
-        c->n_slices, c->slice_rows, c->slice_base, c->col,
-        reinterpret_cast<const float2*>(c->coef), reinterpret_cast<const float2*>(c->diag),
-        c->row_len, reinterpret_cast<const float2*>(v), reinterpret_cast<float2*>(out), skip,
-        counter);
-  else
+        reinterpret_cast<const typename CoefT<VT>::type*>(c->coef),
+        reinterpret_cast<const VT*>(c->diag), c->row_len, reinterpret_cast<const VT*>(v),
+        reinterpret_cast<VT*>(out), skip, counter);
+  };
+  if (precision == SQDK_C128) {
+    if (spw == 4) launch(hit_matvec_kernel<double2, 4>, double2{});
+    else launch(hit_matvec_kernel<double2, 1>, double2{});
+  } else if (precision == SQDK_C64) {
+    if (spw == 4) launch(hit_matvec_kernel<float2, 4>, float2{});
+    else launch(hit_matvec_kernel<float2, 1>, float2{});
+  } else {
     return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  }
   SQDK_LAUNCHED("hit_matvec_kernel");
   return SQDK_OK;
 }

@@ -444,10 +444,13 @@ class PauliOperator:
         _lib.check(L.sqdk_cache_fill(C.byref(h), C.byref(s), precision, sbase.data_ptr(),
                                      col.data_ptr(), coef.data_ptr(), diag.data_ptr(),
                                      rlen.data_ptr(), stream), "sqdk_cache_fill")
+        # few entries per row: the product is latency-bound, a wave takes 4 slices
+        spw = 4 if entries <= 8 * self.space.n else 1
         cs = _lib.SqdkCache(self.space.n, ns, srows.data_ptr(), sbase.data_ptr(), col.data_ptr(),
-                            coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, 0)
+                            coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, spw)
         self._caches[precision] = (cs, srows, sbase, col, coef, diag, rlen)
-        self.cache_info[precision] = {"slices": ns, "entries": entries, "bytes": nbytes}
+        self.cache_info[precision] = {"slices": ns, "entries": entries, "bytes": nbytes,
+                                      "slices_per_wave": spw}
         return True
 
     def cache_struct(self, precision: int):

@@ -145,6 +145,26 @@ def test_sqd_cache_levels_agree(gpu):
     assert abs(e0 - e1) < 1e-10 * abs(e0)
 
 
+def test_slices_per_wave_same_product(gpu):
+    """Both launch shapes of the product give the same bits."""
+    from rqutils_amd import _lib as L
+    import ctypes as C
+    p = small_problem(9, n=12, terms=120, nstates=3000)
+    prob = _problem(p, cache=True)
+    assert prob.op.build_cache(L.C128)
+    cs = prob.op._caches[L.C128][0]
+    v = torch.randn(p.keys.size, dtype=torch.complex128, device="cuda")
+    outs = []
+    for spw in (1, 4):
+        cs.slices_per_wave = spw
+        o = torch.empty_like(v)
+        L.check(L.lib().sqdk_cache_matvec(C.byref(cs), v.data_ptr(), o.data_ptr(),
+                                          L.stream_ptr(v.device)), "cache matvec")
+        outs.append(o.cpu().numpy())
+    assert np.array_equal(outs[0], outs[1])
+    assert _rel(outs[0], p.matvec(v.cpu().numpy())) < 1e-13
+
+
 def test_cache_budget_falls_back(gpu):
     p = small_problem(6, n=11, terms=80, nstates=900)
     prob = _problem(p, cache=True)
