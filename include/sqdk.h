@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 14
+#define SQDK_ABI_VERSION 15
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -142,6 +142,8 @@ typedef struct sqdk_batch {
                                 items, one per lookup, each skipping the other's instances */
   int32_t dense_fine_bits;   /* largest fine_bits over the dense-lookup instances */
   int32_t reserved;
+  const struct sqdk_cache* cache; /* or NULL: the batched solve's products read this hit
+                                     table (sqdk_batch_cache_*) instead of joining */
 } sqdk_batch;
 
 /* Launch tuning of the bucket-join matvec (0 = library default). */
@@ -318,6 +320,16 @@ int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int32_t precis
 /* out = H v through the table (v, out: n complex of the cache's precision);
  * sums in FP64, each row's in group order; a row reads its own hits only. */
 int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* out, void* stream);
+
+/* The same table for a whole batch (rows and slices global, one table): slices
+ * = n_items x chunk_rows / 64; count and fill as above. The product is
+ * sqdk_cache_matvec; sqdk_batch.cache hands it to the batched solve. */
+int sqdk_batch_cache_slices(const sqdk_batch* batch, int64_t* n_slices);
+int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slice_rows, uint64_t* slice_base,
+                           void* stream);
+int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision, const uint64_t* slice_base,
+                          uint32_t* col, void* coef, void* diag, uint32_t* row_len,
+                          void* stream);
 
 /* ---- single-vector LOBPCG ---------------------------------------------- */
 

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 14
+ABI_VERSION = 15
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -53,6 +53,13 @@ class SqdkInst(C.Structure):
                 ("bucket_bits", C.c_int32), ("fine_bits", C.c_int32), ("fbits_base", C.c_int64)]
 
 
+class SqdkCache(C.Structure):
+    _fields_ = [("n", C.c_int64), ("n_slices", C.c_int64), ("slice_rows", C.c_void_p),
+                ("slice_base", C.c_void_p), ("col", C.c_void_p), ("coef", C.c_void_p),
+                ("diag", C.c_void_p), ("row_len", C.c_void_p), ("precision", C.c_int32),
+                ("slices_per_wave", C.c_int32)]
+
+
 class SqdkBatch(C.Structure):
     _fields_ = [
         ("keys", C.c_void_p), ("bstart", C.c_void_p), ("fdir", C.c_void_p), ("items", C.c_void_p),
@@ -63,7 +70,7 @@ class SqdkBatch(C.Structure):
         ("n", C.c_int64), ("n_inst", C.c_int32), ("n_items", C.c_int32),
         ("chunk_rows", C.c_int32), ("max_bucket", C.c_int32), ("max_fine_bits", C.c_int32),
         ("max_bucket_bits", C.c_int32), ("fbits", C.c_void_p), ("dense_fine_bits", C.c_int32),
-        ("reserved", C.c_int32),
+        ("reserved", C.c_int32), ("cache", C.POINTER(SqdkCache)),
     ]
 
 
@@ -90,13 +97,6 @@ class SqdkLocgResult(C.Structure):
 
 
 MATVEC_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p)
-
-
-class SqdkCache(C.Structure):
-    _fields_ = [("n", C.c_int64), ("n_slices", C.c_int64), ("slice_rows", C.c_void_p),
-                ("slice_base", C.c_void_p), ("col", C.c_void_p), ("coef", C.c_void_p),
-                ("diag", C.c_void_p), ("row_len", C.c_void_p), ("precision", C.c_int32),
-                ("slices_per_wave", C.c_int32)]
 
 
 class SqdkOp(C.Structure):
@@ -159,6 +159,12 @@ SIGNATURES = {
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "sqdk_cache_matvec": (C.c_int, [C.POINTER(SqdkCache), C.c_void_p, C.c_void_p, C.c_void_p]),
+    "sqdk_batch_cache_slices": (C.c_int, [C.POINTER(SqdkBatch), C.POINTER(C.c_int64)]),
+    "sqdk_batch_cache_count": (C.c_int, [C.POINTER(SqdkBatch), C.c_void_p, C.c_void_p,
+                                         C.c_void_p]),
+    "sqdk_batch_cache_fill": (C.c_int, [C.POINTER(SqdkBatch), C.c_int32, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
     "sqdk_locg_solve": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.c_void_p,
                                   C.POINTER(SqdkLocgOpts), C.c_void_p, C.c_size_t,
                                   C.POINTER(SqdkLocgResult), C.c_void_p]),

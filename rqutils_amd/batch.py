@@ -62,6 +62,9 @@ class SQDBatch:
         self._concat()
         self.tune = _lib.SqdkTune(rows_per_thread, int(group_split), 1, 0, int(bool(compact)))
         self._work = {}
+        self.use_cache = False   # products (and solves) through the batch's hit table
+        self._caches = {}
+        self.cache_info = {}
 
     def _concat(self):
         dev = self.device
@@ -202,6 +205,48 @@ class SQDBatch:
                 if nb.value else None
         return self._work[key]
 
+    def build_cache(self, precision: int) -> dict:
+        """The hit table of the whole batch in ``precision`` (one table over
+        the concatenated rows; see ``PauliOperator.build_cache``); products and
+        solves read it while ``use_cache`` is set. Returns its size."""
+        if precision in self._caches:
+            return self.cache_info[precision]
+        L = _lib.lib()
+        dev = self.device
+        self.cstruct.cache = C.POINTER(_lib.SqdkCache)()
+        ns = C.c_int64(0)
+        _lib.check(L.sqdk_batch_cache_slices(C.byref(self.cstruct), C.byref(ns)),
+                   "sqdk_batch_cache_slices")
+        ns = ns.value
+        srows = torch.empty(2 * ns, dtype=torch.int32, device=dev)
+        sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+        stream = _lib.stream_ptr(dev)
+        _lib.check(L.sqdk_batch_cache_count(C.byref(self.cstruct), srows.data_ptr(),
+                                            sbase.data_ptr(), stream), "sqdk_batch_cache_count")
+        entries = int(sbase[-1].item())
+        cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
+        col = torch.empty(max(entries, 1), dtype=torch.int32, device=dev)
+        coef = torch.empty(max(entries, 1), dtype=cdt, device=dev)
+        diag = torch.empty(self.n, dtype=cdt, device=dev)
+        rlen = torch.empty(self.n, dtype=torch.int32, device=dev)
+        _lib.check(L.sqdk_batch_cache_fill(C.byref(self.cstruct), precision, sbase.data_ptr(),
+                                           col.data_ptr(), coef.data_ptr(), diag.data_ptr(),
+                                           rlen.data_ptr(), stream), "sqdk_batch_cache_fill")
+        spw = 4 if entries <= 8 * self.n else 1
+        cs = _lib.SqdkCache(self.n, ns, srows.data_ptr(), sbase.data_ptr(), col.data_ptr(),
+                            coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, spw)
+        self._caches[precision] = (cs, srows, sbase, col, coef, diag, rlen)
+        esz = 16 if precision == _lib.C128 else 8
+        self.cache_info[precision] = {"slices": ns, "entries": entries, "slices_per_wave": spw,
+                                      "bytes": entries * (4 + esz) + self.n * (esz + 4) + ns * 16}
+        return self.cache_info[precision]
+
+    def _cache_struct(self, precision: int):
+        if not self.use_cache:
+            return None
+        self.build_cache(precision)
+        return self._caches[precision][0]
+
     def matvec(self, v: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         """Every instance's H_i v_i in one launch (concatenated complex vectors)."""
         prec = _lib.C128 if v.dtype == torch.complex128 else _lib.C64
@@ -209,6 +254,12 @@ class SQDBatch:
             raise ValueError("vector must be complex of the batch's total length")
         v = v.contiguous()
         out = torch.empty_like(v) if out is None else out
+        cs = self._cache_struct(prec)
+        if cs is not None:
+            _lib.check(_lib.lib().sqdk_cache_matvec(C.byref(cs), v.data_ptr(), out.data_ptr(),
+                                                    _lib.stream_ptr(self.device)),
+                       "sqdk_cache_matvec")
+            return out
         work = self._workspace(prec)
         _lib.check(_lib.lib().sqdk_batch_matvec(
             C.byref(self.cstruct), prec, v.data_ptr(), out.data_ptr(), C.byref(self.tune),
@@ -236,6 +287,8 @@ class SQDBatch:
         opts.check_every = int(refresh_every)
         opts.refresh_every = int(refresh_every)
         opts.use_graph = 1 if use_graph else 0
+        cs = self._cache_struct(prec)
+        self.cstruct.cache = C.pointer(cs) if cs is not None else C.POINTER(_lib.SqdkCache)()
         nb = C.c_size_t(0)
         _lib.check(L.sqdk_batch_locg_workspace(C.byref(self.cstruct), C.byref(self.tune),
                                                C.byref(opts), C.byref(nb)),
@@ -245,9 +298,13 @@ class SQDBatch:
         import time
         torch.cuda.synchronize(self.device)
         t0 = time.perf_counter()
-        _lib.check(L.sqdk_batch_locg_solve(C.byref(self.cstruct), C.byref(self.tune), x.data_ptr(),
-                                           C.byref(opts), work.data_ptr(), nb.value, res,
-                                           _lib.stream_ptr(self.device)), "sqdk_batch_locg_solve")
+        try:
+            _lib.check(L.sqdk_batch_locg_solve(C.byref(self.cstruct), C.byref(self.tune),
+                                               x.data_ptr(), C.byref(opts), work.data_ptr(),
+                                               nb.value, res, _lib.stream_ptr(self.device)),
+                       "sqdk_batch_locg_solve")
+        finally:
+            self.cstruct.cache = C.POINTER(_lib.SqdkCache)()
         torch.cuda.synchronize(self.device)
         dt = time.perf_counter() - t0
         infos = [LocgInfo(r.eigval, r.niter, bool(r.converged), r.matvecs, r.resid_norm,

@@ -296,6 +296,9 @@ struct BSolver {
   unsigned items() const { return (unsigned)b->n_items; }
 
   int matvec(const void* in, void* out, bool skip) {
+    if (b->cache)
+      return hit_matvec_impl(b->cache, prec, in, out, s, nullptr, nullptr, b->item_inst,
+                             skip ? conv : nullptr, b->chunk_rows / kWave);
     return batch_matvec_impl(b, prec, in, out, tune, mv_work, mv_bytes, s, nullptr,
                              skip ? conv : nullptr, nullptr);
   }
@@ -499,6 +502,9 @@ int sqdk_batch_locg_solve(const sqdk_batch* batch, const sqdk_tune* tune, void* 
                "unknown criterion %d", opts->criterion);
   SQDK_REQUIRE(opts->maxiter >= 0, "maxiter %d < 0", opts->maxiter);
   SQDK_REQUIRE(batch->row_base && batch->item_base, "batch lacks row_base or item_base");
+  SQDK_REQUIRE(!batch->cache || (batch->cache->precision == opts->precision &&
+                                 batch->cache->n == batch->n),
+               "cached hit table does not match the solve (precision or size)");
   const size_t need = batch_locg_bytes(batch, tune, opts->precision);
   SQDK_REQUIRE(work_bytes >= need, "workspace of %zu bytes, %zu needed", work_bytes, need);
 

@@ -75,8 +75,12 @@ int diag_stats_impl(int64_t n, const double* diag, double* partials, int max_blo
                     double* out, hipStream_t s);
 
 // out = the cached hit table's product (precision must be the cache's).
+// In a batch, item_inst / inst_skip / slices_per_item leave converged
+// instances' slices alone.
 int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void* out,
-                    hipStream_t s, const int32_t* skip = nullptr, int32_t* counter = nullptr);
+                    hipStream_t s, const int32_t* skip = nullptr, int32_t* counter = nullptr,
+                    const uint16_t* item_inst = nullptr, const int32_t* inst_skip = nullptr,
+                    int slices_per_item = 0);
 
 int convert_impl(int64_t n, int from_p, const void* in, int to_p, void* out, hipStream_t s);
 

@@ -817,7 +817,8 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
                  const int32_t* __restrict__ grp_off, const uint64_t* __restrict__ zmask,
                  const typename CoefT<VT>::type* __restrict__ coef,
                  const uint32_t* __restrict__ class_hi, const int32_t* __restrict__ class_off,
-                 int n_classes, int diag_t0, int diag_t1, int m, int d, int max_bucket,
+                 const sqdk_inst* __restrict__ descs, const uint16_t* __restrict__ item_inst,
+                 sqdk_inst single, int dmax, int max_bucket,
                  uint32_t* __restrict__ srows, uint64_t* __restrict__ sbase,
                  uint32_t* __restrict__ col, typename CoefT<VT>::type* __restrict__ hcoef,
                  VT* __restrict__ diag, uint32_t* __restrict__ rlen) {
@@ -829,10 +830,19 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   const int wave = tid / kWave;
   const int wslot = wave * (kWave * ROWS);
   const int w = blockIdx.x;
+  // The chunk's instance (a batch) or the single one; a mixed batch runs one
+  // launch per lookup, each leaving the other's chunks alone.
+  const sqdk_inst D = descs ? descs[item_inst[w]] : single;
+  if (descs && (D.fbits_base >= 0) != DENSE) return;
+  const uint32_t* __restrict__ bstart_i = bstart + D.bstart_base;
+  const uint16_t* __restrict__ fdir_i = fdir + D.fdir_base;
+  const uint32_t* __restrict__ fbits_i = DENSE ? fbits + D.fbits_base : fbits;
+  const int m = D.bucket_bits;
+  const int d = D.fine_bits;
   const int f = m - d;
   const uint64_t lowmask = (m >= 64) ? ~0ull : ((1ull << m) - 1ull);
   const int ndir = 1 << d;
-  const LdsLayout lay = lds_layout(d, max_bucket, sizeof(KT), 0);
+  const LdsLayout lay = lds_layout(dmax, max_bucket, sizeof(KT), 0);
   uint32_t* lpair = reinterpret_cast<uint32_t*>(smem + lay.pairs);
   int* s_occ = reinterpret_cast<int*>(smem + lay.occ);
   KT* lkeys = reinterpret_cast<KT*>(smem + lay.keys);
@@ -842,7 +852,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   const uint64_t bucket = keys[row0] >> m;
   const uint64_t bucket_base = m >= 64 ? 0ull : (bucket << m);
   const bool narrow = ((bucket_base | lowmask) >> 32) == 0;
-  const uint32_t row1 = min(bstart[bucket + 1], row0 + (uint32_t)(ROWS * kBlock));
+  const uint32_t row1 = min(bstart_i[bucket + 1], row0 + (uint32_t)(ROWS * kBlock));
   KT sl[ROWS];
   bool valid[ROWS];
   uint32_t cnt[ROWS];
@@ -860,8 +870,8 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
     RT cr[ROWS], ci[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) cr[r] = ci[r] = RT(0);
-    if (diag_t1 > diag_t0)
-      rows_coef<CT, KT, ROWS>(zmask, coef, diag_t0, diag_t1, bucket_base, narrow, sl, cr, ci);
+    if (D.diag_t1 > D.diag_t0)
+      rows_coef<CT, KT, ROWS>(zmask, coef, D.diag_t0, D.diag_t1, bucket_base, narrow, sl, cr, ci);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       VT dv;
@@ -892,18 +902,18 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       }
     }
   };
-  for (int c = 0; c < n_classes; ++c) {
+  for (int c = D.class_base; c < D.class_base + D.n_classes; ++c) {
     const int g0 = class_off[c], g1 = class_off[c + 1];
     if (g0 >= g1) continue;
     const uint64_t q = bucket ^ (uint64_t)class_hi[c];
-    const uint32_t qs = bstart[q];
-    const uint32_t qe = bstart[q + 1];
+    const uint32_t qs = bstart_i[q];
+    const uint32_t qe = bstart_i[q + 1];
     if (qs == qe) continue;
     const int qn = (int)(qe - qs);
     if constexpr (DENSE) {
       __syncthreads();
       const uint64_t fbase = q << d;
-      for (int e = tid; e < ndir; e += kBlock) ldir[e] = make_uint2(fbits[fbase + e], fdir[fbase + e]);
+      for (int e = tid; e < ndir; e += kBlock) ldir[e] = make_uint2(fbits_i[fbase + e], fdir_i[fbase + e]);
       __syncthreads();
       if (!wave_active) continue;
       for (int g = g0; g < g1; ++g) {
@@ -929,8 +939,8 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       const uint64_t fbase = q << d;
       int occ = 0;
       for (int e = tid; e < ndir; e += kBlock) {
-        const uint32_t lo = fdir[fbase + e];
-        const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdir[fbase + e + 1] : (uint32_t)qn;
+        const uint32_t lo = fdir_i[fbase + e];
+        const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdir_i[fbase + e + 1] : (uint32_t)qn;
         lpair[e] = lo | (hi << 16);
         occ = max(occ, (int)(hi - lo));
       }
@@ -1041,6 +1051,8 @@ template <> __device__ inline double2 from_d2<double2>(double2 a) { return a; }
 // product is masked, which keeps the loads free of branches. FP64 sums of
 // exact products (complex64 operands are widened before multiplying), each
 // row's in group order.
+// In a batch, the slices of an instance whose flag inst_skip is set are left
+// alone (slices_per_item slices per chunk, a multiple of S).
 template <typename VT, int S>
 __global__ void __launch_bounds__(kBlock)
 hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
@@ -1048,12 +1060,14 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
                   const typename CoefT<VT>::type* __restrict__ hcoef, const VT* __restrict__ diag,
                   const uint32_t* __restrict__ rlen, const VT* __restrict__ v,
                   VT* __restrict__ out, const int32_t* __restrict__ skip,
-                  int32_t* __restrict__ counter) {
+                  int32_t* __restrict__ counter, const uint16_t* __restrict__ item_inst,
+                  const int32_t* __restrict__ inst_skip, int slices_per_item) {
   if (skip && skip[0] != 0) return;
   if (counter && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(counter, 1);
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t sid0 = ((int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave) * S;
   if (sid0 >= n_slices) return;
+  if (inst_skip && inst_skip[item_inst[sid0 / slices_per_item]] != 0) return;
   if constexpr (S == 1) {
     // one slice: each lane walks its own row's hits, four entries per step
     const uint32_t first = srows[2 * sid0], rows = srows[2 * sid0 + 1];
@@ -1137,51 +1151,135 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
     if (valid[u]) out[row[u]] = from_d2<VT>(make_double2(sr[u], si[u]));
 }
 
+// Tables of one hit-table pass (a single instance or a batch).
+struct HitTables {
+  const uint64_t* keys;
+  const uint32_t* bstart;
+  const uint16_t* fdir;
+  const uint32_t* fbits;
+  const uint32_t* items;
+  const uint64_t* xmask;
+  const int32_t* grp_off;
+  const uint64_t* zmask;
+  const void* coef;
+  const uint32_t* class_hi;
+  const int32_t* class_off;
+  const sqdk_inst* descs;      // NULL: single instance
+  const uint16_t* item_inst;
+  sqdk_inst single;
+  int n_items, chunk_rows, max_bucket, max_fine_bits, narrow;
+  int dense_fine_bits;         // dense-lookup chunks: their largest fine_bits
+  bool has_dense, has_search;
+};
+
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
-int launch_hit_table(const sqdk_ham* h, const sqdk_space* sp, const void* coef,
-                     uint32_t* srows, uint64_t* sbase, uint32_t* col, void* hcoef, void* diag,
-                     uint32_t* rlen, hipStream_t s) {
+int launch_hit_table(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint32_t* col,
+                     void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
   auto kern = hit_table_kernel<VT, KT, ROWS, DENSE, MODE>;
-  const size_t lds = DENSE ? ((size_t)8 << sp->fine_bits)
-                           : lds_layout(sp->fine_bits, sp->max_bucket, sizeof(KT), 0).total;
+  const int dmax = DENSE ? t.dense_fine_bits : t.max_fine_bits;
+  const size_t lds = DENSE ? ((size_t)8 << dmax) : lds_layout(dmax, t.max_bucket, sizeof(KT), 0).total;
   if (lds > 160 * 1024) return fail(SQDK_ERANGE, "LDS use of %zu bytes exceeds 160 KiB", lds);
   if (lds > 65536)
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  kern<<<(unsigned)sp->n_items, kBlock, lds, s>>>(
-      sp->keys, sp->bstart, sp->fdir, sp->fbits, sp->items, h->xmask, h->grp_off, h->zmask,
-      reinterpret_cast<const typename CoefT<VT>::type*>(coef), h->class_hi, h->class_off,
-      h->n_classes, h->diag_t0, h->diag_t1, sp->bucket_bits, sp->fine_bits, sp->max_bucket,
-      srows, sbase, col, reinterpret_cast<typename CoefT<VT>::type*>(hcoef),
-      reinterpret_cast<VT*>(diag), rlen);
+  kern<<<(unsigned)t.n_items, kBlock, lds, s>>>(
+      t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask, t.grp_off, t.zmask,
+      reinterpret_cast<const typename CoefT<VT>::type*>(t.coef), t.class_hi, t.class_off,
+      t.descs, t.item_inst, t.single, dmax, t.max_bucket, srows, sbase, col,
+      reinterpret_cast<typename CoefT<VT>::type*>(hcoef), reinterpret_cast<VT*>(diag), rlen);
   SQDK_LAUNCHED("hit_table_kernel");
   return SQDK_OK;
 }
 
-template <typename VT, int MODE>
-int hit_table_dispatch(const sqdk_ham* h, const sqdk_space* sp, const void* coef,
-                       uint32_t* srows, uint64_t* sbase, uint32_t* col, void* hcoef, void* diag,
-                       uint32_t* rlen, hipStream_t s) {
-  const int rows = sp->chunk_rows / kBlock;
-  const bool dense = sp->fbits != nullptr;
-  const bool narrow = sp->bucket_bits <= 32;
-#define SQDK_HT(R)                                                                            \
-  if (rows == R) {                                                                            \
-    if (dense)                                                                                \
-      return launch_hit_table<VT, uint32_t, R, true, MODE>(h, sp, coef, srows, sbase, col,    \
-                                                           hcoef, diag, rlen, s);             \
-    if (narrow)                                                                               \
-      return launch_hit_table<VT, uint32_t, R, false, MODE>(h, sp, coef, srows, sbase, col,   \
-                                                            hcoef, diag, rlen, s);            \
-    return launch_hit_table<VT, uint64_t, R, false, MODE>(h, sp, coef, srows, sbase, col,     \
-                                                          hcoef, diag, rlen, s);              \
+template <typename VT, int MODE, int R>
+int hit_table_rows(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint32_t* col,
+                   void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
+  if (t.has_dense)
+    SQDK_TRY((launch_hit_table<VT, uint32_t, R, true, MODE>(t, srows, sbase, col, hcoef, diag,
+                                                            rlen, s)));
+  if (t.has_search) {
+    if (t.narrow)
+      return launch_hit_table<VT, uint32_t, R, false, MODE>(t, srows, sbase, col, hcoef, diag,
+                                                            rlen, s);
+    return launch_hit_table<VT, uint64_t, R, false, MODE>(t, srows, sbase, col, hcoef, diag,
+                                                          rlen, s);
   }
-  SQDK_HT(4)
-  SQDK_HT(8)
-  SQDK_HT(16)
-#undef SQDK_HT
-  return fail(SQDK_EINVAL, "chunk of %d rows: the hit table needs 1024, 2048 or 4096",
-              sp->chunk_rows);
+  return SQDK_OK;
+}
+
+template <typename VT, int MODE>
+int hit_table_dispatch(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint32_t* col,
+                       void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
+  switch (t.chunk_rows / kBlock) {
+    case 4: return hit_table_rows<VT, MODE, 4>(t, srows, sbase, col, hcoef, diag, rlen, s);
+    case 8: return hit_table_rows<VT, MODE, 8>(t, srows, sbase, col, hcoef, diag, rlen, s);
+    case 16: return hit_table_rows<VT, MODE, 16>(t, srows, sbase, col, hcoef, diag, rlen, s);
+    default:
+      return fail(SQDK_EINVAL, "chunk of %d rows: the hit table needs 1024, 2048 or 4096",
+                  t.chunk_rows);
+  }
+}
+
+HitTables hit_tables_single(const sqdk_ham* h, const sqdk_space* sp, const void* coef) {
+  HitTables t{};
+  t.keys = sp->keys;
+  t.bstart = sp->bstart;
+  t.fdir = sp->fdir;
+  t.fbits = sp->fbits;
+  t.items = sp->items;
+  t.xmask = h->xmask;
+  t.grp_off = h->grp_off;
+  t.zmask = h->zmask;
+  t.coef = coef;
+  t.class_hi = h->class_hi;
+  t.class_off = h->class_off;
+  t.descs = nullptr;
+  t.item_inst = nullptr;
+  t.single.bstart_base = 0;
+  t.single.fdir_base = 0;
+  t.single.class_base = 0;
+  t.single.n_classes = h->n_classes;
+  t.single.diag_t0 = h->diag_t0;
+  t.single.diag_t1 = h->diag_t1;
+  t.single.bucket_bits = sp->bucket_bits;
+  t.single.fine_bits = sp->fine_bits;
+  t.single.fbits_base = 0;
+  t.n_items = sp->n_items;
+  t.chunk_rows = sp->chunk_rows;
+  t.max_bucket = sp->max_bucket;
+  t.max_fine_bits = sp->fine_bits;
+  t.dense_fine_bits = sp->fine_bits;
+  t.narrow = sp->bucket_bits <= 32;
+  t.has_dense = sp->fbits != nullptr;
+  t.has_search = sp->fbits == nullptr;
+  return t;
+}
+
+HitTables hit_tables_batch(const sqdk_batch* b, const void* coef) {
+  HitTables t{};
+  t.keys = b->keys;
+  t.bstart = b->bstart;
+  t.fdir = b->fdir;
+  t.fbits = b->fbits;
+  t.items = b->items;
+  t.xmask = b->xmask;
+  t.grp_off = b->grp_off;
+  t.zmask = b->zmask;
+  t.coef = coef;
+  t.class_hi = b->class_hi;
+  t.class_off = b->class_off;
+  t.descs = b->inst;
+  t.item_inst = b->item_inst;
+  memset(&t.single, 0, sizeof(t.single));
+  t.n_items = b->n_items;
+  t.chunk_rows = b->chunk_rows;
+  t.max_bucket = b->max_bucket;
+  t.max_fine_bits = b->max_fine_bits;
+  t.dense_fine_bits = b->dense_fine_bits;
+  t.narrow = b->max_bucket_bits <= 32;
+  t.has_dense = b->fbits != nullptr;
+  t.has_search = true;   // a launch with no searched chunk returns at once
+  return t;
 }
 
 int check_hit_space(const sqdk_ham* ham, const sqdk_space* space) {
@@ -1264,7 +1362,8 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
 }
 
 int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void* out,
-                    hipStream_t s, const int32_t* skip, int32_t* counter) {
+                    hipStream_t s, const int32_t* skip, int32_t* counter,
+                    const uint16_t* item_inst, const int32_t* inst_skip, int slices_per_item) {
   SQDK_REQUIRE(c != nullptr, "cache is NULL");
   SQDK_REQUIRE(v && out, "NULL vector");
   SQDK_REQUIRE(c->precision == precision, "cache precision %d, vectors %d", c->precision,
@@ -1274,6 +1373,8 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
                "cache has NULL tables");
   SQDK_REQUIRE(c->slices_per_wave == 0 || c->slices_per_wave == 1 || c->slices_per_wave == 4,
                "slices_per_wave %d not 1 or 4", c->slices_per_wave);
+  SQDK_REQUIRE(!inst_skip || (item_inst && slices_per_item % 4 == 0),
+               "instance skipping needs the chunk of each slice");
   const int spw = c->slices_per_wave == 4 ? 4 : 1;
   const int64_t grid = ceil_div(c->n_slices, (kBlock / kWave) * spw);
   SQDK_REQUIRE(grid < (1ll << 31), "too many slices");
@@ -1283,7 +1384,8 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
         c->n_slices, c->slice_rows, c->slice_base, c->col,
         reinterpret_cast<const typename CoefT<VT>::type*>(c->coef),
         reinterpret_cast<const VT*>(c->diag), c->row_len, reinterpret_cast<const VT*>(v),
-        reinterpret_cast<VT*>(out), skip, counter);
+        reinterpret_cast<VT*>(out), skip, counter, item_inst, inst_skip,
+        slices_per_item > 0 ? slices_per_item : 1);
   };
   if (precision == SQDK_C128) {
     if (spw == 4) launch(hit_matvec_kernel<double2, 4>, double2{});
@@ -1393,8 +1495,9 @@ extern "C" int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space,
   hipStream_t s = as_stream(stream);
   const int64_t ns = (int64_t)space->n_items * (space->chunk_rows / kWave);
   SQDK_HIP(hipMemsetAsync(slice_base + ns, 0, sizeof(uint64_t), s));
-  SQDK_TRY((hit_table_dispatch<float2, 0>(ham, space, ham->coef32, slice_rows, slice_base,
-                                          nullptr, nullptr, nullptr, nullptr, s)));
+  SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_single(ham, space, ham->coef32),
+                                          slice_rows, slice_base, nullptr, nullptr, nullptr,
+                                          nullptr, s)));
   scan_u64_kernel<<<1, 1024, 0, s>>>(slice_base, ns + 1);
   SQDK_LAUNCHED("scan_u64_kernel");
   return SQDK_OK;
@@ -1410,13 +1513,13 @@ extern "C" int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int
   uint64_t* sb = const_cast<uint64_t*>(slice_base);
   if (precision == SQDK_C128) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return hit_table_dispatch<double2, 1>(ham, space, ham->coef, nullptr, sb, col, coef, diag,
-                                          row_len, s);
+    return hit_table_dispatch<double2, 1>(hit_tables_single(ham, space, ham->coef), nullptr, sb,
+                                          col, coef, diag, row_len, s);
   }
   if (precision == SQDK_C64) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return hit_table_dispatch<float2, 1>(ham, space, ham->coef32, nullptr, sb, col, coef, diag,
-                                         row_len, s);
+    return hit_table_dispatch<float2, 1>(hit_tables_single(ham, space, ham->coef32), nullptr, sb,
+                                         col, coef, diag, row_len, s);
   }
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }
@@ -1426,4 +1529,47 @@ extern "C" int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* o
   using namespace sqdk;
   SQDK_REQUIRE(cache != nullptr, "cache is NULL");
   return hit_matvec_impl(cache, cache->precision, v, out, as_stream(stream), nullptr, nullptr);
+}
+
+extern "C" int sqdk_batch_cache_slices(const sqdk_batch* batch, int64_t* n_slices) {
+  using namespace sqdk;
+  SQDK_REQUIRE(batch && n_slices, "NULL pointer argument");
+  SQDK_REQUIRE(batch->chunk_rows == 1024 || batch->chunk_rows == 2048 || batch->chunk_rows == 4096,
+               "chunk of %d rows: the hit table needs 1024, 2048 or 4096", batch->chunk_rows);
+  *n_slices = (int64_t)batch->n_items * (batch->chunk_rows / kWave);
+  return SQDK_OK;
+}
+
+extern "C" int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slice_rows,
+                                      uint64_t* slice_base, void* stream) {
+  using namespace sqdk;
+  SQDK_REQUIRE(batch && slice_rows && slice_base, "NULL pointer argument");
+  SQDK_REQUIRE(batch->keys && batch->bstart && batch->fdir && batch->items && batch->item_inst &&
+                   batch->inst, "batch has NULL subspace tables");
+  SQDK_REQUIRE(batch->n_items >= 1, "empty batch");
+  int64_t ns = 0;
+  SQDK_TRY(sqdk_batch_cache_slices(batch, &ns));
+  hipStream_t s = as_stream(stream);
+  SQDK_HIP(hipMemsetAsync(slice_base + ns, 0, sizeof(uint64_t), s));
+  SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_batch(batch, batch->coef32), slice_rows,
+                                          slice_base, nullptr, nullptr, nullptr, nullptr, s)));
+  scan_u64_kernel<<<1, 1024, 0, s>>>(slice_base, ns + 1);
+  SQDK_LAUNCHED("scan_u64_kernel");
+  return SQDK_OK;
+}
+
+extern "C" int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision,
+                                     const uint64_t* slice_base, uint32_t* col, void* coef,
+                                     void* diag, uint32_t* row_len, void* stream) {
+  using namespace sqdk;
+  SQDK_REQUIRE(batch && slice_base && col && coef && diag && row_len, "NULL pointer argument");
+  hipStream_t s = as_stream(stream);
+  uint64_t* sb = const_cast<uint64_t*>(slice_base);
+  if (precision == SQDK_C128)
+    return hit_table_dispatch<double2, 1>(hit_tables_batch(batch, batch->coef), nullptr, sb, col,
+                                          coef, diag, row_len, s);
+  if (precision == SQDK_C64)
+    return hit_table_dispatch<float2, 1>(hit_tables_batch(batch, batch->coef32), nullptr, sb,
+                                         col, coef, diag, row_len, s);
+  return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }

@@ -47,10 +47,12 @@ def targets_from_vector(psi: torch.Tensor) -> torch.Tensor:
 
 
 def gaussian_start(n: int, device, seed: int = 0) -> torch.Tensor:
-    """Seeded complex Gaussian start vector (the method without the network)."""
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    return torch.complex(torch.randn(n, generator=g, dtype=torch.float64),
-                         torch.randn(n, generator=g, dtype=torch.float64)).to(device)
+    """Seeded complex Gaussian start vector (the method without the network),
+    drawn on the device."""
+    dev = torch.device(device)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    return torch.complex(torch.randn(n, generator=g, dtype=torch.float64, device=dev),
+                         torch.randn(n, generator=g, dtype=torch.float64, device=dev))
 
 
 def ground_targets(prob, *, tol: float = 1e-10, maxiter: int = 5000, seed: int = 0):

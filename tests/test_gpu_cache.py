@@ -173,3 +173,52 @@ def test_cache_budget_falls_back(gpu):
     got = prob.matvec(v).cpu().numpy()
     assert not prob.op._caches
     assert _rel(got, p.matvec(v.cpu().numpy())) < 1e-12
+
+
+def _batch_instances(k=6, seed=0):
+    probs = []
+    for i in range(k):
+        n = 10 + (i % 4)
+        probs.append(small_problem(seed + i, n=n, terms=40 + 15 * i,
+                                   nstates=min(2 ** n - 1, 300 + 200 * i)))
+    return probs
+
+
+@pytest.mark.parametrize("dense,rows", [(False, 4), (None, 4), (False, 8)])
+def test_batch_cache_matches_join(gpu, dense, rows):
+    """One table over a batch (search and, with dense=None, dense-lookup
+    instances mixed): every instance's product equals its join product."""
+    from rqutils_amd.batch import SQDBatch
+    probs = _batch_instances()
+    inst = [((p.labels, p.coeffs), p.keys, p.n) for p in probs]
+    b = SQDBatch(inst, bucket_cap=512, rows_per_thread=rows, dense=dense)
+    rng = np.random.default_rng(4)
+    v = torch.from_numpy(rng.normal(size=b.n) + 1j * rng.normal(size=b.n)).cuda()
+    join = b.matvec(v).cpu().numpy()
+    b.use_cache = True
+    info = b.build_cache(_lib.C128)
+    assert info["entries"] > 0
+    cached = b.matvec(v).cpu().numpy()
+    for i, p in enumerate(probs):
+        lo, hi = b.row_base_host[i], b.row_base_host[i + 1]
+        want = p.matvec(v.cpu().numpy()[lo:hi])
+        assert _rel(cached[lo:hi], want) < 1e-13, i
+        assert _rel(cached[lo:hi], join[lo:hi]) < 1e-13, i
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_batch_cache_solve(gpu, use_graph):
+    from rqutils_amd.batch import SQDBatch
+    probs = _batch_instances(k=5, seed=70)
+    b = SQDBatch([((p.labels, p.coeffs), p.keys, p.n) for p in probs], bucket_cap=512)
+    rng = np.random.default_rng(5)
+    x0 = torch.from_numpy(rng.normal(size=b.n) + 1j * rng.normal(size=b.n)).cuda()
+    plain = b.solve(x0, precision="c128", tol=1e-9, maxiter=2000, use_graph=use_graph)
+    b.use_cache = True
+    cached = b.solve(x0, precision="c128", tol=1e-9, maxiter=2000, use_graph=use_graph)
+    for i, p in enumerate(probs):
+        a, c = plain.infos[i], cached.infos[i]
+        assert a.converged and c.converged
+        ev = p.ground()
+        assert abs(c.eigval - ev) / abs(ev) < 1e-10
+        assert abs(a.eigval - c.eigval) / abs(ev) < 1e-11
