@@ -446,6 +446,15 @@ int sqdk_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t l
  * q < num_qubits, extra[i*n_extra + q - num_qubits] for the next n_extra
  * columns (per-state features, e.g. the scaled diagonal energy; extra may be
  * NULL when n_extra = 0), 0 up to kpad. */
+/* Fused inference of the 256-wide network: y [n x 2] (FP32) for n keys and
+ * their n x n_extra features, from the transposed bf16 weights W1^T [256 x
+ * kpad], W2^T [256 x 256], W3^T [16 x 256] (rows 2-15 zero) and the FP32
+ * biases. One kernel, activations in registers; the same roundings and order
+ * as the training path's forward pass. No counterpart in the reference. */
+int sqdk_mlp_infer(const uint64_t* keys, int64_t n, int32_t num_qubits, const float* extra,
+                   int32_t n_extra, int32_t kpad, const uint16_t* w1t, const float* b1,
+                   const uint16_t* w2t, const float* b2, const uint16_t* w3t, const float* b3,
+                   float* y, void* stream);
 int sqdk_mlp_encode(const uint64_t* keys, int64_t n, int32_t num_qubits, const float* extra,
                     int32_t n_extra, int32_t kpad, uint16_t* out, void* stream);
 

@@ -192,6 +192,115 @@ adam_kernel(int64_t n, float* __restrict__ w, const float* __restrict__ g, float
 
 inline unsigned grid_for(int64_t n) { return (unsigned)std::max<int64_t>(1, ceil_div(n, kBlock)); }
 
+// ---- fused inference -----------------------------------------------------------
+//
+// y = the network's outputs for n bitstrings in one kernel: encoding, three
+// layers, biases and GELUs, no intermediate in HBM. The layers are computed
+// transposed, H^T = W^T X^T, so that the MFMA accumulator layout of one layer
+// (lane l holds feature 4 (l >> 4) + j of row l & 15) is exactly the B
+// operand the next layer needs: activations stay in registers. A wave takes 16
+// rows at a time; W2^T (256 x 256 bf16) sits in LDS for the workgroup's whole
+// run, each row padded by 16 bytes so that the 16 feature rows a fragment
+// read touches fall in different banks; W1^T and W3^T (small) are read
+// through the caches. Same bf16 roundings, GELU and k order as the training
+// path (encode_kernel, gemm_bf16_kernel, act_kernel).
+constexpr int kInferH = 256;
+constexpr int kInferThreads = 512;
+constexpr int kW2Stride = kInferH + 8;   // bf16 per LDS row of W2^T
+
+__global__ void __launch_bounds__(kInferThreads)
+mlp_infer_kernel(const uint64_t* __restrict__ keys, int64_t n, int nq,
+                 const float* __restrict__ extra, int n_extra, int kpad,
+                 const uint16_t* __restrict__ w1t, const float* __restrict__ b1,
+                 const uint16_t* __restrict__ w2t, const float* __restrict__ b2,
+                 const uint16_t* __restrict__ w3t, const float* __restrict__ b3,
+                 float* __restrict__ y) {
+  extern __shared__ __align__(16) uint16_t lw2[];    // [256][kW2Stride]
+  __shared__ float lb1[kInferH], lb2[kInferH];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kInferH * kInferH / 4; i += kInferThreads) {
+    const int f = i / (kInferH / 4), k4 = i % (kInferH / 4);
+    *reinterpret_cast<uint2*>(lw2 + f * kW2Stride + 4 * k4) =
+        reinterpret_cast<const uint2*>(w2t)[i];
+  }
+  for (int i = tid; i < kInferH; i += kInferThreads) {
+    lb1[i] = b1[i];
+    lb2[i] = b2[i];
+  }
+  __syncthreads();
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int r = lane & 15;          // row of the 16-row block (B / D column)
+  const int kq = 4 * (lane >> 4);   // this lane's first k (B) / feature (D) within a step
+  const int64_t nblk = (n + 15) / 16;
+  const int ks1 = kpad / 16;
+  for (int64_t blk = (int64_t)blockIdx.x * (kInferThreads / 64) + wave; blk < nblk;
+       blk += (int64_t)gridDim.x * (kInferThreads / 64)) {
+    const int64_t row = blk * 16 + r;
+    const bool valid = row < n;
+    const uint64_t key = valid ? keys[row] : 0ull;
+    f32x4 acc[16];
+    bf16x4 hb[16];
+    // layer 1: K = kpad (bits as +-1, then the extra features, then zeros)
+#pragma unroll
+    for (int ft = 0; ft < 16; ++ft) acc[ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < ks1; ++ks) {
+      bf16x4 xb;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = 16 * ks + kq + j;
+        float v = 0.f;
+        if (valid) {
+          if (k < nq) v = ((key >> k) & 1ull) ? -1.f : 1.f;
+          else if (k < nq + n_extra) v = extra[row * n_extra + (k - nq)];
+        }
+        xb[j] = (short)f2bf(v);
+      }
+#pragma unroll
+      for (int ft = 0; ft < 16; ++ft) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(w1t + (int64_t)(16 * ft + r) * kpad +
+                                                           16 * ks + kq);
+        acc[ft] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, xb, acc[ft], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int ft = 0; ft < 16; ++ft) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hb[ft][j] = (short)f2bf(gelu(acc[ft][j] + lb1[16 * ft + kq + j]));
+    }
+    // layer 2: 256 x 256 from LDS
+#pragma unroll
+    for (int ft = 0; ft < 16; ++ft) acc[ft] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      // one k step's 16 fragments live at a time (keeps the registers in bounds)
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int ft = 0; ft < 16; ++ft) {
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(lw2 + (16 * ft + r) * kW2Stride +
+                                                           16 * ks + kq);
+        acc[ft] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hb[ks], acc[ft], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int ft = 0; ft < 16; ++ft) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) hb[ft][j] = (short)f2bf(gelu(acc[ft][j] + lb2[16 * ft + kq + j]));
+    }
+    // layer 3: 2 outputs (W3^T padded to 16 rows of zeros)
+    f32x4 o = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < 16; ++ks) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(w3t + r * kInferH + 16 * ks + kq);
+      o = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, hb[ks], o, 0, 0, 0);
+    }
+    if (kq == 0 && valid) {
+      y[row * 2] = o[0] + b3[0];
+      y[row * 2 + 1] = o[1] + b3[1];
+    }
+  }
+}
+
 }  // namespace
 
 }  // namespace sqdk
@@ -228,6 +337,25 @@ int sqdk_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t l
     gemm_reduce_kernel<<<grid_for(M * N), kBlock, 0, s>>>(M, N, ksplit, work, C, ldc, bias);
     SQDK_LAUNCHED("gemm_reduce_kernel");
   }
+  return SQDK_OK;
+}
+
+int sqdk_mlp_infer(const uint64_t* keys, int64_t n, int32_t num_qubits, const float* extra,
+                   int32_t n_extra, int32_t kpad, const uint16_t* w1t, const float* b1,
+                   const uint16_t* w2t, const float* b2, const uint16_t* w3t, const float* b3,
+                   float* y, void* stream) {
+  SQDK_REQUIRE(keys && w1t && b1 && w2t && b2 && w3t && b3 && y, "NULL pointer argument");
+  SQDK_REQUIRE(n_extra >= 0 && (n_extra == 0 || extra), "extra features missing");
+  SQDK_REQUIRE(n >= 1 && num_qubits >= 1 && num_qubits <= 64, "bad inference shape");
+  SQDK_REQUIRE(kpad % 16 == 0 && kpad >= num_qubits + n_extra && kpad <= 128,
+               "kpad %d must be a multiple of 16 covering the inputs (at most 128)", kpad);
+  const size_t lds = sizeof(uint16_t) * kInferH * kW2Stride;
+  SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(mlp_infer_kernel),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const int64_t blocks = std::min<int64_t>(ceil_div(ceil_div(n, 16), kInferThreads / 64), 256);
+  mlp_infer_kernel<<<(unsigned)blocks, kInferThreads, lds, as_stream(stream)>>>(
+      keys, n, num_qubits, extra, n_extra, kpad, w1t, b1, w2t, b2, w3t, b3, y);
+  SQDK_LAUNCHED("mlp_infer_kernel");
   return SQDK_OK;
 }
 

@@ -290,9 +290,33 @@ class StartNet:
                     print(f"[startnet] step {step} loss {losses[-1]:.4f}", flush=True)
         return losses
 
+    def infer(self, keys: torch.Tensor, extra: torch.Tensor | None = None) -> torch.Tensor:
+        """Outputs y (n x 2, FP32) through the fused inference kernel
+        (``sqdk_mlp_infer``; 256 hidden units): one launch, the same values as
+        ``forward``."""
+        if self.h != 256:
+            raise ValueError("the fused inference kernel is built for 256 hidden units")
+        keys = keys.contiguous()
+        extra = self._check_extra(keys, extra)
+        n = keys.numel()
+        w1t = self.bf16[0].t().contiguous()
+        w2t = self.bf16[2].t().contiguous()
+        w3t = torch.zeros((16, self.h), dtype=torch.int16, device=self.device)
+        w3t[:2] = self.bf16[4].t()
+        y = torch.empty((n, 2), dtype=torch.float32, device=self.device)
+        _lib.check(_lib.lib().sqdk_mlp_infer(
+            keys.data_ptr(), n, self.nq, _p(extra), self.n_extra, self.kpad, w1t.data_ptr(),
+            self.params[1].data_ptr(), w2t.data_ptr(), self.params[3].data_ptr(),
+            w3t.data_ptr(), self.params[5].data_ptr(), y.data_ptr(), self._stream()),
+            "sqdk_mlp_infer")
+        return y
+
     def predict(self, keys: torch.Tensor, extra: torch.Tensor | None = None,
                 chunk: int = 1 << 18) -> torch.Tensor:
-        """Outputs y (n x 2, FP32) over any number of states, in chunks."""
+        """Outputs y (n x 2, FP32) over any number of states: the fused
+        inference kernel (256 hidden units), else the training path in chunks."""
+        if self.h == 256 and keys.numel() > 0:
+            return self.infer(keys, extra)
         outs = []
         for s in range(0, keys.numel(), chunk):
             e = None if extra is None else extra[s:s + chunk].contiguous()

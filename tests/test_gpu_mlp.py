@@ -215,6 +215,33 @@ def test_startnet_forward_backward_matches_oracle(gpu, n_extra):
         assert _rel(gd, gx) < 5e-2, (k, _rel(gd, gx))
 
 
+@pytest.mark.parametrize("nq,n_extra,n", [(20, 0, 3000), (28, 2, 1001), (16, 0, 17),
+                                           (40, 2, 5000), (64, 3, 333)])
+def test_fused_inference_matches_forward(gpu, nq, n_extra, n):
+    """sqdk_mlp_infer (one kernel, activations in registers) against the
+    training path's forward pass (encode, three GEMMs, GELUs) and the oracle."""
+    from rqutils_amd.startnet import StartNet
+    net = StartNet(nq, hidden=256, device=gpu, seed=5, n_extra=n_extra)
+    rng = np.random.default_rng(nq + n)
+    for i in (1, 3, 5):
+        net.params[i].copy_(_f32(0.1 * rng.normal(size=tuple(net.params[i].shape)), gpu))
+    net._sync_bf16()
+    hi = np.uint64(0xFFFFFFFFFFFFFFFF) if nq == 64 else np.uint64((1 << nq) - 1)
+    keys = rng.integers(0, np.iinfo(np.uint64).max, size=n, dtype=np.uint64, endpoint=True) & hi
+    extra = rng.uniform(0, 3, size=(n, n_extra)).astype(np.float32)
+    kt = torch.from_numpy(keys.view(np.int64)).to(gpu)
+    et = _f32(extra, gpu) if n_extra else None
+    y_ref = net.forward(kt, et).cpu().numpy()
+    y = net.infer(kt, et).cpu().numpy()
+    assert np.max(np.abs(y - y_ref)) <= 1e-5 * max(1.0, np.max(np.abs(y_ref)))
+    p = [t.cpu().numpy().astype(np.float64) for t in net.params]
+    pb = [mo.bf16_round(p[0]), p[1], mo.bf16_round(p[2]), p[3], mo.bf16_round(p[4]), p[5]]
+    x = mo.encode(keys, nq, net.kpad, extra=mo.bf16_round(extra) if n_extra else None)
+    _, _, y_o, _ = mo.loss_and_grads(pb, x, np.zeros((n, 3)) + [0, 0, 1], lam=0.0,
+                                     rounding=mo.bf16_round)
+    assert _rel(y.astype(np.float64), y_o) < 1e-4
+
+
 def test_startnet_start_beats_gaussian(gpu):
     """Train on six disordered 16-qubit Heisenberg chains (ground vectors from
     the FP64 LOBPCG), then start the FP32/FP64 LOBPCG on two held-out chains
