@@ -96,7 +96,10 @@ class StartNet:
         self.kpad = 16 * math.ceil(self.n_in / 16)
         self.h = int(hidden)
         self.lam = float(lam)
-        self.device = torch.device(device if device is not None else "cuda")
+        dev = torch.device(device if device is not None else "cuda")
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         g = torch.Generator(device="cpu").manual_seed(seed)
 
         def init(fan_in, shape):
