@@ -84,7 +84,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="heisenberg28",
-                    choices=["heisenberg28", "random40", "small16", "batch64", "netstart",
+                    choices=["heisenberg28", "random40", "small16", "batch64", "netstart", "nextrows",
                              "svsim"])
     ap.add_argument("--instances", type=int, default=64, help="instances of batch64")
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-solution block")
@@ -127,6 +127,9 @@ def main():
         return bench_svsim(args, dev)
     if args.workload == "netstart":
         bench_netstart(args, dev)
+        return
+    if args.workload == "nextrows":
+        bench_next_rows(args, dev)
         return
     if args.calibrate_copy:
         # A plain copy of known size: 1 GiB read + 1 GiB written per launch.
@@ -645,6 +648,104 @@ def bench_netstart(args, dev):
                          "parallelism": "single GPU"},
               "network_start": res}
     print(json.dumps(record), flush=True)
+
+
+def bench_next_rows(args, dev):
+    """The rows next to the hot path, timed against their CPU ports: the
+    full-space product PauliSumXZ.matmul (symplectic.py:131-150) on the whole
+    2^22 space of a 22-qubit chain with 1,000 disorder strings (complex128;
+    reported as products/s and gathers/s, beside the least HBM traffic of one
+    fused pass and the bytes the reference's per-group passes would move),
+    and hproj (sqd.py:251-296) on 2^18 sampled
+    states of a 24-qubit chain (one hit-table build, unpacking and the scipy
+    CSR assembly). CPU: the oracle's NumPy restatements on a sample of 3
+    groups, scaled by the group count."""
+    import torch
+    from oracle import sqd_oracle as so
+    from rqutils_amd import workloads as W
+    from rqutils_amd.operator import keys_to_states
+    from rqutils_amd.paulis.symplectic import PauliSumXZ
+    from rqutils_amd.sqd import hproj
+    rec = {}
+    nq = 22
+    w = W.heisenberg28(seed=2, n_states=1 << 10, n_random=1000, num_qubits=nq)
+    ham = PauliSumXZ.from_paulisum(w.hamiltonian, force_real=True, add_padding=True)
+    g = torch.Generator(device=dev).manual_seed(3)
+    v = torch.complex(torch.randn(1 << nq, generator=g, device=dev, dtype=torch.float64),
+                      torch.randn(1 << nq, generator=g, device=dev, dtype=torch.float64))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = ham.matmul(v)
+    torch.cuda.synchronize()
+    first_s = time.perf_counter() - t0
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(args.steps):
+        out = ham.matmul(v)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    groups = int(ham.x.shape[0])
+    # The reference's per-group passes move 48 B per amplitude and group
+    # through memory; the fused kernel keeps each output in registers, so its
+    # least HBM traffic is one read of the right-hand side and one write of
+    # the output (32 B per amplitude), and the group gathers hit the cache.
+    algo = 48 * (1 << nq) * groups
+    min_bytes = 32 * (1 << nq)
+    gathers = (1 << nq) * groups
+    vh = v.cpu().numpy()
+    t0 = time.perf_counter()
+    part = so.fullspace_matmul(ham.x, ham.z, ham.c, nq, vh, groups=[0, 1, 2])
+    cpu_s = (time.perf_counter() - t0) * groups / 3
+    got = ham.matmul(v).cpu().numpy()
+    # the sampled groups' share of the product, computed on the device
+    ham3 = PauliSumXZ(ham.x[:3], ham.z[:3], ham.c[:3], nq, ham.padding)
+    share = ham3.matmul(v).cpu().numpy()
+    diff = float(np.linalg.norm(share - part) / np.linalg.norm(part))
+    rec["fullspace_matmul"] = {
+        "num_qubits": nq, "groups": groups, "terms": len(w.labels), "precision": "c128",
+        "first_call_s": round(first_s, 3), "ms": round(ms, 3),
+        "products_per_s": round(1e3 / ms, 2),
+        "reference_pass_bytes": algo,
+        "reference_pass_equivalent_gbs": round(algo / (ms * 1e-3) / 1e9, 1),
+        "min_hbm_bytes": min_bytes,
+        "min_hbm_gbs": round(min_bytes / (ms * 1e-3) / 1e9, 1),
+        "gathers_per_s": round(gathers / (ms * 1e-3), 1),
+        "cpu_port_s_extrapolated": round(cpu_s, 2),
+        "cpu_sample": f"oracle fullspace_matmul over 3 of {groups} groups, x{groups / 3:.1f}",
+        "rel_diff_vs_oracle_on_3_groups": diff, "norm": float(np.linalg.norm(got))}
+    log(f"full-space matmul 2^{nq} x {groups} groups: {ms:.2f} ms per product (first call "
+        f"{first_s:.2f} s with setup), {gathers / (ms * 1e-3) / 1e9:.0f} G gathers/s "
+        f"(the reference's passes would move {algo / (ms * 1e-3) / 1e12:.1f} TB/s); CPU port "
+        f"{cpu_s:.1f} s (extrapolated); 3-group share vs oracle {diff:.1e}")
+    w2 = W.heisenberg28(seed=2, n_states=1 << 18, n_random=1000, num_qubits=24)
+    states = keys_to_states(w2.keys, 24)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m = hproj(w2.hamiltonian, states, unique_states=True)
+    torch.cuda.synchronize()
+    gpu_s = time.perf_counter() - t0
+    xs, zs, cs, _ = so.from_paulisum(w2.labels, w2.coeffs, force_real=True, add_padding=True)
+    packed = so.pack_states(states)
+    t0 = time.perf_counter()
+    r, c, _ = so.hproj_coo(xs[:3], zs[:3], cs[:3], packed)
+    cpu_h = (time.perf_counter() - t0) * xs.shape[0] / 3
+    rec["hproj"] = {"num_qubits": 24, "states": int(w2.n), "groups": int(xs.shape[0]),
+                    "nnz": int(m.nnz), "seconds": round(gpu_s, 3),
+                    "cpu_port_s_extrapolated": round(cpu_h, 2),
+                    "cpu_sample": f"oracle hproj_coo over 3 of {xs.shape[0]} groups, "
+                                  f"x{xs.shape[0] / 3:.1f}"}
+    log(f"hproj 2^18 states x {xs.shape[0]} groups: {gpu_s:.2f} s ({m.nnz} nonzeros, scipy CSR "
+        f"included); CPU port {cpu_h:.1f} s (extrapolated)")
+    print(json.dumps({"metric": METRIC, "value": rec["fullspace_matmul"]["ms"], "unit": "ms",
+                      "n_gpus": 1, "steps": args.steps, "warmup": 1,
+                      "ms_per_step": rec["fullspace_matmul"]["ms"], "higher_is_better": False,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+                      "data": "synthetic (seeded generator, rqutils_amd.workloads.heisenberg28)",
+                      "config": {"workload": "nextrows", "parallelism": "single GPU"},
+                      "next_rows": rec}), flush=True)
 
 
 def bench_batch(args, dev):

@@ -210,6 +210,33 @@ def hproj_coo(xsigs, zsigs, coeffs, states):
     return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
 
 
+def fullspace_matmul(xsigs, zsigs, coeffs, num_qubits, rhs, groups=None):
+    """``H @ rhs`` over the whole 2^n space (rqutils/paulis/symplectic.py:131-150):
+    for every group, out[i] += rhs[i ^ x] * sum_k c_k (-1)^{popcount(i & z_k)}
+    (the sign from the output index i, zero-padded terms contributing 0).
+    ``xsigs`` / ``zsigs`` are the padded packed signatures of ``from_paulisum``;
+    ``groups`` restricts the sum to those group indices (for timing samples)."""
+    n = num_qubits
+    rhs = np.asarray(rhs)
+    idx = np.arange(2 ** n, dtype=np.uint64)
+    xm = packed_to_keys(np.asarray(xsigs), n)
+    out = np.zeros(rhs.shape, dtype=np.result_type(rhs.dtype, np.asarray(coeffs).dtype))
+    for g in (range(xm.size) if groups is None else groups):
+        zm = packed_to_keys(np.asarray(zsigs[g]), n)
+        diag = np.zeros(idx.size, dtype=out.dtype)
+        for k in range(zm.size):
+            if coeffs[g][k] == 0:
+                continue
+            par = _popcount64(idx & zm[k]) & np.uint64(1)
+            diag += coeffs[g][k] * (1.0 - 2.0 * par.astype(np.float64))
+        out += rhs[(idx ^ xm[g]).astype(np.int64)] * (diag if rhs.ndim == 1 else diag[:, None])
+    return out
+
+
+def _popcount64(a):
+    return np.bitwise_count(np.asarray(a, dtype=np.uint64)).astype(np.uint64)
+
+
 def packed_to_keys(states_p, num_qubits):
     """Integer key ``sum_q s_q 2^q`` of each padded packed row (pad bit must be 0).
 

@@ -518,8 +518,16 @@ def full_space_matmul(ham: PauliSumXZ, rhs):
     if n > 31:
         raise ValueError("full-space products are limited to 31 qubits")
     dev = _device(None if is_numpy or vec.device.type != "cuda" else vec.device)
-    space = Subspace(torch.arange(2 ** n, dtype=torch.int64, device=dev), n, device=dev)
-    op = PauliOperator(ham, space, prune=False)
+    # The whole-space subspace and the operator are built once per
+    # Hamiltonian and device and kept on the Hamiltonian object.
+    ops = ham.__dict__.setdefault("_fullspace_ops", {})
+    op = ops.get(str(dev))
+    if op is None:
+        space = Subspace(torch.arange(2 ** n, dtype=torch.int64, device=dev), n, device=dev,
+                         dense=None)
+        op = PauliOperator(ham, space, prune=False)
+        ops.clear()
+        ops[str(dev)] = op
     cols = vec.reshape(vec.shape[0], -1)
     out = torch.empty(cols.shape, dtype=torch.complex128, device=dev)
     for j in range(cols.shape[1]):

@@ -177,29 +177,34 @@ def hproj(hamiltonian, states, unique_states: bool = False) -> csr_array:
     n = keys.shape[0]
     space = Subspace(keys, states.shape[1])
     op = PauliOperator(hamiltonian, space, prune=False)
-    rows, cols, vals = [], [], []
-    t = op.host
-    row_index = torch.arange(n, device=space.device)
-    if t.diag_terms:
-        d = op.diagonal_of_group0()
-        rows.append(row_index)
-        cols.append(row_index)
-        vals.append(d)
-    coef = op.coef
-    for g in range(op.n_groups):
-        xs = space.xsource(int(t.xmask[g]))
-        dg = torch.empty(n, dtype=torch.complex128, device=space.device)
-        _lib.check(_lib.lib().sqdk_diagonal(
-            op.zmask.data_ptr(), coef.data_ptr(), int(t.grp_off[g]), int(t.grp_off[g + 1]),
-            space.keys.data_ptr(), n, dg.data_ptr(), _lib.stream_ptr(space.device)),
-            "sqdk_diagonal")
-        ok = xs >= 0
-        rows.append(row_index[ok])
-        cols.append(xs[ok].long())
-        vals.append(dg[ok])
-    r = torch.cat(rows).cpu().numpy() if rows else np.zeros(0, np.int64)
-    c = torch.cat(cols).cpu().numpy() if cols else np.zeros(0, np.int64)
-    v = torch.cat(vals).cpu().numpy() if vals else np.zeros(0, np.complex128)
+    # Every (row, off-diagonal group) hit and its coefficient come from the
+    # hit table (one bucket-join pass), unpacked from its 64-row slices:
+    # entry k of the l-th row of slice s sits at slice_base[s] + 64 k + l.
+    op.build_cache(_lib.C128)
+    _, srows, sbase, col, coef, diag, rlen = op._caches[_lib.C128]
+    dev = space.device
+    sr = srows.view(-1, 2).long()
+    first, nrow = sr[:, 0], sr[:, 1]
+    sid = torch.repeat_interleave(torch.arange(first.numel(), device=dev), nrow)
+    lane = torch.arange(sid.numel(), device=dev) - torch.repeat_interleave(
+        torch.cumsum(nrow, 0) - nrow, nrow)
+    row = first[sid] + lane
+    ln = rlen[row].long()
+    k = torch.arange(int(ln.sum().item()), device=dev) - torch.repeat_interleave(
+        torch.cumsum(ln, 0) - ln, ln)
+    e = sbase[torch.repeat_interleave(sid, ln)] + 64 * k + torch.repeat_interleave(lane, ln)
+    rows = [torch.repeat_interleave(row, ln)]
+    cols = [col[e].long()]
+    vals = [coef[e]]
+    if op.host.diag_terms:
+        idx = torch.arange(n, device=dev)
+        rows.append(idx)
+        cols.append(idx)
+        vals.append(diag)
+    r = torch.cat(rows).cpu().numpy()
+    c = torch.cat(cols).cpu().numpy()
+    v = torch.cat(vals).cpu().numpy()
+    op.drop_cache()
     if not np.iscomplexobj(hamiltonian.c):
         v = v.real
     return csr_array(coo_array((v, (r, c)), shape=(n, n)))

@@ -198,6 +198,24 @@ def test_hproj_matches_dense(gpu):
     assert np.allclose(m, p.dense(), atol=1e-12)
 
 
+@pytest.mark.parametrize("seed,n,nstates,full", [(22, 12, 3000, False), (23, 11, 1500, True)])
+def test_hproj_matches_oracle_csr(gpu, seed, n, nstates, full):
+    """hproj (unpacked from the hit table) against the oracle's sort-join CSR,
+    with repeated input rows; full-weight strings give complex coefficients."""
+    from rqutils_amd.sqd import hproj
+    p = small_problem(seed, n=n, terms=120, nstates=nstates, full=full)
+    rng = np.random.default_rng(seed)
+    states = p.states[rng.integers(0, p.states.shape[0], size=2 * p.states.shape[0])]
+    states = np.concatenate([p.states, states])
+    m = hproj((p.labels, p.coeffs), states)
+    ref = p.csr()
+    assert m.shape == ref.shape
+    assert abs(m - ref).max() <= 1e-12 * max(1.0, abs(ref).max())
+    # unique_states=True with the distinct sorted rows gives the same matrix
+    m2 = hproj((p.labels, p.coeffs), p.states, unique_states=True)
+    assert abs(m2 - m).max() == 0
+
+
 def test_full_space_matmul_vs_dense(gpu):
     from rqutils_amd.paulis import PauliSumXZ
     rng = np.random.default_rng(5)

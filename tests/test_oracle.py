@@ -43,6 +43,28 @@ def test_matvec_matches_dense_kronecker():
         assert np.allclose(p.matvec(v), p.dense() @ v, atol=1e-12)
 
 
+@pytest.mark.parametrize("seed,full", [(0, False), (1, True), (2, False)])
+def test_fullspace_matmul_matches_dense_kronecker(seed, full):
+    """The full-space product (symplectic.py:131-150) against the dense
+    Kronecker matrix, on a vector and on a matrix of right-hand sides; the
+    sign of each term comes from the output index (a slip to the source index
+    would transpose the Y phases, which the complex strings expose)."""
+    rng = np.random.default_rng(seed)
+    n = 7
+    from tests.helpers import random_labels
+    labels = random_labels(rng, n, 25, 1, 4, full=full)
+    coeffs = rng.normal(size=len(labels))
+    x, z, c, _ = so.from_paulisum(labels, coeffs, add_padding=True)
+    h = dense.dense_hamiltonian(labels, coeffs)
+    v = rng.normal(size=2 ** n) + 1j * rng.normal(size=2 ** n)
+    assert np.allclose(so.fullspace_matmul(x, z, c, n, v), h @ v, atol=1e-12)
+    m = rng.normal(size=(2 ** n, 3))
+    assert np.allclose(so.fullspace_matmul(x, z, c, n, m), h @ m, atol=1e-12)
+    # a sample of groups sums to the whole when all are taken
+    part = sum(so.fullspace_matmul(x, z, c, n, v, groups=[g]) for g in range(x.shape[0]))
+    assert np.allclose(part, h @ v, atol=1e-12)
+
+
 def test_matvec_hermitian():
     p = small_problem(7, n=10, terms=60, nstates=400)
     rng = np.random.default_rng(1)
