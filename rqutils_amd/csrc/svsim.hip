@@ -27,7 +27,9 @@ namespace sqdk {
 namespace {
 
 constexpr int kSvThreads = 256;
-constexpr int kMinLow = 6;          // contiguous low qubits in every tile
+// Contiguous low qubits in every tile: runs of 128 bytes (one cache line),
+// 8 complex128 or 16 complex64 amplitudes.
+template <typename VT> constexpr int kMinLow = sizeof(VT) == 16 ? 3 : 4;
 constexpr int kMaxRunGates = 120;   // gates per launch (kernel argument space)
 
 struct SvGate {
@@ -78,7 +80,7 @@ svsim_run_kernel(VT* __restrict__ state, int64_t n_tiles, SvRun run) {
   using RT = typename SvReal<VT>::type;
   constexpr int kTile = 1 << K;
   __shared__ VT tile[kTile];
-  __shared__ uint64_t hoff[1 << (K - kMinLow)];
+  __shared__ uint64_t hoff[1 << (K - kMinLow<VT>)];
   const int tid = threadIdx.x;
   const int low = run.low;
   const uint32_t lowmask = (1u << low) - 1u;
@@ -161,19 +163,49 @@ svsim_gate_kernel(VT* __restrict__ state, int n, uint64_t x, uint64_t z, double 
 // only when a gate's X bit is not a register bit (a new layout) or a gate
 // flips several tile bits (applied in LDS as in svsim_run_kernel).
 
-enum SvOpKind : uint32_t { kOpDiag = 0, kOpPair = 1, kOpLds = 2, kOpLoad = 3, kOpStore = 4 };
-
-struct SvOp {
-  uint32_t xl, zl;   // tile-local X and Z masks (LDS gates)
-  uint64_t zo;       // Z bits outside the tile
-  double c, s;
-  uint32_t kind;
-  uint8_t j;         // pair: the register bit X flips
-  uint8_t zr;        // register ops: Z bits on the register positions (element coordinates)
-  uint16_t zt;       // register ops: Z bits on the thread positions (thread-id coordinates)
-  uint8_t rpos[4];   // load: local position of each register bit, ascending
-  uint32_t pad;
+// Op codes. Register gates: kRegDiag (Z-only) or kRegPair + j (X on register
+// bit j), plus kRegNoCos when the gate's cos is 0 (the cos term is skipped, as
+// in the reference); the code selects one straight-line variant per op.
+enum SvOpCode : uint32_t {
+  kRegPair = 0,      // + j, j < 4
+  kRegDiag = 4,
+  kRegNoCos = 8,     // added to kRegPair + j or kRegDiag
+  kOpLds = 16,       // X on several tile bits: applied in LDS
+  kOpLoad = 17,      // registers <- LDS in a new layout
+  kOpStore = 18,     // registers -> LDS
 };
+
+// The first 32 bytes are all a register gate reads (one scalar load, issued
+// one op ahead); the rest serves LDS gates and layout loads.
+struct SvOp {
+  double c, s;
+  uint64_t zo;       // Z bits outside the tile
+  uint32_t code;     // SvOpCode
+  uint16_t zt;       // register ops: Z bits on the thread positions (thread-id coordinates)
+  uint8_t zr;        // register ops: Z bits on the register positions (element coordinates)
+  uint8_t pad0;
+  uint32_t xl, zl;   // tile-local X and Z masks (LDS gates)
+  uint8_t rpos[4];   // load: local position of each register bit, ascending
+  uint32_t pad1;
+};
+static_assert(sizeof(SvOp) == 48, "op layout");
+
+struct SvHead {
+  double c, s;
+  uint64_t zo;
+  uint32_t code, zt, zr;
+};
+
+__device__ inline SvHead op_head(const SvOp& o) {
+  SvHead h;
+  h.c = o.c;
+  h.s = o.s;
+  h.zo = o.zo;
+  h.code = o.code;
+  h.zt = o.zt;
+  h.zr = o.zr;
+  return h;
+}
 
 constexpr int kMaxOps = 80;   // ops per launch (kernel argument space)
 
@@ -183,28 +215,47 @@ struct SvRunReg {
   SvOp op[kMaxOps];
 };
 
-template <typename VT, int R>
-__device__ inline void reg_rotate_all(VT (&reg)[1 << R], typename SvReal<VT>::type c,
-                                      typename SvReal<VT>::type st, uint32_t zr) {
+// out = c a + i sp b, with the cos term present (C) or skipped.
+template <bool C, typename VT>
+__device__ inline VT rotc(VT a, VT b, typename SvReal<VT>::type c,
+                          typename SvReal<VT>::type sp) {
+  VT o;
+  if constexpr (C) {
+    o.x = __builtin_fma(-sp, b.y, c * a.x);
+    o.y = __builtin_fma(sp, b.x, c * a.y);
+  } else {
+    o.x = -sp * b.y;
+    o.y = sp * b.x;
+  }
+  return o;
+}
+
+// Z-only gate on every register: amplitude e takes sign parity(e & zr).
+template <bool C, int R, typename VT>
+__device__ inline void reg_diag(VT (&reg)[1 << R], typename SvReal<VT>::type c,
+                                typename SvReal<VT>::type sp, typename SvReal<VT>::type sn,
+                                uint32_t zr) {
 #pragma unroll
   for (int e = 0; e < (1 << R); ++e) {
-    const auto sp = (__popc((uint32_t)e & zr) & 1) ? -st : st;
-    reg[e] = rot(reg[e], reg[e], c, sp);
+    const auto se = (__popc((uint32_t)e & zr) & 1) ? sn : sp;
+    reg[e] = rotc<C>(reg[e], reg[e], c, se);
   }
 }
 
-template <typename VT, int R, int B>
-__device__ inline void reg_pairs(VT (&reg)[1 << R], typename SvReal<VT>::type c,
-                                 typename SvReal<VT>::type st, uint32_t zr) {
+// Single-qubit X on register bit B (a power of two): pairs (e, e | B).
+template <bool C, int B, int R, typename VT>
+__device__ inline void reg_pair(VT (&reg)[1 << R], typename SvReal<VT>::type c,
+                                typename SvReal<VT>::type sp, typename SvReal<VT>::type sn,
+                                uint32_t zr) {
 #pragma unroll
   for (int e = 0; e < (1 << R); ++e) {
     if (e & B) continue;
     const int f = e | B;
-    const auto sl = (__popc((uint32_t)e & zr) & 1) ? -st : st;
-    const auto sm = (__popc((uint32_t)f & zr) & 1) ? -st : st;
+    const auto se = (__popc((uint32_t)e & zr) & 1) ? sn : sp;
+    const auto sf = (__popc((uint32_t)f & zr) & 1) ? sn : sp;
     const VT a = reg[e], b = reg[f];
-    reg[e] = rot(a, b, c, sl);
-    reg[f] = rot(b, a, c, sm);
+    reg[e] = rotc<C>(a, b, c, se);
+    reg[f] = rotc<C>(b, a, c, sf);
   }
 }
 
@@ -216,34 +267,72 @@ svsim_reg_kernel(VT* __restrict__ state, int64_t n_tiles, SvRunReg run) {
   constexpr int E = 1 << R;
   static_assert(kTile == kSvThreads * E, "a tile is 256 threads x 2^R amplitudes");
   __shared__ VT tile[kTile];
-  __shared__ uint64_t hoff[1 << (K - kMinLow)];
+  __shared__ uint64_t hoff[1 << (K - kMinLow<VT>)];
   const int tid = threadIdx.x;
   const int low = run.low;
+  const int nops = run.nops;
   const uint32_t lowmask = (1u << low) - 1u;
   const uint64_t nmask = run.n >= 64 ? ~0ull : ((1ull << run.n) - 1ull);
   const uint64_t hmask = run.qmask & ~(uint64_t)lowmask;
   const uint64_t omask = nmask & ~run.qmask;
   for (int h = tid; h < (1 << (K - low)); h += kSvThreads) hoff[h] = deposit((uint64_t)h, hmask);
   __syncthreads();
-  VT reg[E];
-  for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const uint64_t base = deposit((uint64_t)t, omask);
-    for (int e = tid; e < kTile; e += kSvThreads)
-      tile[e] = state[base | hoff[e >> low] | (uint32_t)(e & lowmask)];
-    uint32_t tb = 0;                  // thread part of the local index in the current layout
-    uint32_t rp[4] = {0, 0, 0, 0};    // register bit positions of the current layout
-    for (int oi = 0; oi < run.nops; ++oi) {
-      const SvOp& O = run.op[oi];
-      const RT c = (RT)O.c, s = (RT)O.s;
-      if (O.kind == kOpLoad) {
+  // Thread tid moves elements tid + 256 i of a tile. The next tile's
+  // amplitudes are loaded into registers (pre) while this tile is processed.
+  auto gidx = [&](uint64_t b, int i) {
+    const int e = tid + i * kSvThreads;
+    return b | hoff[e >> low] | (uint32_t)(e & lowmask);
+  };
+  // Tiles t, t + grid, ... of this block. The next tile's amplitudes are
+  // loaded into registers (pre) while this tile is processed; at the end of a
+  // tile its amplitudes go to registers (out), the next tile enters LDS, and
+  // only then are the stores and the following loads issued, so no wait on a
+  // load ever covers a store just issued. The loads past a block's last tile
+  // read that tile again (in bounds, never used), so they need no condition.
+  if ((int64_t)blockIdx.x >= n_tiles) return;
+  RT pre_x[E], pre_y[E];   // plain reals: an array of VT is not kept in registers
+  auto fetch = [&](uint64_t b) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const VT a = state[gidx(b, i)];
+      pre_x[i] = a.x;
+      pre_y[i] = a.y;
+    }
+  };
+  auto stage = [&]() {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      VT a;
+      a.x = pre_x[i];
+      a.y = pre_y[i];
+      tile[tid + i * kSvThreads] = a;
+    }
+  };
+  int64_t t = blockIdx.x;
+  uint64_t base = deposit((uint64_t)t, omask);
+  fetch(base);
+  stage();
+  int64_t tn = t + gridDim.x < n_tiles ? t + gridDim.x : t;
+  uint64_t nbase = deposit((uint64_t)tn, omask);
+  fetch(nbase);
+  for (;;) {
+    // The ops come as segments: a layout load, register gates, a store; or
+    // one LDS gate. The registers live within one segment only.
+    int oi = 0;
+    SvHead H = op_head(run.op[0]);
+    while (oi < nops) {
+      if (H.code == kOpLoad) {
+        const SvOp& O = run.op[oi];
         __syncthreads();
+        uint32_t rp[4] = {0, 0, 0, 0};
         uint32_t v = (uint32_t)tid;
 #pragma unroll
         for (int i = 0; i < R; ++i) {  // insert a zero bit at each register position
           rp[i] = O.rpos[i];
           v = ((v >> rp[i]) << (rp[i] + 1)) | (v & ((1u << rp[i]) - 1u));
         }
-        tb = v;
+        const uint32_t tb = v;       // thread part of the local index in this layout
+        VT reg[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           uint32_t d = 0;
@@ -251,7 +340,45 @@ svsim_reg_kernel(VT* __restrict__ state, int64_t n_tiles, SvRunReg run) {
           for (int i = 0; i < R; ++i) d |= (uint32_t)((e >> i) & 1) << rp[i];
           reg[e] = tile[tb | d];
         }
-      } else if (O.kind == kOpStore) {
+        ++oi;
+        H = op_head(run.op[oi < nops ? oi : nops - 1]);
+#pragma unroll 2
+        while (oi < nops && H.code != kOpStore) {
+          const SvHead g = H;
+          ++oi;
+          H = op_head(run.op[oi < nops ? oi : nops - 1]);   // the next op's header, early
+          const RT c = (RT)g.c, s = (RT)g.s;
+          // the thread's sign (thread bits and outside bits), then per element
+          const int pt = (__popc((uint32_t)tid & g.zt) ^ __popcll(base & g.zo)) & 1;
+          const RT sp = pt ? -s : s, sn = -sp;
+          const uint32_t zr = g.zr;
+          if constexpr (R > 3) {
+            switch (g.code) {
+              case kRegPair + 0: reg_pair<true, 1, R>(reg, c, sp, sn, zr); break;
+              case kRegPair + 1: reg_pair<true, 2, R>(reg, c, sp, sn, zr); break;
+              case kRegPair + 2: reg_pair<true, 4, R>(reg, c, sp, sn, zr); break;
+              case kRegPair + 3: reg_pair<true, 8, R>(reg, c, sp, sn, zr); break;
+              case kRegDiag: reg_diag<true, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 0: reg_pair<false, 1, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 1: reg_pair<false, 2, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 2: reg_pair<false, 4, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 3: reg_pair<false, 8, R>(reg, c, sp, sn, zr); break;
+              default: reg_diag<false, R>(reg, c, sp, sn, zr); break;
+            }
+          } else {
+            switch (g.code) {
+              case kRegPair + 0: reg_pair<true, 1, R>(reg, c, sp, sn, zr); break;
+              case kRegPair + 1: reg_pair<true, 2, R>(reg, c, sp, sn, zr); break;
+              case kRegPair + 2: reg_pair<true, 4, R>(reg, c, sp, sn, zr); break;
+              case kRegDiag: reg_diag<true, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 0: reg_pair<false, 1, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 1: reg_pair<false, 2, R>(reg, c, sp, sn, zr); break;
+              case kRegNoCos + kRegPair + 2: reg_pair<false, 4, R>(reg, c, sp, sn, zr); break;
+              default: reg_diag<false, R>(reg, c, sp, sn, zr); break;
+            }
+          }
+        }
+        // the store that closes the segment
 #pragma unroll
         for (int e = 0; e < E; ++e) {
           uint32_t d = 0;
@@ -259,7 +386,12 @@ svsim_reg_kernel(VT* __restrict__ state, int64_t n_tiles, SvRunReg run) {
           for (int i = 0; i < R; ++i) d |= (uint32_t)((e >> i) & 1) << rp[i];
           tile[tb | d] = reg[e];
         }
-      } else if (O.kind == kOpLds) {
+        ++oi;
+        H = op_head(run.op[oi < nops ? oi : nops - 1]);
+      } else {
+        // a gate with X on several tile bits, in LDS
+        const SvOp& O = run.op[oi];
+        const RT c = (RT)O.c, s = (RT)O.s;
         __syncthreads();
         const int tpar = __popcll(base & O.zo) & 1;
         const int b = __ffs(O.xl) - 1;
@@ -273,28 +405,33 @@ svsim_reg_kernel(VT* __restrict__ state, int64_t n_tiles, SvRunReg run) {
           tile[l] = rot(a, bb, c, sl);
           tile[m] = rot(bb, a, c, sm);
         }
-      } else {
-        // register gate: the thread's sign, then the element's (uniform) parity
-        const int pt = (__popc((uint32_t)tid & O.zt) ^ __popcll(base & O.zo)) & 1;
-        const RT st = pt ? -s : s;
-        if (O.kind == kOpDiag) {
-          reg_rotate_all<VT, R>(reg, c, st, O.zr);
-        } else {
-          switch (O.j) {
-            case 0: reg_pairs<VT, R, 1>(reg, c, st, O.zr); break;
-            case 1: reg_pairs<VT, R, 2>(reg, c, st, O.zr); break;
-            case 2: reg_pairs<VT, R, 4>(reg, c, st, O.zr); break;
-            default:
-              if constexpr (R > 3) reg_pairs<VT, R, 8>(reg, c, st, O.zr);
-              break;
-          }
-        }
+        ++oi;
+        H = op_head(run.op[oi < nops ? oi : nops - 1]);
       }
     }
     __syncthreads();
-    for (int e = tid; e < kTile; e += kSvThreads)
-      state[base | hoff[e >> low] | (uint32_t)(e & lowmask)] = tile[e];
-    __syncthreads();  // the next tile reuses the LDS buffer
+    RT out_x[E], out_y[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const VT a = tile[tid + i * kSvThreads];
+      out_x[i] = a.x;
+      out_y[i] = a.y;
+    }
+    __syncthreads();  // every thread has read this tile: LDS takes the next one
+    stage();
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      VT a;
+      a.x = out_x[i];
+      a.y = out_y[i];
+      state[gidx(base, i)] = a;
+    }
+    if (t + gridDim.x >= n_tiles) break;
+    t += gridDim.x;
+    base = nbase;
+    tn = t + gridDim.x < n_tiles ? t + gridDim.x : t;
+    nbase = deposit((uint64_t)tn, omask);
+    fetch(nbase);
   }
 }
 
@@ -348,7 +485,7 @@ int launch_reg_run(VT* state, int n, int low, uint64_t qmask, const int* pos,
   auto push = [&](const SvOp& op) { run.op[run.nops++] = op; };
   auto store = [&]() {
     SvOp op{};
-    op.kind = kOpStore;
+    op.code = kOpStore;
     push(op);
     have = false;
   };
@@ -372,7 +509,7 @@ int launch_reg_run(VT* state, int n, int low, uint64_t qmask, const int* pos,
       else tset[ti++] = p;
     }
     SvOp op{};
-    op.kind = kOpLoad;
+    op.code = kOpLoad;
     for (int i = 0; i < R; ++i) op.rpos[i] = (uint8_t)rset[i];
     push(op);
     have = true;
@@ -397,14 +534,17 @@ int launch_reg_run(VT* state, int n, int low, uint64_t qmask, const int* pos,
         if (have) store();
         choose(b, t);
       }
-      op.kind = bits ? kOpPair : kOpDiag;
+      uint32_t code = kRegDiag;
       for (int i = 0; i < R; ++i)
-        if (rset[i] == b) op.j = (uint8_t)i;
+        if (bits && rset[i] == b) code = kRegPair + (uint32_t)i;
+      // the cos term is skipped when cos is 0 in the kernel's precision
+      if ((typename SvReal<VT>::type)g[t].cos == 0) code += kRegNoCos;
+      op.code = code;
       op.zr = (uint8_t)extract_bits(zl[t], rset, R);
       op.zt = (uint16_t)extract_bits(zl[t], tset, K - R);
     } else {
       if (have) store();
-      op.kind = kOpLds;
+      op.code = kOpLds;
     }
     push(op);
   }
@@ -415,7 +555,7 @@ template <typename VT, int K, int R>
 int launch_runs(int n, VT* state, const sqdk_gate* gates, int64_t n_gates, int32_t* runs_out,
                 hipStream_t s) {
   const int k = n < K ? n : K;
-  const int min_low = k < kMinLow ? k : kMinLow;
+  const int min_low = k < kMinLow<VT> ? k : kMinLow<VT>;
   const uint64_t lowfix = (1ull << min_low) - 1ull;
   const uint64_t nmask = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
   const int64_t n_tiles = (int64_t)1 << (n - k);
