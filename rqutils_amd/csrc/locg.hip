@@ -71,6 +71,7 @@ enum Step : int {
   kStepFusedRR,        // fused: criterion + generalized 3x3 Rayleigh-Ritz
   kStepRefreshRho,     // fused refresh: theta = Re D1 / D0, C0 = -theta
   kStepVerify,         // D0 = |Ax|^2, D1 = |r|^2 after refresh: info only
+  kStepZeroResid,      // D0 = |r|^2 of the start: exactly 0 -> converged at rho
 };
 
 __device__ bool criterion_met(const ScalarParams& p, double rnorm, double axnorm, double theta,
@@ -102,6 +103,17 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
     case kStepRho:
       S[kTheta] = make_double2(D[0].x, 0.0);
       C[0] = make_double2(-D[0].x, 0.0);
+      break;
+    case kStepZeroResid:
+      // The start is an exact eigenvector (always so in a one-state
+      // subspace): the 2x2 Rayleigh-Ritz of the reference (ground_locg.py:
+      // 274-286, 424-435) would divide 0 by 0 here, so the solve stops at
+      // theta = rho with the normalized start.
+      if (D[0].x == 0.0) {
+        F[kConv] = 1;
+        S[kInfo] = make_double2(0.0, fabs(S[kTheta].x));
+        S[kInfo2] = make_double2(0.0, 0.0);
+      }
       break;
     case kStepInvNormGuard: {
       const double nr = sqrt(D[0].x);
@@ -218,7 +230,8 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
     }
     case kStepVerify: {
       const double axn = sqrt(D[0].x), rn = sqrt(D[1].x);
-      const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S);
+      // a zero residual is an exact eigenpair even where the threshold is 0
+      const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S) || rn == 0.0;
       F[kConv] = conv ? 1 : 0;
       break;
     }
@@ -352,23 +365,24 @@ int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_h
   SQDK_TRY(sv.dots({{x, ax}}, false));
   SQDK_TRY(sv.scalar(kStepRho, false));
   SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, false));
-  // body_iter1 (:274-286)
+  // body_iter1 (:274-286); skipped when the start is an exact eigenvector
   SQDK_TRY(sv.dots({{r, r}}, false));
-  SQDK_TRY(sv.scalar(kStepInvNormGuard, false));
-  SQDK_TRY(sv.lin(p, {r}, {C0}, false));
-  SQDK_TRY(sv.apply(x, ax, false));
-  SQDK_TRY(sv.apply(p, ap, false));
-  SQDK_TRY(sv.dots({{p, ax}, {x, ax}, {p, ap}}, false));
-  SQDK_TRY(sv.scalar(kStepEig2, false));
-  SQDK_TRY(sv.lin(y, {p, x}, {C0, C1}, false));   // t
-  SQDK_TRY(sv.lin(ay, {x, p}, {C2, C3}, false));  // u
-  SQDK_TRY(sv.dots({{ay, ay}, {y, y}}, false));
-  SQDK_TRY(sv.scalar(kStepInvNorm2, false));
-  SQDK_TRY(sv.lin(x, {ay}, {C0}, false));
-  SQDK_TRY(sv.lin(y, {y}, {C1}, false));
-  SQDK_TRY(sv.apply(x, ax, false));
-  SQDK_TRY(sv.scalar(kStepNegTheta, false));
-  SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, false));
+  SQDK_TRY(sv.scalar(kStepZeroResid, false));
+  SQDK_TRY(sv.scalar(kStepInvNormGuard, true));
+  SQDK_TRY(sv.lin(p, {r}, {C0}, true));
+  SQDK_TRY(sv.apply(x, ax, true));
+  SQDK_TRY(sv.apply(p, ap, true));
+  SQDK_TRY(sv.dots({{p, ax}, {x, ax}, {p, ap}}, true));
+  SQDK_TRY(sv.scalar(kStepEig2, true));
+  SQDK_TRY(sv.lin(y, {p, x}, {C0, C1}, true));   // t
+  SQDK_TRY(sv.lin(ay, {x, p}, {C2, C3}, true));  // u
+  SQDK_TRY(sv.dots({{ay, ay}, {y, y}}, true));
+  SQDK_TRY(sv.scalar(kStepInvNorm2, true));
+  SQDK_TRY(sv.lin(x, {ay}, {C0}, true));
+  SQDK_TRY(sv.lin(y, {y}, {C1}, true));
+  SQDK_TRY(sv.apply(x, ax, true));
+  SQDK_TRY(sv.scalar(kStepNegTheta, true));
+  SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, true));
 
   int done = 0;
   while (done < o.maxiter) {
@@ -451,6 +465,8 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
   SQDK_TRY(sv.dots({{X, AX}}, false));
   SQDK_TRY(sv.scalar(kStepRho, false));
   SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, false));
+  SQDK_TRY(sv.dots({{R, R}}, false));
+  SQDK_TRY(sv.scalar(kStepZeroResid, false));   // an exact eigenvector stops here
   SQDK_TRY(precondition(false));
 
   const void* V[3] = {X, W, P};
