@@ -156,7 +156,10 @@ __global__ void bscalar_kernel(int step, int n_inst, const int64_t* __restrict__
       const double rn = sqrt(fmax(s.gram[1].x, 0.0));
       const double axn = sqrt(fmax(s.gram[12].x, 0.0));
       double thr;
-      const bool ok = bcriterion(prm, rn, axn, s.theta, row_base[i + 1] - row_base[i], thr);
+      // a zero residual (an exact eigenvector, e.g. a one-state instance) has
+      // converged even where the threshold is 0
+      const bool ok = bcriterion(prm, rn, axn, s.theta, row_base[i + 1] - row_base[i], thr) ||
+                      rn == 0.0;
       s.rn = rn;
       s.axn = axn;
       s.thr = thr;
@@ -188,7 +191,8 @@ __global__ void bscalar_kernel(int step, int n_inst, const int64_t* __restrict__
     case kVerify: {
       const double axn = sqrt(s.dots[0].x), rn = sqrt(s.dots[1].x);
       double thr;
-      const bool ok = bcriterion(prm, rn, axn, s.theta, row_base[i + 1] - row_base[i], thr);
+      const bool ok = bcriterion(prm, rn, axn, s.theta, row_base[i + 1] - row_base[i], thr) ||
+                      rn == 0.0;
       s.rn = rn;
       s.axn = axn;
       s.thr = thr;

@@ -87,3 +87,27 @@ def test_widest_keys_matvec_and_energy(gpu, n):
     eigval, eigvec, basis = sqd((labels, coeffs), states[order])
     assert np.array_equal(basis, states)
     assert abs(eigval - e_ref) < 1e-9 * abs(e_ref), ("cached", eigval, e_ref)
+
+
+@pytest.mark.parametrize("criterion", ["relative", "reference"])
+def test_batch_with_a_one_state_instance(gpu, criterion):
+    """A one-state instance in a batch converges at its diagonal element and
+    leaves the other instances' solves unchanged."""
+    from rqutils_amd.batch import SQDBatch
+    from tests.helpers import small_problem
+    one = small_problem(40, n=9, terms=50, nstates=1)
+    probs = [small_problem(41, n=10, terms=60, nstates=300), one,
+             small_problem(42, n=11, terms=70, nstates=500)]
+    b = SQDBatch([((p.labels, p.coeffs), p.keys, p.n) for p in probs], bucket_cap=512)
+    rng = np.random.default_rng(7)
+    x0 = torch.from_numpy(rng.normal(size=b.n) + 1j * rng.normal(size=b.n)).cuda()
+    res = b.solve(x0, precision="c128", criterion=criterion, tol=1e-8, maxiter=2000)
+    for i, p in enumerate(probs):
+        info = res.infos[i]
+        ev = np.real(p.dense()[0, 0]) if p is one else p.ground()
+        assert info.converged and np.isfinite(info.eigval), (i, info)
+        # the reference criterion stops at |r| < (|Ax| - theta) N 10 tol (7e-4 here)
+        etol = 1e-12 if p is one else (1e-9 if criterion == "relative" else 1e-6)
+        assert abs(info.eigval - ev) <= etol * max(1.0, abs(ev)), (i, info.eigval, ev)
+        x = b.segment(res.vectors, i).cpu().numpy()
+        assert np.all(np.isfinite(x)), i
