@@ -111,3 +111,9 @@ def test_batch_with_a_one_state_instance(gpu, criterion):
         assert abs(info.eigval - ev) <= etol * max(1.0, abs(ev)), (i, info.eigval, ev)
         x = b.segment(res.vectors, i).cpu().numpy()
         assert np.all(np.isfinite(x)), i
+
+
+def test_sqd_empty_sample_is_an_error(gpu):
+    from rqutils_amd.sqd import sqd
+    with pytest.raises(ValueError, match="subspace size 0"):
+        sqd((["XZI", "ZZZ"], [0.5, -1.0]), np.zeros((0, 3), dtype=np.uint8))
