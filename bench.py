@@ -597,6 +597,23 @@ def svsim_block(dev, n=30, layers=10, cpu=True, cpu_qubits=18):
         log(f"svsim {name} {n}q: {len(gates)} gates in {runs} passes, {ms:.2f} ms "
             f"({rec['hbm_gbs']} GB/s of state traffic)")
         out[name] = rec
+    # The practical read-write streaming rate on this state for comparison: a
+    # plain in-place multiply by 1.0 (no change to the values), best of three.
+    best = None
+    for _ in range(3):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        state.mul_(1.0)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        t = e0.elapsed_time(e1)
+        best = t if best is None else min(best, t)
+    out["inplace_stream_reference"] = {
+        "op": "torch in-place multiply of the state by 1.0", "ms": round(best, 3),
+        "gbs": round(32 * (1 << n) / (best * 1e-3) / 1e9, 1)}
+    log(f"in-place torch stream over the same state: {best:.2f} ms "
+        f"({out['inplace_stream_reference']['gbs']} GB/s)")
     del state
     if cpu:
         from oracle import svsim_oracle as sv
