@@ -58,38 +58,52 @@ __device__ inline void eig3_reference(const double2 m[3][3], double& lam, double
 }
 
 // ---- robust small eigensolver for the fused Rayleigh-Ritz ------------------
+//
+// The sizes are template parameters and every loop is unrolled, so the 3x3
+// arrays live in registers (indexed by a runtime size they would go to
+// scratch memory, which made one Rayleigh-Ritz step cost tens of microseconds).
 
-// Cyclic complex Jacobi on an n x n (n <= 3) Hermitian matrix: each rotation
+// Cyclic complex Jacobi on an N x N (N <= 3) Hermitian matrix: each rotation
 // first removes the phase of a_pq, then applies the real Jacobi rotation.
 // Returns the lowest eigenvalue and its unit eigenvector.
-__device__ inline void jacobi_lowest(int n, double2 a[3][3], double& lam, double2 vec[3]) {
+template <int N>
+__device__ inline void jacobi_lowest(double2 a[3][3], double& lam, double2 vec[3]) {
   double2 v[3][3];
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = 0; j < 3; ++j) v[i][j] = make_double2(i == j ? 1.0 : 0.0, 0.0);
   for (int sweep = 0; sweep < 30; ++sweep) {
     double off = 0.0, diag = 0.0;
-    for (int i = 0; i < n; ++i) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
       diag += a[i][i].x * a[i][i].x;
-      for (int j = i + 1; j < n; ++j) off += cabs2(a[i][j]);
+#pragma unroll
+      for (int j = i + 1; j < N; ++j) off += cabs2(a[i][j]);
     }
     if (off <= 1e-32 * diag || off == 0.0) break;
-    for (int p = 0; p < n; ++p) {
-      for (int q = p + 1; q < n; ++q) {
+#pragma unroll
+    for (int p = 0; p < N; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < N; ++q) {
         const double b = sqrt(cabs2(a[p][q]));
         if (b == 0.0) continue;
         // phase: D = diag(.., e^{-i phi} at q) with a_pq = b e^{i phi}
         const double2 ph = make_double2(a[p][q].x / b, -a[p][q].y / b);  // e^{-i phi}
-        for (int k = 0; k < n; ++k) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
           a[k][q] = cmul(a[k][q], ph);          // column q times e^{-i phi}
           v[k][q] = cmul(v[k][q], ph);
         }
-        for (int k = 0; k < n; ++k) a[q][k] = cmul(a[q][k], cconj(ph));  // row q times e^{i phi}
+#pragma unroll
+        for (int k = 0; k < N; ++k) a[q][k] = cmul(a[q][k], cconj(ph));  // row q times e^{i phi}
         // now a_pq = b is real; real rotation zeroing it
         const double app = a[p][p].x, aqq = a[q][q].x;
         const double tau = (aqq - app) / (2.0 * b);
         const double t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
         const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
-        for (int k = 0; k < n; ++k) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
           const double2 akp = a[k][p], akq = a[k][q];
           a[k][p] = csub(cscale(akp, c), cscale(akq, s));
           a[k][q] = cadd(cscale(akp, s), cscale(akq, c));
@@ -97,7 +111,8 @@ __device__ inline void jacobi_lowest(int n, double2 a[3][3], double& lam, double
           v[k][p] = csub(cscale(vkp, c), cscale(vkq, s));
           v[k][q] = cadd(cscale(vkp, s), cscale(vkq, c));
         }
-        for (int k = 0; k < n; ++k) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
           const double2 apk = a[p][k], aqk = a[q][k];
           a[p][k] = csub(cscale(apk, c), cscale(aqk, s));
           a[q][k] = cadd(cscale(apk, s), cscale(aqk, c));
@@ -107,30 +122,120 @@ __device__ inline void jacobi_lowest(int n, double2 a[3][3], double& lam, double
       }
     }
   }
-  int imin = 0;
-  for (int i = 1; i < n; ++i)
-    if (a[i][i].x < a[imin][imin].x) imin = i;
-  lam = a[imin][imin].x;
-  for (int i = 0; i < n; ++i) vec[i] = v[i][imin];
-  for (int i = n; i < 3; ++i) vec[i] = make_double2(0.0, 0.0);
+  // the lowest diagonal entry (the first of equal ones) and its column
+  lam = a[0][0].x;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) vec[i] = i < N ? v[i][0] : make_double2(0.0, 0.0);
+#pragma unroll
+  for (int j = 1; j < N; ++j) {
+    if (a[j][j].x < lam) {
+      lam = a[j][j].x;
+#pragma unroll
+      for (int i = 0; i < N; ++i) vec[i] = v[i][j];
+    }
+  }
 }
 
-// Lower Cholesky factor of an n x n Hermitian matrix; false if a pivot^2 is
+// Lower Cholesky factor of an N x N Hermitian matrix; false if a pivot^2 is
 // not above `floor` (the matrix has unit diagonal, so pivots are sines).
-__device__ inline bool cholesky(int n, const double2 g[3][3], double2 l[3][3], double floor) {
+template <int N>
+__device__ inline bool cholesky(const double2 g[3][3], double2 l[3][3], double floor) {
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = 0; j < 3; ++j) l[i][j] = make_double2(0.0, 0.0);
-  for (int j = 0; j < n; ++j) {
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
     double s = g[j][j].x;
+#pragma unroll
     for (int k = 0; k < j; ++k) s -= cabs2(l[j][k]);
     if (!(s > floor)) return false;
     const double ljj = sqrt(s);
     l[j][j] = make_double2(ljj, 0.0);
-    for (int i = j + 1; i < n; ++i) {
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
       double2 t = g[i][j];
+#pragma unroll
       for (int k = 0; k < j; ++k) t = csub(t, cmul(l[i][k], cconj(l[j][k])));
       l[i][j] = cscale(t, 1.0 / ljj);
     }
+  }
+  return true;
+}
+
+// The generalized eigenproblem M c = lam G c on the first N basis vectors,
+// after the diagonal scaling dsc: Cholesky G = L L^H (false if G is not
+// numerically positive definite), C = L^-1 M L^-H, its lowest eigenpair by
+// Jacobi, c = L^-H y with the scaling undone.
+template <int N>
+__device__ inline bool rr_solve(const double2 G[3][3], const double2 M[3][3], const double dsc[3],
+                                double floor, double2 c[3], double& lam) {
+  double2 Gs[3][3], Ms[3][3], L[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const bool in = i < N && j < N;
+      Gs[i][j] = in ? cscale(G[i][j], dsc[i] * dsc[j]) : make_double2(0.0, 0.0);
+      Ms[i][j] = in ? cscale(M[i][j], dsc[i] * dsc[j]) : make_double2(0.0, 0.0);
+    }
+  if (!cholesky<N>(Gs, L, floor)) return false;
+  // Linv = L^{-1} (lower triangular)
+  double2 Li[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Li[i][j] = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    Li[j][j] = make_double2(1.0 / L[j][j].x, 0.0);
+#pragma unroll
+    for (int i = j + 1; i < N; ++i) {
+      double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = j; k < i; ++k) t = cadd(t, cmul(L[i][k], Li[k][j]));
+      Li[i][j] = cscale(make_double2(-t.x, -t.y), 1.0 / L[i][i].x);
+    }
+  }
+  // Cm = Linv Ms Linv^H
+  double2 T[3][3], Cm[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) t = cadd(t, cmul(Li[i][k], Ms[k][j]));
+      T[i][j] = t;
+    }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int k = 0; k < N; ++k) t = cadd(t, cmul(T[i][k], cconj(Li[j][k])));
+      Cm[i][j] = t;
+    }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    Cm[i][i].y = 0.0;
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      const double2 h = cscale(cadd(Cm[i][j], cconj(Cm[j][i])), 0.5);
+      Cm[i][j] = h;
+      Cm[j][i] = cconj(h);
+    }
+  }
+  double2 y[3];
+  jacobi_lowest<N>(Cm, lam, y);
+  // c = Linv^H y, then undo the diagonal scaling
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < N; ++k) t = cadd(t, cmul(cconj(Li[k][i]), y[k]));
+    c[i] = i < N ? cscale(t, dsc[i]) : make_double2(0.0, 0.0);
   }
   return true;
 }
@@ -139,7 +244,8 @@ __device__ inline bool cholesky(int n, const double2 g[3][3], double2 l[3][3], d
 // g[0..5] = XX RR PP XR XP RP, g[6..11] = X.AX R.AR P.AP X.AR X.AP R.AP:
 // minimise c^H M c / c^H G c. Returns 0 and the update coefficients
 // U = (c0, c1, c2, c1 s, c2 s, lam) with s normalising P' = c1 R + c2 P, or 2
-// when R is numerically parallel to X. P is used only if pvalid.
+// when R is numerically parallel to X. P is used only if pvalid; if the
+// 3-vector Gram matrix is not numerically positive definite, (X, R) is used.
 __device__ inline int fused_rr(const double2* g, bool pvalid, int precision, double2 U[6],
                                bool& pvalid_out) {
   // G_ij = (S_i, S_j), M_ij = (S_i, A S_j) over S = (X, R, P)
@@ -148,84 +254,37 @@ __device__ inline int fused_rr(const double2* g, bool pvalid, int precision, dou
   G[0][1] = g[3]; G[0][2] = g[4]; G[1][2] = g[5];
   M[0][0] = g[6]; M[1][1] = g[7]; M[2][2] = g[8];
   M[0][1] = g[9]; M[0][2] = g[10]; M[1][2] = g[11];
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     G[i][i].y = 0.0;
     M[i][i].y = 0.0;
+#pragma unroll
     for (int j = 0; j < i; ++j) {
       G[i][j] = cconj(G[j][i]);
       M[i][j] = cconj(M[j][i]);
     }
   }
-  int use = (pvalid && G[2][2].x > 0.0) ? 3 : 2;
-  double dsc[3] = {0.0, 0.0, 0.0};
-  for (int i = 0; i < use; ++i) dsc[i] = 1.0 / sqrt(G[i][i].x);
+  const bool three = pvalid && G[2][2].x > 0.0;
+  const double dsc[3] = {1.0 / sqrt(G[0][0].x), 1.0 / sqrt(G[1][1].x),
+                         three ? 1.0 / sqrt(G[2][2].x) : 0.0};
   const double floor = precision == SQDK_C128 ? 1e-14 : 1e-8;
-  double2 Gs[3][3], Ms[3][3], L[3][3];
-  bool ok = false;
-  while (use >= 2 && !ok) {
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        const bool in = i < use && j < use;
-        Gs[i][j] = in ? cscale(G[i][j], dsc[i] * dsc[j]) : make_double2(0.0, 0.0);
-        Ms[i][j] = in ? cscale(M[i][j], dsc[i] * dsc[j]) : make_double2(0.0, 0.0);
-      }
-    ok = cholesky(use, Gs, L, floor);
-    if (!ok) --use;
-  }
-  if (!ok) return 2;  // R is numerically parallel to X: nothing left to improve
-  // Linv = L^{-1} (lower triangular)
-  double2 Li[3][3];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) Li[i][j] = make_double2(0.0, 0.0);
-  for (int j = 0; j < use; ++j) {
-    Li[j][j] = make_double2(1.0 / L[j][j].x, 0.0);
-    for (int i = j + 1; i < use; ++i) {
-      double2 t = make_double2(0.0, 0.0);
-      for (int k = j; k < i; ++k) t = cadd(t, cmul(L[i][k], Li[k][j]));
-      Li[i][j] = cscale(make_double2(-t.x, -t.y), 1.0 / L[i][i].x);
-    }
-  }
-  // Cm = Linv Ms Linv^H
-  double2 T[3][3], Cm[3][3];
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      double2 t = make_double2(0.0, 0.0);
-      for (int k = 0; k < use; ++k) t = cadd(t, cmul(Li[i][k], Ms[k][j]));
-      T[i][j] = t;
-    }
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      double2 t = make_double2(0.0, 0.0);
-      for (int k = 0; k < use; ++k) t = cadd(t, cmul(T[i][k], cconj(Li[j][k])));
-      Cm[i][j] = t;
-    }
-  for (int i = 0; i < use; ++i) {
-    Cm[i][i].y = 0.0;
-    for (int j = 0; j < i; ++j) {
-      const double2 h = cscale(cadd(Cm[i][j], cconj(Cm[j][i])), 0.5);
-      Cm[i][j] = h;
-      Cm[j][i] = cconj(h);
-    }
-  }
-  double lam;
-  double2 y[3];
-  jacobi_lowest(use, Cm, lam, y);
-  // c = Linv^H y, then undo the diagonal scaling
   double2 c[3];
-  for (int i = 0; i < 3; ++i) {
-    double2 t = make_double2(0.0, 0.0);
-    for (int k = 0; k < use; ++k) t = cadd(t, cmul(cconj(Li[k][i]), y[k]));
-    c[i] = i < use ? cscale(t, dsc[i]) : make_double2(0.0, 0.0);
-  }
+  double lam;
+  bool ok = three && rr_solve<3>(G, M, dsc, floor, c, lam);
+  if (!ok) ok = rr_solve<2>(G, M, dsc, floor, c, lam);
+  if (!ok) return 2;  // R is numerically parallel to X: nothing left to improve
   // fix the phase so the X coefficient is real and non-negative
   const double a0 = sqrt(cabs2(c[0]));
   if (a0 > 0.0) {
     const double2 ph = make_double2(c[0].x / a0, -c[0].y / a0);
+#pragma unroll
     for (int i = 0; i < 3; ++i) c[i] = cmul(c[i], ph);
   }
   // P' = c1 R + c2 P, normalised with the Gram matrix
   double pn2 = 0.0;
+#pragma unroll
   for (int i = 1; i < 3; ++i)
+#pragma unroll
     for (int j = 1; j < 3; ++j) pn2 += cmul(cconj(c[i]), cmul(G[i][j], c[j])).x;
   const double pscale = pn2 > 0.0 ? 1.0 / sqrt(pn2) : 0.0;
   U[0] = c[0];
