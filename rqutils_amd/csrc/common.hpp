@@ -82,6 +82,8 @@ template <typename VT> struct Vec;
 template <> struct Vec<double2> {
   __device__ static inline double2 load(const double2* p, int64_t i) { return p[i]; }
   __device__ static inline void store(double2* p, int64_t i, double2 v) { p[i] = v; }
+  // the value as a store and a load would give it back
+  __device__ static inline double2 round(double2 v) { return v; }
 };
 template <> struct Vec<float2> {
   __device__ static inline double2 load(const float2* p, int64_t i) {
@@ -90,6 +92,9 @@ template <> struct Vec<float2> {
   }
   __device__ static inline void store(float2* p, int64_t i, double2 v) {
     p[i] = make_float2((float)v.x, (float)v.y);
+  }
+  __device__ static inline double2 round(double2 v) {
+    return make_double2((double)(float)v.x, (double)(float)v.y);
   }
 };
 

@@ -59,10 +59,17 @@ int gram_impl(int64_t n, int precision, const void* const* v, const void* const*
 //   P' = d1 W + d2 P, AP' = d1 AW + d2 AP, R' = AX' - theta' X',
 // and with diag given W' = the preconditioned R' (the statistics at
 // slots[stat_slot]; W may alias R when diag is NULL).
+// With gpart (kGramPartialDoubles doubles), the pass also leaves the next
+// iteration's Gram sums, except R.AR, as per-block partials there;
+// gram_rar_impl adds R.AR after the next product and reduces all 14.
 int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, void* W,
                       const void* AW, void* P, void* AP, const double* diag,
-                      const double* slots, int c_base, int stat_slot, hipStream_t s,
-                      const int32_t* skip = nullptr);
+                      const double* slots, int c_base, int stat_slot, double* gpart,
+                      hipStream_t s, const int32_t* skip = nullptr);
+int gram_rar_impl(int64_t n, int precision, const void* W, const void* AW, double* gpart,
+                  double* out, hipStream_t s, const int32_t* skip = nullptr);
+// Partial sums of the split Gram path: 28 doubles per block of a streaming pass.
+constexpr int64_t kGramPartialDoubles = (int64_t)28 * 8192;
 
 // W = the preconditioned R at theta = slots[theta_slot].x.
 int precond_impl(int64_t n, int precision, const void* R, const double* diag,

@@ -46,6 +46,7 @@ enum Flag : int { kConv = 0, kNiter = 1, kPValid = 2, kStatus = 3, kMatvecs = 4,
 constexpr size_t kSlotBytes = kNumSlots * sizeof(double2);
 constexpr size_t kFlagBytes = kNumFlags * sizeof(int32_t);
 constexpr size_t kPartialBytes = sizeof(double) * 2 * kMaxPairs * kDotBlocks;
+constexpr size_t kGramPartialBytes = sizeof(double) * (size_t)kGramPartialDoubles;
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -287,6 +288,7 @@ struct Solver {
   double2* S;
   int32_t* F;
   double* partials;
+  double* gpart;      // the split Gram path's partials (fused mode)
   char* vec[7];
   double* diag = nullptr;   // FUSED with the diagonal preconditioner
   void* mv_work = nullptr;
@@ -474,18 +476,29 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
   const void* res = sv.diag ? R : nullptr;
   int done = 0;
   int since_refresh = 0;
+  // The Gram sums come from a full pass over the six basis vectors on the
+  // first iteration and after X, AX, R or P were rebuilt; otherwise the
+  // previous update pass left all of them but R.AR as partials, and only W
+  // and A W are read after the product.
+  bool full_gram = true;
+  double* gram_out = reinterpret_cast<double*>(sv.S + kGram);
   while (true) {
     const int chunk = std::min(o.check_every, o.maxiter - done);
     for (int it = 0; it < chunk; ++it) {
       SQDK_TRY(sv.apply(W, AW, true));
-      SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, res, reinterpret_cast<double*>(sv.S + kGram),
-                         sv.partials, sv.s, sv.F + kConv));
+      if (full_gram)
+        SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, res, gram_out, sv.partials, sv.s,
+                           sv.F + kConv));
+      else
+        SQDK_TRY(gram_rar_impl(sv.n, sv.prec, W, AW, sv.gpart, gram_out, sv.s, sv.F + kConv));
       SQDK_TRY(sv.scalar(kStepFusedRR, true));
       SQDK_TRY(fused_update_impl(sv.n, sv.prec, X, AX, R, W, AW, P, AP, sv.diag,
-                                 reinterpret_cast<const double*>(sv.S), kUpd, kDiagStat, sv.s,
-                                 sv.F + kConv));
+                                 reinterpret_cast<const double*>(sv.S), kUpd, kDiagStat,
+                                 sv.gpart, sv.s, sv.F + kConv));
+      full_gram = false;
       if (o.refresh_every > 0 && ++since_refresh >= o.refresh_every) {
         since_refresh = 0;
+        full_gram = true;
         SQDK_TRY(sv.apply(X, AX, true));
         SQDK_TRY(sv.apply(P, AP, true));
         SQDK_TRY(sv.dots({{X, X}, {X, AX}}, true));
@@ -512,6 +525,7 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
       SQDK_TRY(sv.apply(P, AP, false));
       SQDK_TRY(precondition(false));
       since_refresh = 0;
+      full_gram = true;
     }
   }
   return SQDK_OK;
@@ -535,7 +549,8 @@ int sqdk_locg_workspace(const sqdk_op* op, int64_t n, const sqdk_locg_opts* opts
   size_t mv = 0;
   if (op && !op->fn && op->space) mv = align256(matvec_work_bytes(op->space, op->tune, opts->precision));
   const size_t dg = opts->precond ? align256((size_t)n * sizeof(double)) : 0;
-  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) + 7 * vb + mv + dg;
+  *bytes = align256(kSlotBytes) + align256(kFlagBytes) + align256(kPartialBytes) +
+           align256(kGramPartialBytes) + 7 * vb + mv + dg;
   return SQDK_OK;
 }
 
@@ -579,6 +594,8 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   w += align256(kFlagBytes);
   sv.partials = reinterpret_cast<double*>(w);
   w += align256(kPartialBytes);
+  sv.gpart = reinterpret_cast<double*>(w);
+  w += align256(kGramPartialBytes);
   const size_t vb = align256((size_t)n * elem_bytes(o.precision));
   for (int i = 0; i < 7; ++i) sv.vec[i] = w + i * vb;
   if (!op->fn) {

@@ -127,17 +127,26 @@ __device__ inline double precond_scale(double d, double theta, double2 st) {
 
 // W (the search direction in the basis) may be R itself (no preconditioner,
 // diag == NULL) or a separate vector that receives diag-preconditioned R'.
+// With gpart, the pass also sums, per block, the next iteration's Gram sums
+// over the values it stores: every sum of gram_partial_kernel except R.AR
+// (the next product's, summed by gram_rar_kernel), with X.AR taken as
+// AX.R (A Hermitian).
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
 fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, VT* W,
                     const VT* __restrict__ AW, VT* __restrict__ P, VT* __restrict__ AP,
                     const double* __restrict__ diag, const double2* __restrict__ c,
-                    const double2* __restrict__ dstat, const int32_t* __restrict__ skip) {
+                    const double2* __restrict__ dstat, double* __restrict__ gpart,
+                    const int32_t* __restrict__ skip) {
   if (skip && skip[0] != 0) return;
+  __shared__ double red[2 * kGramSums * (kBlock / kWave)];
   // c[0..2] = c0, c1, c2; c[3..4] = d1, d2; c[5].x = theta
   const double2 c0 = c[0], c1 = c[1], c2 = c[2], d1 = c[3], d2 = c[4];
   const double theta = c[5].x;
   const double2 st = diag ? *dstat : make_double2(0.0, 0.0);
+  double acc[2 * kGramSums];
+#pragma unroll
+  for (int k = 0; k < 2 * kGramSums; ++k) acc[k] = 0.0;
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     const double2 x = Vec<VT>::load(X, i), w = Vec<VT>::load(W, i), p = Vec<VT>::load(P, i);
@@ -147,12 +156,57 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
     const double2 pn = cadd(cmul(d1, w), cmul(d2, p));
     const double2 apn = cadd(cmul(d1, aw), cmul(d2, ap));
     const double2 rn = csub(axn, cscale(xn, theta));
+    const double2 wn = diag ? cscale(rn, precond_scale(diag[i], theta, st)) : rn;
     Vec<VT>::store(X, i, xn);
     Vec<VT>::store(AX, i, axn);
     Vec<VT>::store(P, i, pn);
     Vec<VT>::store(AP, i, apn);
     Vec<VT>::store(R, i, rn);
-    if (diag) Vec<VT>::store(W, i, cscale(rn, precond_scale(diag[i], theta, st)));
+    if (diag) Vec<VT>::store(W, i, wn);
+    if (gpart) {
+      const double2 xs = Vec<VT>::round(xn), ws = Vec<VT>::round(wn), ps = Vec<VT>::round(pn);
+      const double2 axs = Vec<VT>::round(axn), aps = Vec<VT>::round(apn);
+      const double2 rs = Vec<VT>::round(rn);
+      const double2 prods[kGramSums] = {
+          cmulc(xs, xs), cmulc(ws, ws), cmulc(ps, ps), cmulc(xs, ws), cmulc(xs, ps),
+          cmulc(ws, ps), cmulc(xs, axs), make_double2(0.0, 0.0), cmulc(ps, aps),
+          cmulc(axs, ws), cmulc(xs, aps), cmulc(ws, aps), cmulc(axs, axs), cmulc(rs, rs)};
+#pragma unroll
+      for (int k = 0; k < kGramSums; ++k) {
+        acc[2 * k] += prods[k].x;
+        acc[2 * k + 1] += prods[k].y;
+      }
+    }
+  }
+  if (!gpart) return;
+  block_sum<2 * kGramSums>(acc, red);
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 2 * kGramSums; ++k) {
+      if (k == 14 || k == 15) continue;   // R.AR: gram_rar_kernel's
+      gpart[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
+    }
+  }
+}
+
+// The one Gram sum that needs the new product: R.AR (W.AW) per block, into
+// the slots the fused update left free (same grid as that update).
+template <typename VT>
+__global__ void __launch_bounds__(kBlock)
+gram_rar_kernel(int64_t n, const VT* __restrict__ W, const VT* __restrict__ AW,
+                double* __restrict__ gpart, const int32_t* __restrict__ skip) {
+  if (skip && skip[0] != 0) return;
+  __shared__ double red[2 * (kBlock / kWave)];
+  double acc[2] = {0.0, 0.0};
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * kBlock) {
+    const double2 pr = cmulc(Vec<VT>::load(W, i), Vec<VT>::load(AW, i));
+    acc[0] += pr.x;
+    acc[1] += pr.y;
+  }
+  block_sum<2>(acc, red);
+  if (threadIdx.x == 0) {
+    gpart[(int64_t)14 * gridDim.x + blockIdx.x] = acc[0];
+    gpart[(int64_t)15 * gridDim.x + blockIdx.x] = acc[1];
   }
 }
 
@@ -299,21 +353,37 @@ int lincomb_impl(int64_t n, int precision, void* out, int k, const void* const* 
 
 int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, void* W,
                       const void* AW, void* P, void* AP, const double* diag,
-                      const double* slots, int c_base, int stat_slot, hipStream_t s,
-                      const int32_t* skip) {
+                      const double* slots, int c_base, int stat_slot, double* gpart,
+                      hipStream_t s, const int32_t* skip) {
   const unsigned g = stream_grid(n);
+  static_assert(kGramPartialDoubles >= (int64_t)2 * kGramSums * 8192, "split Gram partials");
   const double2* c = reinterpret_cast<const double2*>(slots) + c_base;
   const double2* st = reinterpret_cast<const double2*>(slots) + stat_slot;
   if (precision == SQDK_C128)
     fused_update_kernel<double2><<<g, kBlock, 0, s>>>(n, (double2*)X, (double2*)AX, (double2*)R,
                                                       (double2*)W, (const double2*)AW,
                                                       (double2*)P, (double2*)AP, diag, c, st,
-                                                      skip);
+                                                      gpart, skip);
   else
     fused_update_kernel<float2><<<g, kBlock, 0, s>>>(n, (float2*)X, (float2*)AX, (float2*)R,
                                                      (float2*)W, (const float2*)AW, (float2*)P,
-                                                     (float2*)AP, diag, c, st, skip);
+                                                     (float2*)AP, diag, c, st, gpart, skip);
   SQDK_LAUNCHED("fused_update_kernel");
+  return SQDK_OK;
+}
+
+int gram_rar_impl(int64_t n, int precision, const void* W, const void* AW, double* gpart,
+                  double* out, hipStream_t s, const int32_t* skip) {
+  const unsigned g = stream_grid(n);   // the fused update's grid: its partials share gpart
+  if (precision == SQDK_C128)
+    gram_rar_kernel<double2><<<g, kBlock, 0, s>>>(n, (const double2*)W, (const double2*)AW,
+                                                  gpart, skip);
+  else
+    gram_rar_kernel<float2><<<g, kBlock, 0, s>>>(n, (const float2*)W, (const float2*)AW, gpart,
+                                                 skip);
+  SQDK_LAUNCHED("gram_rar_kernel");
+  dots_final_kernel<<<2 * kGramSums, kBlock, 0, s>>>((int)g, gpart, out, skip);
+  SQDK_LAUNCHED("dots_final_kernel");
   return SQDK_OK;
 }
 
