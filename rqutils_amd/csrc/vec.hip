@@ -167,14 +167,19 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
       const double2 xs = Vec<VT>::round(xn), ws = Vec<VT>::round(wn), ps = Vec<VT>::round(pn);
       const double2 axs = Vec<VT>::round(axn), aps = Vec<VT>::round(apn);
       const double2 rs = Vec<VT>::round(rn);
+      // squared norms (sums 0, 1, 2, 12, 13) are real: only their real parts
+      auto nrm = [](double2 a) { return make_double2(a.x * a.x + a.y * a.y, 0.0); };
       const double2 prods[kGramSums] = {
-          cmulc(xs, xs), cmulc(ws, ws), cmulc(ps, ps), cmulc(xs, ws), cmulc(xs, ps),
+          nrm(xs), nrm(ws), nrm(ps), cmulc(xs, ws), cmulc(xs, ps),
           cmulc(ws, ps), cmulc(xs, axs), make_double2(0.0, 0.0), cmulc(ps, aps),
-          cmulc(axs, ws), cmulc(xs, aps), cmulc(ws, aps), cmulc(axs, axs), cmulc(rs, rs)};
+          cmulc(axs, ws), cmulc(xs, aps), cmulc(ws, aps), nrm(axs), nrm(rs)};
+      constexpr unsigned kReal = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 7) | (1u << 12) |
+                                 (1u << 13);
 #pragma unroll
       for (int k = 0; k < kGramSums; ++k) {
+        if (k == 7) continue;
         acc[2 * k] += prods[k].x;
-        acc[2 * k + 1] += prods[k].y;
+        if (!((kReal >> k) & 1u)) acc[2 * k + 1] += prods[k].y;
       }
     }
   }
