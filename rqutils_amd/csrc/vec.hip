@@ -36,7 +36,11 @@ dots_partial_kernel(int64_t n, int npairs, PairList pl, double* __restrict__ par
   }
   block_sum<2 * kMaxPairs>(acc, red);
   if (threadIdx.x == 0) {
-    for (int k = 0; k < 2 * npairs; ++k) partials[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
+    // a constant bound: indexed by the runtime pair count, acc would live in
+    // scratch memory, and every accumulation above with it
+#pragma unroll
+    for (int k = 0; k < 2 * kMaxPairs; ++k)
+      if (k < 2 * npairs) partials[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
   }
 }
 
