@@ -378,7 +378,13 @@ typedef struct sqdk_locg_result {
   int32_t matvecs;
   int32_t status;          /* 0; 2: Rayleigh-Ritz breakdown (Cholesky pivot below the
                               floor with P already dropped); 3: NaN or Inf in the
-                              Gram sums (non-finite operator or vectors) */
+                              sums, norms or small eigenproblems (non-finite operator
+                              or vectors; converged is then 0); 4: the start is an
+                              exact eigenvector and n > 1 (its residual is 0: an
+                              invariant subspace such as a one-hot on a state with
+                              no off-diagonal neighbour, so eigval need not be the
+                              lowest eigenvalue; the reference divides 0 by 0 here,
+                              ground_locg.py:274-286, 424-435) */
 } sqdk_locg_result;
 
 /* The operator: the built-in Pauli-sum matvec (ham + space + tune) when fn is

@@ -16,7 +16,7 @@ import numpy as np
 __all__ = [
     "from_paulisum", "pack_states", "uniquify_states", "get_xsource",
     "xsource_by_lookup", "get_diagonal", "apply_xgrp", "apply_h_x_cached",
-    "apply_h", "vinit_from_min_diag", "hproj_coo", "packed_to_keys",
+    "apply_h", "apply_h_rows", "vinit_from_min_diag", "hproj_coo", "packed_to_keys",
     "PHASES",
 ]
 
@@ -172,6 +172,34 @@ def apply_h(vec, xsigs, zsigs, coeffs, states):
     """``Hv`` recomputing the source indices per call -- rqutils/sqd.py:590-605."""
     xs = np.stack([get_xsource(xsigs[g], states) for g in range(xsigs.shape[0])])
     return apply_h_x_cached(vec, xs, zsigs, coeffs, states)
+
+
+def apply_h_rows(rows, vec, xsigs, zsigs, coeffs, states):
+    """Rows ``rows`` of ``apply_h_x_cached`` (rqutils/sqd.py:641-654), restated
+    row by row for subspaces too large for the J x N source cache.
+
+    For each group in stored order, the source of row i is the position of
+    ``S[i] ^ x`` among the sorted packed rows ``S`` -- the same index the
+    sort-join of ``get_xsource`` (sqd.py:476-500) leaves, found here by a
+    binary search over the rows compared as byte strings (lexicographic, as
+    the reference's multi-key sort orders them) -- or -1; its coefficient is
+    ``get_diagonal`` (sqd.py:562-575) on that row.
+    """
+    states = np.ascontiguousarray(states, dtype=np.uint8)
+    nb = states.shape[1]
+    key = states.view(f"V{nb}").ravel()
+    rows = np.asarray(rows, dtype=np.int64)
+    srows = states[rows]
+    n = states.shape[0]
+    out = np.zeros(rows.size, dtype=np.result_type(vec.dtype, coeffs.dtype))
+    for g in range(xsigs.shape[0]):
+        t = np.ascontiguousarray(srows ^ xsigs[g])
+        tv = t.view(f"V{nb}").ravel()
+        j = np.searchsorted(key, tv)
+        jc = np.minimum(j, n - 1)
+        src = np.where((j < n) & (key[jc] == tv), j, -1).astype(np.int32)
+        out = out + apply_xgrp(src, get_diagonal(zsigs[g], coeffs[g], srows), vec)
+    return out
 
 
 def vinit_from_min_diag(xsigs, zsigs, coeffs, states_u, n_unique):

@@ -12,7 +12,7 @@ work while the others continue.
 from __future__ import annotations
 
 import ctypes as C
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 
 import numpy as np
 import torch
@@ -33,6 +33,7 @@ class BatchSolve:
     vectors: torch.Tensor          # concatenated eigenvectors (solver precision)
     infos: list                    # LocgInfo per instance
     seconds: float = 0.0
+    judged: list = field(default_factory=list)   # instances the Gaussian judge solve won
 
 
 class SQDBatch:
@@ -267,10 +268,53 @@ class SQDBatch:
             _lib.stream_ptr(self.device)), "sqdk_batch_matvec")
         return out
 
-    def solve(self, x0: torch.Tensor, *, precision: str = "c64", criterion: str = "relative",
-              tol: float = 1e-6, maxiter: int = 1000, refresh_every: int = 16,
-              use_graph: bool = True) -> BatchSolve:
-        """Fused LOBPCG on every instance from the concatenated start vectors x0."""
+    def start_vector(self, start="gaussian", *, mix: float | None = None,
+                     seed: int = 0) -> torch.Tensor:
+        """Concatenated start vectors: each instance's ``sqd.start_vector``
+        (``"gaussian"``, ``"min_diag"`` or a ``StartNet``) with seed ``seed +
+        i``. A network is applied to the instances of its qubit count; the
+        others start from the Gaussian, the method without the network."""
+        from .sqd import start_vector
+        from .startnet import StartNet
+        parts = []
+        for i, op in enumerate(self.ops):
+            st = start
+            if isinstance(start, StartNet) and start.nq != op.space.num_qubits:
+                st = "gaussian"
+            parts.append(start_vector(op, st, mix=mix, seed=seed + i))
+        return torch.cat(parts)
+
+    def solve(self, x0: torch.Tensor | None = None, *, precision: str = "c64",
+              criterion: str = "relative", tol: float = 1e-6, maxiter: int = 1000,
+              refresh_every: int = 16, use_graph: bool = True, start=None,
+              mix: float | None = None, seed: int = 0, judge: bool = False) -> BatchSolve:
+        """Fused LOBPCG on every instance from the concatenated start vectors
+        ``x0``, or from ``start_vector(start, mix=mix, seed=seed)``.
+
+        ``judge``: solve the batch once more from Gaussian starts and keep, per
+        instance, the lower of the two energies (both are Rayleigh quotients,
+        upper bounds of the ground energy): the plain method judges the chosen
+        start, and the result is at least as good as the plain method's.
+        ``BatchSolve.judged`` lists the instances the Gaussian solve won."""
+        if x0 is None:
+            x0 = self.start_vector("gaussian" if start is None else start, mix=mix, seed=seed)
+        elif start is not None:
+            raise ValueError("give either x0 or start")
+        res = self._solve(x0, precision, criterion, tol, maxiter, refresh_every, use_graph)
+        if judge:
+            t_first = res.seconds
+            ref = self._solve(self.start_vector("gaussian", seed=seed), precision, criterion,
+                              tol, maxiter, refresh_every, use_graph)
+            for i, (a, b) in enumerate(zip(res.infos, ref.infos)):
+                if b.converged and (not a.converged or b.eigval < a.eigval):
+                    res.infos[i] = b
+                    self.segment(res.vectors, i).copy_(self.segment(ref.vectors, i))
+                    res.judged.append(i)
+            res.seconds = t_first + ref.seconds
+        return res
+
+    def _solve(self, x0, precision, criterion, tol, maxiter, refresh_every,
+               use_graph) -> BatchSolve:
         L = _lib.lib()
         prec = _PRECISIONS[precision]
         dtype = torch.complex128 if prec == _lib.C128 else torch.complex64

@@ -165,6 +165,9 @@ __global__ void bscalar_kernel(int step, int n_inst, const int64_t* __restrict__
       s.thr = thr;
       s.matvecs += 1;  // the A R of this iteration ran
       if (ok) {
+        // a zero residual before the first update: the start spans an
+        // invariant subspace of a multi-state instance (status 4)
+        if (rn == 0.0 && s.niter == 0 && row_base[i + 1] - row_base[i] > 1) s.status = 4;
         conv[i] = 1;
         atomicAdd(&counters[0], 1);
         break;
@@ -593,7 +596,7 @@ int sqdk_batch_locg_solve(const sqdk_batch* batch, const sqdk_tune* tune, void* 
     results[i].resid_norm = host[i].rn;
     results[i].ax_norm = host[i].axn;
     results[i].niter = host[i].niter;
-    results[i].converged = conv[i];
+    results[i].converged = conv[i] && host[i].status != 3;
     results[i].matvecs = host[i].matvecs + 2;  // + the verification products
     results[i].status = host[i].status;
   }

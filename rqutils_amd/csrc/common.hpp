@@ -75,6 +75,7 @@ __host__ __device__ inline double2 cmulc(double2 a, double2 b) {
 __host__ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __host__ __device__ inline double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 __host__ __device__ inline double cabs2(double2 a) { return a.x * a.x + a.y * a.y; }
+__host__ __device__ inline bool czero(double2 a) { return a.x == 0.0 && a.y == 0.0; }
 
 // Storage traits: vectors live in double2 (complex128) or float2 (complex64);
 // loads widen to double2, stores round once.

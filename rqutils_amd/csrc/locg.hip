@@ -53,9 +53,27 @@ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 struct ScalarParams {
   double tol;
   double nlen;     // vector length in the reference criterion
+  int64_t n;       // vector length of the solve
   int criterion;
   int precision;
 };
+
+// Status flags (sqdk_locg_result.status).
+constexpr int kStatusBreakdown = 2;    // Rayleigh-Ritz breakdown (fused)
+constexpr int kStatusNonFinite = 3;    // NaN or Inf in the sums or the small eigenproblem
+constexpr int kStatusInvariant = 4;    // exact eigenvector start with n > 1
+
+__device__ inline bool finite_all(const double2* D, int k) {
+  bool ok = true;
+  for (int i = 0; i < k; ++i) ok = ok && isfinite(D[i].x) && isfinite(D[i].y);
+  return ok;
+}
+
+// Stop the solve: the remaining kernels of the iteration see the flag.
+__device__ inline void stop_nonfinite(int32_t* F) {
+  F[kStatus] = kStatusNonFinite;
+  F[kConv] = 1;
+}
 
 enum Step : int {
   kStepInitNorm = 0,   // C0 = 1 / sqrt(D0)
@@ -110,10 +128,16 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       // subspace): the 2x2 Rayleigh-Ritz of the reference (ground_locg.py:
       // 274-286, 424-435) would divide 0 by 0 here, so the solve stops at
       // theta = rho with the normalized start.
+      // With more than one state this is an invariant subspace of the
+      // start (a one-hot on a state without off-diagonal neighbours), and
+      // theta need not be the lowest eigenvalue: status 4 says so.
       if (D[0].x == 0.0) {
         F[kConv] = 1;
+        if (prm.n > 1) F[kStatus] = kStatusInvariant;
         S[kInfo] = make_double2(0.0, fabs(S[kTheta].x));
         S[kInfo2] = make_double2(0.0, 0.0);
+      } else if (!isfinite(D[0].x) || !isfinite(S[kTheta].x)) {
+        stop_nonfinite(F);
       }
       break;
     case kStepInvNormGuard: {
@@ -128,6 +152,10 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       m[0][1] = cconj(D[0]);
       m[0][0] = D[1];
       m[1][1] = D[2];
+      if (!finite_all(D, 3)) {
+        stop_nonfinite(F);
+        break;
+      }
       double lam;
       double2 k[2];
       eig2_reference(m, lam, k);
@@ -143,7 +171,8 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
     }
     case kStepInvNorm2:
       C[0] = make_double2(1.0 / sqrt(D[0].x), 0.0);
-      C[1] = make_double2(1.0 / sqrt(D[1].x), 0.0);
+      // t = 0 (x already optimal in span(x, y, p)): y' = 0 instead of 0/0
+      C[1] = make_double2(D[1].x > 0.0 ? 1.0 / sqrt(D[1].x) : 0.0, 0.0);
       break;
     case kStepNegTheta:
       C[0] = make_double2(-S[kTheta].x, 0.0);
@@ -167,9 +196,35 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       m[0][0] = D[3];
       m[1][1] = D[4];
       m[2][2] = D[5];
+      if (!finite_all(D, 6)) {
+        stop_nonfinite(F);
+        break;
+      }
+      // A zero basis vector (p zeroed by the 0.99 rule, y = 0 after an update
+      // that left x unchanged) has an all-zero row and column, which would add
+      // a spurious eigenvalue 0 to the 3x3 problem: the reference then returns
+      // NaN or, for an operator with a positive spectrum, theta = 0
+      // (ground_locg.py:300-305, 421, 438-471). Rayleigh-Ritz runs on the
+      // nonzero vectors instead.
+      const bool yzero = czero(D[0]) && czero(D[2]) && czero(D[4]);
+      const bool pzero = czero(D[1]) && czero(D[2]) && czero(D[5]);
       double lam;
       double2 k[3];
-      eig3_reference(m, lam, k);
+      if (!yzero && !pzero) {
+        eig3_reference(m, lam, k);
+      } else if (yzero && pzero) {
+        lam = D[3].x;
+        k[0] = make_double2(1.0, 0.0);
+        k[1] = k[2] = make_double2(0.0, 0.0);
+      } else {
+        const int j = pzero ? 1 : 2;
+        const double2 m2[2][2] = {{m[0][0], m[0][j]}, {m[j][0], m[j][j]}};
+        double2 k2[2];
+        eig2_reference(m2, lam, k2);
+        k[0] = k2[0];
+        k[j] = k2[1];
+        k[3 - j] = make_double2(0.0, 0.0);
+      }
       S[kTheta] = make_double2(lam, 0.0);
       for (int i = 0; i < 3; ++i) S[kKappa + i] = k[i];
       C[0] = k[1];
@@ -179,7 +234,8 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
     case kStepTU: {
       const double ns = sqrt(D[0].x);
       const double2 k0 = S[kKappa];
-      C[0] = cscale(k0, 1.0 / ns);
+      // s = 0: t = 0 (the reference divides by ns = 0 here)
+      C[0] = ns > 0.0 ? cscale(k0, 1.0 / ns) : make_double2(0.0, 0.0);
       C[1] = make_double2(-ns, 0.0);
       C[2] = k0;
       C[3] = make_double2(1.0, 0.0);
@@ -190,6 +246,9 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S);
       F[kNiter] += 1;
       if (conv) F[kConv] = 1;
+      // The reference iterates on NaN until maxiter; a non-finite residual,
+      // norm or theta ends the solve with status 3 instead.
+      if (!isfinite(axn) || !isfinite(rn) || !isfinite(S[kTheta].x)) stop_nonfinite(F);
       break;
     }
     case kStepFusedRR: {
@@ -200,8 +259,7 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       bool finite = isfinite(theta);
       for (int i = 0; i < 14; ++i) finite = finite && isfinite(g[i].x) && isfinite(g[i].y);
       if (!finite) {
-        F[kStatus] = 3;
-        F[kConv] = 1;
+        stop_nonfinite(F);
         break;
       }
       const double rn = sqrt(fmax(g[13].x, 0.0));   // |R| (the residual itself)
@@ -213,7 +271,7 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       double2 U[6];
       bool pv = false;
       if (fused_rr(g, F[kPValid] != 0, prm.precision, U, pv) != 0) {
-        F[kStatus] = 2;
+        F[kStatus] = kStatusBreakdown;
         F[kConv] = 1;
         break;
       }
@@ -612,6 +670,7 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   }
   sv.prm.tol = o.tol;
   sv.prm.nlen = (double)(o.n_for_tol > 0 ? o.n_for_tol : n);
+  sv.prm.n = n;
   sv.prm.criterion = o.criterion;
   sv.prm.precision = o.precision;
 
@@ -631,7 +690,8 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   result->resid_norm = slots[kInfo].x;
   result->ax_norm = slots[kInfo].y;
   result->niter = flags[kNiter];
-  result->converged = flags[kConv];
+  // a stop on NaN or Inf sets the flag that ends the loop but is no convergence
+  result->converged = flags[kConv] && flags[kStatus] != kStatusNonFinite;
   result->matvecs = sv.matvecs + flags[kMatvecs];
   result->status = flags[kStatus];
   return SQDK_OK;

@@ -10,26 +10,57 @@ namespace sqdk {
 
 // ---- small closed forms, as ground_locg.py computes them -------------------
 
-// Lowest eigenpair of a 2x2 Hermitian matrix (ground_locg.py:424-435).
+template <int N>
+__device__ inline void jacobi_lowest(double2 a[3][3], double& lam, double2 vec[3]);
+
+// Lowest eigenpair of a 2x2 Hermitian matrix (ground_locg.py:424-435). Where
+// the closed form divides by zero (d0 - lam + m10 == 0, e.g. a diagonal
+// matrix whose lower entry is the smaller) the reference returns NaN; here the
+// pair then comes from the Jacobi rotation instead.
 __device__ inline void eig2_reference(const double2 m[2][2], double& lam, double2 vec[2]) {
   const double d0 = m[0][0].x, d1 = m[1][1].x;
   const double det = d0 * d1 - cabs2(m[1][0]);
   const double tr = d0 + d1;
-  lam = (tr - sqrt(tr * tr - 4.0 * det)) * 0.5;
+  lam = (tr - sqrt(fmax(tr * tr - 4.0 * det, 0.0))) * 0.5;
   const double2 num = make_double2(d1 - lam + m[1][0].x, -m[1][0].y);
   const double2 den = make_double2(d0 - lam + m[1][0].x, m[1][0].y);
   const double dd = cabs2(den);
   const double2 e0 = make_double2((num.x * den.x + num.y * den.y) / dd,
                                   (num.y * den.x - num.x * den.y) / dd);
   const double nrm = sqrt(e0.x * e0.x + e0.y * e0.y + 1.0);
-  vec[0] = cscale(e0, 1.0 / nrm);
-  vec[1] = make_double2(-1.0 / nrm, 0.0);
+  if (dd > 0.0 && isfinite(nrm)) {
+    vec[0] = cscale(e0, 1.0 / nrm);
+    vec[1] = make_double2(-1.0 / nrm, 0.0);
+    return;
+  }
+  double2 a[3][3] = {{m[0][0], m[0][1], make_double2(0.0, 0.0)},
+                     {m[1][0], m[1][1], make_double2(0.0, 0.0)},
+                     {make_double2(0.0, 0.0), make_double2(0.0, 0.0), make_double2(0.0, 0.0)}};
+  double2 v3[3];
+  jacobi_lowest<2>(a, lam, v3);
+  vec[0] = v3[0];
+  vec[1] = v3[1];
+}
+
+__device__ inline void cross3(const double2 a[3], const double2 b[3], double2 e[3]) {
+  e[0] = csub(cmul(a[1], b[2]), cmul(a[2], b[1]));
+  e[1] = csub(cmul(a[2], b[0]), cmul(a[0], b[2]));
+  e[2] = csub(cmul(a[0], b[1]), cmul(a[1], b[0]));
 }
 
 // Lowest eigenpair of a 3x3 Hermitian matrix by Cardano's method
 // (J. Kopp, Int. J. Mod. Phys. C 19, 523 (2008); ground_locg.py:438-471):
 // characteristic coefficients, the trigonometric root, and the eigenvector as
 // the conjugated cross product of columns 1 and 2 of (A - lambda I).
+//
+// Guards where the reference's formulas give NaN or a wrong vector: rounding
+// can make the two square-root arguments slightly negative (clamped at 0), and
+// columns 1 and 2 of A - lambda I can be parallel, zero or pure rounding noise
+// (a diagonal matrix, a double eigenvalue), so their cross product vanishes or
+// points anywhere. The eigenvector is then the conjugated cross product of the
+// strongest column pair (each row of the singular Hermitian matrix
+// A - lambda I is orthogonal to it), and when A - lambda I has rank one or
+// zero, the lowest Jacobi eigenvector.
 __device__ inline void eig3_reference(const double2 m[3][3], double& lam, double2 vec[3]) {
   const double d0 = m[0][0].x, d1 = m[1][1].x, d2 = m[2][2].x;
   const double o10 = cabs2(m[1][0]), o20 = cabs2(m[2][0]), o21 = cabs2(m[2][1]);
@@ -38,22 +69,53 @@ __device__ inline void eig3_reference(const double2 m[3][3], double& lam, double
   double c0 = d0 * o21 + d1 * o20 + d2 * o10;
   c0 -= d0 * d1 * d2;
   c0 -= 2.0 * cmul(cmul(m[0][2], m[1][0]), m[2][1]).x;
-  const double p = c2 * c2 - 3.0 * c1;
+  const double p = fmax(c2 * c2 - 3.0 * c1, 0.0);
   const double q = -13.5 * c0 - c2 * c2 * c2 + 4.5 * c2 * c1;
   const double phi =
-      atan2(sqrt(27.0 * (0.25 * c1 * c1 * (p - c1) + c0 * (q + 6.75 * c0))), q) / 3.0;
+      atan2(sqrt(fmax(27.0 * (0.25 * c1 * c1 * (p - c1) + c0 * (q + 6.75 * c0)), 0.0)), q) / 3.0;
   const double cphi = cos(phi), sphi = sin(phi);
   const double s3 = sqrt(3.0);
   const double xmin = fmin(fmin(2.0 * cphi, -cphi - s3 * sphi), -cphi + s3 * sphi);
   lam = sqrt(p) / 3.0 * xmin - c2 / 3.0;
-  double2 v0[3] = {m[0][1], m[1][1], m[2][1]};
-  double2 v1[3] = {m[0][2], m[1][2], m[2][2]};
-  v0[1].x -= lam;
-  v1[2].x -= lam;
-  double2 e[3] = {csub(cmul(v0[1], v1[2]), cmul(v0[2], v1[1])),
-                  csub(cmul(v0[2], v1[0]), cmul(v0[0], v1[2])),
-                  csub(cmul(v0[0], v1[1]), cmul(v0[1], v1[0]))};
-  const double nrm = sqrt(cabs2(e[0]) + cabs2(e[1]) + cabs2(e[2]));
+  // columns of A - lambda I
+  double2 col[3][3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) col[j][i] = m[i][j];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) col[j][j].x -= lam;
+  double fro = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) fro += cabs2(col[j][0]) + cabs2(col[j][1]) + cabs2(col[j][2]);
+  double2 e[3], e01[3], e02[3];
+  cross3(col[1], col[2], e);
+  cross3(col[0], col[1], e01);
+  cross3(col[0], col[2], e02);
+  double en = cabs2(e[0]) + cabs2(e[1]) + cabs2(e[2]);
+  const double n01 = cabs2(e01[0]) + cabs2(e01[1]) + cabs2(e01[2]);
+  const double n02 = cabs2(e02[0]) + cabs2(e02[1]) + cabs2(e02[2]);
+  const double best = fmax(en, fmax(n01, n02));
+  // |cross| below 1e-10 |A - lambda I|_F^2 for every pair: rank one or zero
+  if (!(isfinite(best) && isfinite(fro) && best > 1e-20 * fro * fro)) {
+    double2 a[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) a[i][j] = m[i][j];
+    jacobi_lowest<3>(a, lam, vec);
+    return;
+  }
+  // The reference's pair unless it is far weaker than the best one (then it
+  // is mostly rounding noise: for diag(3, -2, 1) column 1 is 4e-16 and the
+  // reference's vector points along e0 instead of e1).
+  if (!(en >= 1e-6 * best)) {
+    const bool use01 = n01 >= n02;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) e[i] = use01 ? e01[i] : e02[i];
+    en = use01 ? n01 : n02;
+  }
+  const double nrm = sqrt(en);
   for (int i = 0; i < 3; ++i) vec[i] = cscale(cconj(e[i]), 1.0 / nrm);
 }
 

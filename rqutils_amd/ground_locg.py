@@ -38,7 +38,8 @@ __all__ = ["ground_locg", "solve", "LocgInfo", "eigenpair_2x2", "eigenpair_3x3"]
 LOG = logging.getLogger(__name__)
 
 STATUS_BREAKDOWN = 2      # LocgInfo.status: Rayleigh-Ritz breakdown (Cholesky pivot floor)
-STATUS_NONFINITE = 3      # LocgInfo.status: NaN or Inf in the Gram sums
+STATUS_NONFINITE = 3      # LocgInfo.status: NaN or Inf in the sums or small eigenproblems
+STATUS_INVARIANT = 4      # LocgInfo.status: exact eigenvector start with more than one state
 
 _MODES = {"reference": _lib.LOCG_REFERENCE, "fused": _lib.LOCG_FUSED}
 _CRITERIA = {"reference": _lib.CRIT_REFERENCE, "relative": _lib.CRIT_RELATIVE}
@@ -184,6 +185,10 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
                     info.niter)
     elif info.status == STATUS_BREAKDOWN:
         LOG.warning("LOBPCG stopped at iteration %d: Rayleigh-Ritz breakdown", info.niter)
+    elif info.status == STATUS_INVARIANT:
+        LOG.warning("LOBPCG start is an exact eigenvector (zero residual) of a %d-state "
+                    "operator: theta = %.12g is an eigenvalue but not necessarily the lowest",
+                    n, info.eigval)
     return x, info
 
 
@@ -198,7 +203,10 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
         mat: Matrix :math:`A` (dense array/tensor), a callable
             :math:`x \mapsto Ax` on device tensors, or a ``PauliOperator``.
         xinit: Initial vector, or an integer for a one-hot start (then
-            ``vspace=(dim, dtype)`` is required for a callable ``mat``).
+            ``vspace=(dim, dtype)`` is required for a callable ``mat``). With a
+            ``PauliOperator`` also ``"min_diag"``, ``"gaussian"`` or a trained
+            ``StartNet`` (``sqd.start_vector``: a network start carries a
+            Gaussian part of norm ``startnet.DEFAULT_MIX``).
         args: Extra arguments passed to a callable ``mat``.
         maxiter: Maximum number of loop iterations.
         tol: Convergence tolerance; defaults to the machine epsilon of the
@@ -238,7 +246,15 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
         base_dtype = dense.dtype
 
     # Start vector.
-    if isinstance(xinit, (int, np.integer)) or (isinstance(xinit, torch.Tensor) and
+    from .startnet import StartNet
+    if isinstance(xinit, (str, StartNet)):
+        if not isinstance(mat, PauliOperator):
+            raise ValueError("a named or network start needs a PauliOperator")
+        from .sqd import start_vector
+        x0 = start_vector(mat, xinit)
+        if not mat.is_complex:
+            x0 = x0.real.contiguous() if xinit == "min_diag" else x0
+    elif isinstance(xinit, (int, np.integer)) or (isinstance(xinit, torch.Tensor) and
                                                  xinit.ndim == 0 and not xinit.is_floating_point()
                                                  and not xinit.is_complex()):
         idx = int(xinit)

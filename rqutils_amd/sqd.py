@@ -25,12 +25,13 @@ import torch
 from scipy.sparse import coo_array, csr_array
 
 from . import _lib
-from .ground_locg import solve
+from .ground_locg import STATUS_INVARIANT, solve
 from .intake import uniquify_states, unpack_keys
 from .operator import PauliOperator, Subspace, keys_to_states, states_to_keys
 from .paulis.symplectic import PauliSumXZ, _bits_to_u64, unpack_rows
 
-__all__ = ["sqd", "hproj", "get_xsource", "get_diagonal", "SQDProblem"]
+__all__ = ["sqd", "hproj", "get_xsource", "get_diagonal", "SQDProblem", "start_vector",
+           "vinit_index"]
 
 LOG = logging.getLogger(__name__)
 
@@ -83,19 +84,75 @@ class SQDProblem:
     def matvec(self, v: torch.Tensor, out=None) -> torch.Tensor:
         return self.op.matvec(v, out)
 
+    def vinit_index(self) -> int:
+        """Row of the reference's default one-hot start (rqutils/sqd.py:373-398):
+        the first smallest real X = 0 diagonal entry when the Hamiltonian has
+        an X = 0 group, else row 0. Fill rows are never stored here, so the
+        reference's masking of them (sqd.py:381-382) has nothing to do."""
+        return vinit_index(self.op)
+
     def vinit_min_diag(self, dtype=torch.complex128) -> torch.Tensor:
         """One-hot start at the smallest X=0 diagonal entry, else at row 0
         (rqutils/sqd.py:373-398)."""
         v = torch.zeros(self.n, dtype=dtype, device=self.space.device)
-        if np.all(self.hamiltonian.x[0] == 0):
-            diag = self.op.diagonal_of_group0().real
-            v[int(torch.argmin(diag).item())] = 1
-        else:
-            v[0] = 1
+        v[self.vinit_index()] = 1
         return v
+
+    def start_vector(self, start="min_diag", *, mix: Optional[float] = None, seed: int = 0,
+                     dtype=torch.complex128) -> torch.Tensor:
+        """Start vector of a solve over this subspace (``start_vector`` below)."""
+        return start_vector(self.op, start, mix=mix, seed=seed).to(dtype)
 
     def solve(self, x0: torch.Tensor, **kw):
         return solve(self.op, x0, **kw)
+
+
+def vinit_index(op: PauliOperator) -> int:
+    """Row of the reference's one-hot start over ``op``'s subspace
+    (rqutils/sqd.py:373-398): the first smallest real X = 0 diagonal entry
+    (``torch.argmin`` returns the first of equal entries, as ``jnp.argmin``
+    does), or row 0 when the Hamiltonian has no X = 0 group."""
+    if np.all(op.ham.x[0] == 0):
+        return int(torch.argmin(op.diagonal_of_group0().real).item())
+    return 0
+
+
+def start_vector(op: PauliOperator, start="min_diag", *, mix: Optional[float] = None,
+                 seed: int = 0) -> torch.Tensor:
+    """Normalised complex128 start vector of a solve over ``op``'s subspace.
+
+    ``start``: ``"min_diag"`` (the reference's one-hot, sqd.py:373-398),
+    ``"gaussian"`` (seeded complex Gaussian), a trained
+    :class:`~rqutils_amd.startnet.StartNet` (its predicted amplitudes and
+    signs over the subspace's keys, with the per-state features it was
+    trained on), or a vector over the subspace. ``mix``: norm of a seeded
+    Gaussian part added to a one-hot, network or given start (the rest keeps
+    norm sqrt(1 - mix^2)); default 0, and ``startnet.DEFAULT_MIX`` for a
+    network. A start without a Gaussian part can lie (nearly) in an invariant
+    subspace that misses the ground state; the Gaussian part gives the ground
+    vector a component of order mix / sqrt(n), which LOBPCG amplifies instead
+    of stopping at an excited state.
+    """
+    from types import SimpleNamespace
+    from .startnet import DEFAULT_MIX, StartNet, gaussian_start, mix_gaussian
+    space = op.space
+    if isinstance(start, StartNet):
+        prob = SimpleNamespace(space=space, op=op)
+        return start.start_for(prob, mix=DEFAULT_MIX if mix is None else mix, seed=seed)
+    if isinstance(start, str):
+        if start == "gaussian":
+            g = gaussian_start(space.n, space.device, seed)
+            return g / torch.linalg.vector_norm(g)
+        if start != "min_diag":
+            raise ValueError(f"unknown start {start!r}")
+        x = torch.zeros(space.n, dtype=torch.complex128, device=space.device)
+        x[vinit_index(op)] = 1
+    else:
+        x = start if isinstance(start, torch.Tensor) else torch.as_tensor(np.asarray(start))
+        x = x.reshape(-1).to(space.device, torch.complex128)
+        if x.numel() != space.n:
+            raise ValueError(f"start vector of length {x.numel()}, subspace has {space.n}")
+    return mix_gaussian(x, mix or 0.0, seed)
 
 
 def _check_cache_level(cache_level):
@@ -105,7 +162,10 @@ def _check_cache_level(cache_level):
 
 def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: bool = True,
         cache_level: tuple[int, int] = (1, 0), *, device=None, mode: str = "reference",
-        precision: str = "c128", maxiter: int = 1000, tol: Optional[float] = None):
+        precision: str = "c128", maxiter: int = 1000, tol: Optional[float] = None,
+        start="min_diag", mix: Optional[float] = None, seed: int = 0,
+        restart: Optional[str] = "gaussian", judge: bool = False,
+        precond: Optional[str] = None, info: Optional[dict] = None):
     r"""Sample-based quantum diagonalization (rqutils/sqd.py:181-248).
 
     Args:
@@ -125,6 +185,25 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
             Every level gives the same operator.
         mode / precision / maxiter / tol: solver controls (defaults reproduce
             the reference: the reference iteration in FP64 with ``tol`` = eps).
+        start / mix / seed: the start vector (``SQDProblem.start_vector``):
+            ``"min_diag"`` (the reference's one-hot, the default),
+            ``"gaussian"``, a trained ``StartNet`` or a vector over the sorted
+            distinct states.
+        restart: When the start is an exact eigenvector of a multi-state
+            subspace (its residual is exactly zero: a one-hot on a state with
+            no off-diagonal neighbour in the subspace), LOBPCG cannot leave it
+            and its energy need not be the ground energy. The reference then
+            returns NaN (0/0 in its 2x2 step, ground_locg.py:274-286, 424-435).
+            ``"gaussian"`` (default) solves again from a seeded Gaussian
+            start; ``None`` returns NaN as the reference does. Either way a
+            warning is logged.
+        judge: Also solve from a seeded Gaussian start and keep the lower of
+            the two energies (each is a Rayleigh quotient, an upper bound of
+            the ground energy); the plain method judges the chosen start.
+        precond: ``"diag"`` for the Jacobi-preconditioned fused iteration
+            (``mode="fused"``).
+        info: Optional dict that receives the solve's diagnostics
+            (``LocgInfo`` under "locg", and "restarted" / "judged" flags).
 
     Returns:
         ``eigval``, or ``(eigval, eigvec, basis_states)`` with ``eigvec`` over
@@ -141,18 +220,39 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
     num_qubits = states.shape[1]
     # pack + sort + unique on the device (the distinct rows of uniquify_states)
     keys = uniquify_states(states, device)
-    start = time.time()
+    t_start = time.time()
     prob = SQDProblem(hamiltonian, keys, num_qubits, device=device,
                       cache="auto" if cache_level[0] == 1 else False)
     real = not prob.op.is_complex
     dtype = torch.complex128 if precision == "c128" else torch.complex64
-    x0 = prob.vinit_min_diag(dtype)
+    if restart not in ("gaussian", None):
+        raise ValueError("restart must be 'gaussian' or None")
+    x0 = prob.start_vector(start, mix=mix, seed=seed, dtype=dtype)
     if tol is None:
         tol = float(np.finfo(np.float64 if precision == "c128" else np.float32).eps)
-    vec, info = prob.solve(x0, mode=mode, precision=precision, criterion="reference", tol=tol,
-                           maxiter=maxiter, n_for_tol=states_size)
-    LOG.info("Found ground eigenpair in %f seconds.", time.time() - start)
-    eigval = float(info.eigval)
+    kw = dict(mode=mode, precision=precision, criterion="reference", tol=tol, maxiter=maxiter,
+              n_for_tol=states_size, precond=precond)
+    vec, linfo = prob.solve(x0, **kw)
+    restarted = judged = False
+    if linfo.status == STATUS_INVARIANT:
+        if restart is None:
+            LOG.warning("sqd: the start spans an invariant subspace (zero residual); returning "
+                        "NaN as the reference does")
+            linfo.eigval = float("nan")
+        else:
+            LOG.warning("sqd: the start spans an invariant subspace (zero residual, energy "
+                        "%.12g); solving again from a Gaussian start", linfo.eigval)
+            vec, linfo = prob.solve(prob.start_vector("gaussian", seed=seed, dtype=dtype), **kw)
+            restarted = True
+    if judge:
+        vj, ij = prob.solve(prob.start_vector("gaussian", seed=seed, dtype=dtype), **kw)
+        if ij.converged and (not linfo.converged or ij.eigval < linfo.eigval):
+            vec, linfo = vj, ij
+            judged = True
+    if info is not None:
+        info.update(locg=linfo, restarted=restarted, judged=judged, n=prob.n)
+    LOG.info("Found ground eigenpair in %f seconds.", time.time() - t_start)
+    eigval = float(linfo.eigval)
     if not return_eigvec:
         return eigval
     v = vec.cpu().numpy()

@@ -21,7 +21,13 @@ import torch
 from . import _lib
 
 __all__ = ["StartNet", "targets_from_vector", "ground_targets", "gaussian_start",
-           "diag_feature", "energy_features"]
+           "diag_feature", "energy_features", "mix_gaussian", "DEFAULT_MIX"]
+
+# Norm of the Gaussian part a network start carries by default (the network's
+# part keeps norm sqrt(1 - DEFAULT_MIX^2)). A network that puts its start on
+# the wrong few states could otherwise leave LOBPCG at an excited state; with
+# it the ground vector's component is at least of order DEFAULT_MIX / sqrt(n).
+DEFAULT_MIX = 0.1
 
 
 def _p(t):
@@ -53,6 +59,20 @@ def gaussian_start(n: int, device, seed: int = 0) -> torch.Tensor:
     g = torch.Generator(device=dev).manual_seed(seed)
     return torch.complex(torch.randn(n, generator=g, dtype=torch.float64, device=dev),
                          torch.randn(n, generator=g, dtype=torch.float64, device=dev))
+
+
+def mix_gaussian(x: torch.Tensor, mix: float, seed: int = 0) -> torch.Tensor:
+    """sqrt(1 - mix^2) x / |x| + mix g / |g| with a seeded Gaussian g,
+    normalised (complex128)."""
+    if not 0.0 <= mix < 1.0:
+        raise ValueError("mix must lie in [0, 1)")
+    x = x.to(torch.complex128)
+    x = x / torch.linalg.vector_norm(x)
+    if mix > 0.0:
+        g = gaussian_start(x.numel(), x.device, seed)
+        x = math.sqrt(1.0 - mix * mix) * x + mix * g / torch.linalg.vector_norm(g)
+        x = x / torch.linalg.vector_norm(x)
+    return x
 
 
 def ground_targets(prob, *, tol: float = 1e-10, maxiter: int = 5000, seed: int = 0):
@@ -348,9 +368,24 @@ class StartNet:
         y = self.predict(keys, extra, chunk).to(torch.float64)
         a = y[:, 0] - y[:, 0].max()
         x = torch.exp(a) * torch.where(y[:, 1] > 0, 1.0, -1.0)
-        x = (x / torch.linalg.vector_norm(x)).to(torch.complex128)
-        if mix > 0.0:
-            g = gaussian_start(x.numel(), x.device, seed)
-            x = math.sqrt(1.0 - mix * mix) * x + mix * g / torch.linalg.vector_norm(g)
-            x = x / torch.linalg.vector_norm(x)
-        return x
+        return mix_gaussian(x, mix, seed)
+
+    def features(self, prob) -> torch.Tensor | None:
+        """The per-state features this network was built for over ``prob``'s
+        subspace (anything with ``.op``): none, ``diag_feature`` (1) or
+        ``energy_features`` (2)."""
+        if self.n_extra == 0:
+            return None
+        if self.n_extra == 1:
+            return diag_feature(prob)
+        if self.n_extra == 2:
+            return energy_features(prob)
+        raise ValueError(f"no feature set of width {self.n_extra}")
+
+    def start_for(self, prob, *, mix: float = DEFAULT_MIX, seed: int = 0) -> torch.Tensor:
+        """Start vector over ``prob``'s subspace (an ``SQDProblem`` or anything
+        with ``.space.keys`` and ``.op``) with a Gaussian part of norm ``mix``."""
+        if prob.space.num_qubits != self.nq:
+            raise ValueError(f"network built for {self.nq} qubits, subspace has "
+                             f"{prob.space.num_qubits}")
+        return self.start_vector(prob.space.keys, self.features(prob), mix=mix, seed=seed)

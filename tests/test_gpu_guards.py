@@ -1,0 +1,163 @@
+"""Guards of the solver where the reference's formulas give NaN or a wrong
+state: degenerate closed forms (rqutils/ground_locg.py:424-471), zero basis
+vectors in the reference iteration (:300-305, 421), non-finite operators, and
+a start that is an exact eigenvector of a multi-state subspace (:274-286).
+Each case first shows on the CPU, through the oracle's restatement, that the
+reference fails there, then checks the device path against eigh."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import dense, locg_oracle as lo
+from tests.helpers import OracleProblem, random_labels
+
+pytestmark = pytest.mark.gpu
+
+_DEGENERATE_3X3 = {
+    "diag": np.diag([3.0, -2.0, 1.0]),
+    "rank1": np.outer([1.0, 2.0, -1.0], [1.0, 2.0, -1.0]),
+    "rank2_double": np.diag([-1.0, -1.0, 2.0]),
+    "zero_row": np.array([[2.0, 0.5, 0.0], [0.5, 3.0, 0.0], [0.0, 0.0, 0.0]]),
+    "complex_double": None,
+}
+
+
+def _complex_double():
+    # U diag(-1, -1, 4) U^H with a seeded unitary: a double lowest eigenvalue
+    rng = np.random.default_rng(12)
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)) + 1j * rng.normal(size=(3, 3)))
+    return q @ np.diag([-1.0, -1.0, 4.0]) @ q.conj().T
+
+
+def _check_pair(a, lam, vec):
+    w = np.linalg.eigvalsh(a)
+    scale = max(1.0, np.abs(w).max())
+    assert np.isfinite(lam) and np.all(np.isfinite(vec)), (lam, vec)
+    assert abs(np.linalg.norm(vec) - 1.0) < 1e-12
+    assert abs(lam - w[0]) < 1e-12 * scale, (lam, w)
+    assert np.linalg.norm(a @ vec - lam * vec) < 1e-10 * scale
+
+
+@pytest.mark.parametrize("name", sorted(_DEGENERATE_3X3))
+def test_eigenpair_3x3_degenerate(gpu, name):
+    from rqutils_amd.ground_locg import eigenpair_3x3
+    a = _DEGENERATE_3X3[name]
+    a = (_complex_double() if a is None else a).astype(np.complex128)
+    lam, vec = eigenpair_3x3(a)
+    _check_pair(a, lam, vec)
+
+
+@pytest.mark.parametrize("a", [np.diag([-1.0, 1.0]), np.diag([1.0, -1.0]),
+                               np.diag([2.0, 2.0]), np.zeros((2, 2)),
+                               np.array([[0.0, 1j], [-1j, 0.0]])])
+def test_eigenpair_2x2_degenerate(gpu, a):
+    from rqutils_amd.ground_locg import eigenpair_2x2
+    a = a.astype(np.complex128)
+    lam, vec = eigenpair_2x2(a)
+    _check_pair(a, lam, vec)
+
+
+@pytest.mark.parametrize("a", [[[2.0, 0.5], [0.5, 3.0]], [[-2.0, 0.5], [0.5, 3.0]]])
+def test_reference_mode_with_zeroed_p_ends_finite(gpu, a):
+    """In a two-state problem p is always zeroed by the 0.99 rule after the
+    first update; the reference then returns NaN (oracle below). The device
+    reference iteration runs its Rayleigh-Ritz on the nonzero vectors and
+    ends at the lowest eigenvalue."""
+    from rqutils_amd.ground_locg import ground_locg
+    a = np.asarray(a, dtype=np.complex128)
+    x0 = np.array([1.0, 0.3 + 0.1j])
+    with np.errstate(all="ignore"):
+        th_ref, _, _ = lo.ground_locg(lambda v: a @ v, x0, maxiter=5, tol=1e-300)
+    assert np.isnan(th_ref)
+    ev, vec, it = ground_locg(torch.from_numpy(a).cuda(), torch.from_numpy(x0).cuda(),
+                              maxiter=5, tol=1e-300, mode="reference")
+    w = np.linalg.eigvalsh(a)
+    vv = vec.cpu().numpy()
+    assert np.isfinite(ev) and abs(ev - w[0]) < 1e-12 * max(1, abs(w[0])), (ev, w)
+    assert np.all(np.isfinite(vv))
+    assert np.linalg.norm(a @ vv - ev * vv) < 1e-10 * np.linalg.norm(vv)
+
+
+def test_reference_mode_stops_on_nan(gpu):
+    """A NaN in the operator's output ends the reference-mode solve with
+    status 3 (the reference iterates on NaN until maxiter)."""
+    from rqutils_amd.ground_locg import STATUS_NONFINITE, solve
+    n = 64
+    a = torch.randn(n, n, dtype=torch.float64, generator=torch.Generator().manual_seed(1))
+    a = (a + a.T).to(torch.complex128).cuda()
+    calls = []
+
+    def op(x):
+        calls.append(1)
+        y = a @ x
+        if len(calls) > 6:
+            y[5] = float("nan")
+        return y
+
+    x0 = torch.randn(n, dtype=torch.complex128, generator=torch.Generator().manual_seed(2)).cuda()
+    _, info = solve(op, x0, mode="reference", precision="c128", criterion="reference",
+                    tol=1e-300, maxiter=500)
+    assert info.status == STATUS_NONFINITE and not info.converged
+    assert len(calls) < 20
+
+
+def _isolated_min_diag_problem():
+    """A 10-qubit subspace whose smallest-diagonal state has no off-diagonal
+    neighbour in it and lies above the ground energy (found by a seeded
+    search)."""
+    n = 10
+    for seed in range(200):
+        rng = np.random.default_rng(1000 + seed)
+        labels = random_labels(rng, n, 60, 1, 4)
+        coeffs = rng.normal(size=60)
+        h = dense.dense_hamiltonian(labels, coeffs)
+        keys = np.sort(rng.choice(2 ** n, size=400, replace=False))
+        d = np.real(np.diag(h))
+        s = keys[np.argmin(d[keys])]
+        xs = {int(sum(1 << (n - 1 - q) for q, ch in enumerate(lab) if ch in "XY"))
+              for lab in labels} - {0}
+        keys = np.setdiff1d(keys, [s ^ x for x in xs])
+        sub = h[np.ix_(keys, keys)]
+        e0 = np.linalg.eigvalsh(sub)[0]
+        if d[s] - e0 > 0.5:
+            return OracleProblem(labels, coeffs, keys.astype(np.uint64), n), e0, d[s]
+    raise AssertionError("no isolated minimum found")
+
+
+def test_invariant_start_is_flagged_and_restarted(gpu):
+    """The reference's default one-hot on an isolated state has a zero
+    residual: the device solve stops with status 4 (not a ground state),
+    sqd() solves again from a Gaussian start by default and returns NaN, as
+    the reference does, with restart=None."""
+    from rqutils_amd.ground_locg import STATUS_INVARIANT
+    from rqutils_amd.operator import keys_to_states
+    from rqutils_amd.sqd import SQDProblem, sqd
+    p, e0, d_iso = _isolated_min_diag_problem()
+    prob = SQDProblem((p.labels, p.coeffs), p.keys, p.n)
+    _, info = prob.solve(prob.vinit_min_diag(), mode="fused", precision="c128",
+                         criterion="relative", tol=1e-10)
+    assert info.status == STATUS_INVARIANT and abs(info.eigval - d_iso) < 1e-10
+    states = keys_to_states(p.keys, p.n)
+    out = {}
+    ev, vec, basis = sqd((p.labels, p.coeffs), states, info=out)
+    assert out["restarted"] and abs(ev - e0) < 1e-9 * abs(e0), (ev, e0)
+    ev_nan = sqd((p.labels, p.coeffs), states, return_eigvec=False, restart=None)
+    assert np.isnan(ev_nan)
+    # the fused mode with the diagonal preconditioner restarts the same way
+    ev2 = sqd((p.labels, p.coeffs), states, return_eigvec=False, mode="fused",
+              precision="c64", tol=1e-7, precond="diag")
+    assert abs(ev2 - e0) < 1e-6 * abs(e0), (ev2, e0)
+
+
+def test_sqd_judge_keeps_the_lower_energy(gpu):
+    """judge=True also solves from a Gaussian start and keeps the lower
+    Rayleigh quotient; on an ordinary subspace both agree with eigh."""
+    from rqutils_amd.operator import keys_to_states
+    from rqutils_amd.sqd import sqd
+    from tests.helpers import small_problem
+    p = small_problem(31, n=10, terms=60, nstates=400)
+    e0 = p.ground()
+    out = {}
+    ev = sqd((p.labels, p.coeffs), keys_to_states(p.keys, p.n), return_eigvec=False,
+             judge=True, info=out)
+    assert abs(ev - e0) < 1e-9 * abs(e0) and "judged" in out

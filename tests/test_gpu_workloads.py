@@ -1,0 +1,258 @@
+"""The five benchmark workloads through the HIP path at their real sizes.
+
+* 16-qubit sector, 200 terms, 4,096 states (the committed ``small16.npz``
+  fixture): product within 1e-12, source indices bit for bit, FP64 LOBPCG
+  from the fixture's seeded Gaussian start to a 1e-8 residual within 1e-10 of
+  the fixture's dense eigenvalue.
+* 28-qubit Heisenberg chain plus 2,000 disorder strings, 2^22 domain-wall
+  sampled states, and the 40-qubit random Pauli sum of 20,000 strings over
+  2^24 uniform sector states: source indices bit for bit against a binary
+  search of the sorted keys, the product against the oracle's per-row
+  restatement on sampled rows (complex128 within 1e-12, complex64 within
+  2e-6), and the fast solve (fused FP32/FP64 with the diagonal preconditioner
+  over the cached hit table) and the plain FP64 reference iteration against
+  the FP64 energy at a 1e-10 residual (1e-6 relative).
+* 64 instances of 20-32 qubits in one batch: each instance's batched product
+  bit-identical to its own launch, each batched FP32/FP64 energy within 1e-6
+  of its own FP64 solve.
+* The 28-input start network trained on FP64 ground vectors of 2^20-state
+  28-qubit chains, evaluated on held-out seeds: every network-started solve
+  within 1e-6 of the Gaussian-started FP64 energy.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import sqd_oracle as so
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+# ---- 16 qubits: the committed fixture -----------------------------------------------
+
+def test_small16_fixture_through_the_hip_path(gpu):
+    from rqutils_amd.sqd import SQDProblem
+    g = np.load(os.path.join(GOLDEN, "small16.npz"), allow_pickle=False)
+    labels = [str(s) for s in g["labels"]]
+    n = int(g["n"])
+    prob = SQDProblem((labels, g["coeffs"]), g["keys"], n, device=gpu)
+    hv = prob.matvec(torch.from_numpy(g["v"]).to(gpu)).cpu().numpy()
+    assert _rel(hv, g["hv"]) < 1e-12
+    for k, mask in enumerate(g["xs_masks"]):
+        got = prob.space.xsource(int(mask)).cpu().numpy()
+        assert np.array_equal(got, g["xs"][k]), k
+    rng = np.random.default_rng(int(g["seed_x0"]))
+    x0 = torch.from_numpy(rng.normal(size=prob.n) + 1j * rng.normal(size=prob.n)).to(gpu)
+    e0 = float(g["e0"])
+    for mode in ("reference", "fused"):
+        for cache in (False, True):
+            prob.op.use_cache = cache
+            vec, info = prob.solve(x0, mode=mode, precision="c128", criterion="relative",
+                                   tol=1e-8, maxiter=3000)
+            assert info.converged and info.status == 0, (mode, cache, info)
+            assert abs(info.eigval - e0) < 1e-10 * abs(e0), (mode, cache, info.eigval, e0)
+            x = vec.cpu().numpy()
+            x = x / np.linalg.norm(x)
+            r = prob.matvec(torch.from_numpy(x).to(gpu)).cpu().numpy() - info.eigval * x
+            assert np.linalg.norm(r) <= 1.01e-8 * abs(info.eigval), (mode, cache)
+    prob.op.use_cache = False
+
+
+def test_small16_default_start_matches_oracle(gpu):
+    """The reference's default start over the fixture's subspace (and with
+    fill rows, states_size > N) picks the oracle's row."""
+    from rqutils_amd.operator import keys_to_states
+    from rqutils_amd.sqd import SQDProblem
+    g = np.load(os.path.join(GOLDEN, "small16.npz"), allow_pickle=False)
+    labels = [str(s) for s in g["labels"]]
+    n = int(g["n"])
+    prob = SQDProblem((labels, g["coeffs"]), g["keys"], n, device=gpu)
+    x, z, c, _ = so.from_paulisum(labels, g["coeffs"], force_real=True, add_padding=True)
+    packed = so.pack_states(keys_to_states(g["keys"], n))
+    su, nu = so.uniquify_states(packed, packed.shape[0] + 100)
+    want = int(np.flatnonzero(so.vinit_from_min_diag(x, z, c, su, nu))[0])
+    assert prob.vinit_index() == want
+
+
+# ---- 28 and 40 qubits at full size ---------------------------------------------------
+
+@pytest.fixture(scope="module", params=["heisenberg28", "random40"])
+def full(request, gpu):
+    from rqutils_amd import workloads as W
+    from rqutils_amd.sqd import SQDProblem
+    w = W.make(request.param)
+    print(f"[workload] {w.name}: {w.n} states, {len(w.labels)} terms", flush=True)
+    prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, device=gpu)
+    yield w, prob
+    prob.op.drop_cache()
+    del prob
+    torch.cuda.empty_cache()
+
+
+def _searchsorted_xsource(keys, mask):
+    t = keys ^ np.uint64(mask)
+    j = np.searchsorted(keys, t)
+    jc = np.minimum(j, keys.size - 1)
+    return np.where((j < keys.size) & (keys[jc] == t), j, -1).astype(np.int32)
+
+
+def test_full_size_xsource_bit_exact(full):
+    w, prob = full
+    masks = prob.op.host.xmask
+    hits = [int(torch.count_nonzero(prob.space.xsource(int(m)) >= 0).item()) for m in masks[:64]]
+    picks = sorted({0, len(masks) // 2, len(masks) - 1, int(np.argmax(hits))})
+    for gi in picks:
+        got = prob.space.xsource(int(masks[gi])).cpu().numpy()
+        want = _searchsorted_xsource(w.keys, int(masks[gi]))
+        assert np.array_equal(got, want), (w.name, gi)
+    # an odd-weight mask the operator pruned still finds nothing, bit for bit
+    got = prob.space.xsource(1).cpu().numpy()
+    assert np.array_equal(got, _searchsorted_xsource(w.keys, 1))
+
+
+def test_full_size_matvec_rows_vs_oracle(full):
+    from rqutils_amd.operator import keys_to_states
+    w, prob = full
+    n = w.n
+    rng = np.random.default_rng(11)
+    v = rng.normal(size=n) + 1j * rng.normal(size=n)
+    vt = torch.from_numpy(v).to(prob.space.device)
+    out128 = prob.matvec(vt).cpu().numpy()
+    out64 = prob.matvec(vt.to(torch.complex64)).cpu().numpy().astype(np.complex128)
+    # sampled rows: random ones plus rows with hits in a few groups
+    rows = rng.choice(n, 2048, replace=False)
+    for m in prob.op.host.xmask[:16]:
+        src = prob.space.xsource(int(m)).cpu().numpy()
+        hit = np.flatnonzero(src >= 0)
+        if hit.size:
+            rows = np.concatenate([rows, rng.choice(hit, min(32, hit.size), replace=False)])
+    rows = np.unique(rows)
+    x, z, c, _ = so.from_paulisum(w.labels, w.coeffs, force_real=True, add_padding=True)
+    packed = so.pack_states(keys_to_states(w.keys, w.num_qubits))
+    want = so.apply_h_rows(rows, v, x, z, c, packed)
+    assert _rel(out128[rows], want) < 1e-12, w.name
+    assert _rel(out64[rows], want) < 2e-6, w.name
+    assert _rel(out64, out128) < 2e-6, w.name
+
+
+def test_full_size_energy_vs_fp64(full):
+    from rqutils_amd.startnet import gaussian_start
+    w, prob = full
+    dev = prob.space.device
+    x0 = gaussian_start(prob.n, dev, 77)
+    prob.op.use_cache = True
+    _, ref = prob.solve(x0, mode="fused", precision="c128", criterion="relative", tol=1e-10,
+                        maxiter=5000)
+    assert ref.converged, ref
+    e_ref = ref.eigval
+    runs = {"fused_c64_precond": dict(mode="fused", precision="c64", precond="diag"),
+            "fused_c64": dict(mode="fused", precision="c64"),
+            "reference_c128": dict(mode="reference", precision="c128")}
+    for name, kw in runs.items():
+        _, info = prob.solve(x0, criterion="relative", tol=1e-6, maxiter=3000, **kw)
+        err = abs(info.eigval - e_ref) / abs(e_ref)
+        print(f"[energy] {w.name} {name}: {info.niter} it, rel err {err:.2e}", flush=True)
+        assert info.converged and info.status == 0, (w.name, name, info)
+        assert err <= 1e-6, (w.name, name, info.eigval, e_ref)
+    # the join product (no table) gives the same energy
+    prob.op.use_cache = False
+    _, info = prob.solve(x0, mode="fused", precision="c64", precond="diag", criterion="relative",
+                         tol=1e-6, maxiter=3000)
+    assert info.converged and abs(info.eigval - e_ref) <= 1e-6 * abs(e_ref), (w.name, info)
+
+
+# ---- 64 instances in one batch -------------------------------------------------------
+
+def test_batch64_products_and_energies(gpu):
+    from rqutils_amd import workloads as W
+    from rqutils_amd.batch import SQDBatch
+    from rqutils_amd.operator import PauliOperator, Subspace
+    from rqutils_amd import _lib
+    ws = [W.batch_instance(s) for s in range(64)]
+    inst = [(w.hamiltonian, w.keys, w.num_qubits) for w in ws]
+    b = SQDBatch(inst, device=gpu, bucket_cap=2048, rows_per_thread=4)
+    print(f"[batch64] {b.n} rows", flush=True)
+    rng = np.random.default_rng(5)
+    v = torch.from_numpy(rng.normal(size=b.n) + 1j * rng.normal(size=b.n)).to(gpu)
+    outs = {dt: b.matvec(v.to(dt)) for dt in (torch.complex128, torch.complex64)}
+    for i, w in enumerate(ws):
+        sp = Subspace(w.keys, w.num_qubits, device=gpu, bucket_cap=2048, rows_per_thread=4,
+                      dense=False)
+        op = PauliOperator(w.hamiltonian, sp)
+        for dt, out in outs.items():
+            own = op.matvec(b.segment(v.to(dt), i).contiguous())
+            assert torch.equal(b.segment(out, i), own), (i, dt)
+    e_ref = {}
+    for cache in (False, True):
+        b.use_cache = cache
+        if cache:
+            b.build_cache(_lib.C64)
+        res = b.solve(precision="c64", tol=1e-6, maxiter=3000, start="gaussian", seed=3)
+        for i, w in enumerate(ws):
+            if not cache:
+                from rqutils_amd.ground_locg import solve
+                from rqutils_amd.startnet import gaussian_start
+                sp = Subspace(w.keys, w.num_qubits, device=gpu)
+                op = PauliOperator(w.hamiltonian, sp)
+                op.use_cache = True
+                _, ref = solve(op, gaussian_start(sp.n, gpu, 900 + i), mode="fused",
+                               precision="c128", criterion="relative", tol=1e-10, maxiter=5000)
+                assert ref.converged, (i, ref)
+                e_ref[i] = ref.eigval
+                del op, sp
+            info = res.infos[i]
+            err = abs(info.eigval - e_ref[i]) / abs(e_ref[i])
+            assert info.converged and err <= 1e-6, (cache, i, info, e_ref[i])
+    b.use_cache = False
+
+
+# ---- the 28-input start network ------------------------------------------------------
+
+def test_network_start_heldout_energies(gpu):
+    """A 28-input network (bits as +-1 only) trained on FP64 ground vectors of
+    2^20-state chains; on held-out seeds every network-started solve (with the
+    default Gaussian part, through sqd.start_vector) reaches the Gaussian
+    start's FP64 energy within 1e-6, and so does the batched solve of the
+    held-out instances with the network as start and the Gaussian as judge."""
+    from rqutils_amd import workloads as W
+    from rqutils_amd.batch import SQDBatch
+    from rqutils_amd.sqd import SQDProblem
+    from rqutils_amd.startnet import StartNet, ground_targets
+    train = [W.heisenberg28(seed=1000 + s, n_states=1 << 20) for s in range(6)]
+    test = [W.heisenberg28(seed=2000 + s, n_states=1 << 20) for s in range(4)]
+    keys, tgts = [], []
+    for w in train:
+        p = SQDProblem(w.hamiltonian, w.keys, 28, device=gpu)
+        tg, _, info = ground_targets(p)
+        assert info.converged
+        keys.append(p.space.keys)
+        tgts.append(tg)
+    net = StartNet(28, device=gpu, seed=0)
+    assert net.n_in == 28
+    losses = net.train(keys, tgts, steps=600, lr=3e-3)
+    assert losses[-1] < losses[0]
+    refs = []
+    for w in test:
+        p = SQDProblem(w.hamiltonian, w.keys, 28, device=gpu)
+        _, _, ref = ground_targets(p, seed=w.seed)
+        refs.append(ref.eigval)
+        for precond in (None, "diag"):
+            x0 = p.start_vector(net, seed=w.seed)
+            _, info = p.solve(x0, mode="fused", precision="c64", criterion="relative", tol=1e-6,
+                              maxiter=3000, precond=precond)
+            err = abs(info.eigval - ref.eigval) / abs(ref.eigval)
+            print(f"[netstart] seed {w.seed} precond={precond}: {info.niter} it, err {err:.1e}",
+                  flush=True)
+            assert info.converged and err <= 1e-6, (w.seed, precond, info, ref.eigval)
+    b = SQDBatch([(w.hamiltonian, w.keys, 28) for w in test], device=gpu)
+    res = b.solve(start=net, judge=True, precision="c64", tol=1e-6, maxiter=3000)
+    for i, e in enumerate(refs):
+        assert res.infos[i].converged and abs(res.infos[i].eigval - e) <= 1e-6 * abs(e), i

@@ -8,7 +8,8 @@ import numpy as np
 import pytest
 
 from oracle import dense, locg_oracle as lo, sqd_oracle as so
-from tests.helpers import OracleProblem, small_problem
+from rqutils_amd.operator import keys_to_states
+from tests.helpers import OracleProblem, random_labels, small_problem
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
@@ -414,3 +415,96 @@ def test_oracle_xsource_is_a_dictionary_lookup(case):
     pos = {int(k): i for i, k in enumerate(keys)}
     want = np.array([pos.get(int(k) ^ x, -1) for k in keys])
     assert np.array_equal(got, want)
+
+
+# ---- default start (vinit) -------------------------------------------------------
+
+def _brute_vinit(labels, coeffs, keys, n):
+    """Index of the first smallest diagonal entry <s|H|s> over the real rows,
+    from dense Kronecker products (independent of the X/Z code), or 0 when no
+    term is diagonal."""
+    if not any(set(lab) <= {"I", "Z"} for lab, c in zip(labels, coeffs) if c != 0):
+        return 0
+    diag = np.real(np.diag(dense.dense_hamiltonian(labels, np.real(coeffs))))
+    d = diag[keys.astype(np.int64)]
+    return int(np.flatnonzero(d == d.min())[0])
+
+
+@pytest.mark.parametrize("case", ["random", "ties", "fill_lowest", "no_diag"])
+def test_vinit_from_min_diag_matches_brute_force(case):
+    """Pins the default start (rqutils/sqd.py:373-398): argmin over the real
+    rows with the first of equal entries, fill rows (0xFF) never chosen even
+    when their diagonal would be the smallest, row 0 without an X = 0 group."""
+    rng = np.random.default_rng({"random": 1, "ties": 2, "fill_lowest": 3, "no_diag": 4}[case])
+    n = 8
+    if case == "random":
+        labels = random_labels(rng, n, 40, 1, 3)
+        coeffs = rng.normal(size=40)
+    elif case == "ties":
+        # small integer Z couplings: the diagonal takes few values, many rows tie
+        labels = ["ZZIIIIII", "IIZZIIII", "IIIIZIZI"]
+        coeffs = np.array([1.0, -1.0, 1.0])
+    elif case == "fill_lowest":
+        # sum_q Z_q: the all-ones row (the fill row's bits) has the lowest diagonal
+        labels = ["I" * q + "Z" + "I" * (n - 1 - q) for q in range(n)] + ["XXIIIIII"]
+        coeffs = np.concatenate([np.ones(n), [0.3]])
+    else:
+        labels = ["XIIIIIIY", "IXZIIIII", "IIIYXIII"]
+        coeffs = np.array([1.0, -0.5, 0.7])
+    keys = np.sort(rng.choice(2 ** n - 1, size=120, replace=False)).astype(np.uint64)
+    keys = keys[::-1][:100][::-1].copy()            # never the all-ones key
+    x, z, c, _ = so.from_paulisum(labels, coeffs, force_real=True, add_padding=True)
+    packed = so.pack_states(keys_to_states(keys, n))
+    su, nu = so.uniquify_states(packed[rng.permutation(len(keys))], len(keys) + 17)
+    assert nu == len(keys) and np.all(su[nu:] == 255)
+    v = so.vinit_from_min_diag(x, z, c, su, nu)
+    want = _brute_vinit(labels, coeffs, keys, n)
+    assert v.shape == (len(keys) + 17,) and np.count_nonzero(v) == 1
+    assert int(np.flatnonzero(v)[0]) == want, (case, np.flatnonzero(v), want)
+    if case == "ties":
+        d = np.real(np.diag(dense.dense_hamiltonian(labels, coeffs)))[keys.astype(np.int64)]
+        assert np.count_nonzero(d == d.min()) > 1, "the case must hold a tie"
+    if case == "fill_lowest":
+        # without the masking the fill rows would win: the guard matters here
+        diag = np.real(so.get_diagonal(z[0], c[0], su))
+        assert diag[nu:].min() < diag[:nu].min()
+
+
+# ---- where the reference's closed forms fail (the device guards these) ----------
+
+def test_reference_closed_forms_fail_where_guarded():
+    """The reference's own formulas (the oracle's restatement) give a wrong
+    vector for the diagonal 3x3 (rounding noise in column 1 of A - lambda I
+    picks e0 instead of e1) and NaN for the diagonal 2x2 whose lower entry is
+    the smaller: the cases the device closed forms guard."""
+    a3 = np.diag([3.0, -2.0, 1.0]).astype(complex)
+    lam3, v3 = lo.eigenpair_3x3(a3)
+    assert abs(lam3 + 2.0) < 1e-12
+    assert not np.all(np.isfinite(v3)) or np.linalg.norm(a3 @ v3 - lam3 * v3) > 1.0
+    with np.errstate(all="ignore"):
+        _, v2 = lo.eigenpair_2x2(np.diag([-1.0, 1.0]).astype(complex))
+    assert not np.all(np.isfinite(v2))
+
+
+def test_oracle_locg_reference_has_no_zero_guard():
+    """Documents the reference behaviour the restart replaces: the oracle's
+    reference iteration from an exact eigenvector start divides 0 by 0."""
+    a = np.diag([1.0, -1.0, 2.0]).astype(complex)
+    x0 = np.array([0.0, 1.0, 0.0], complex)
+    with np.errstate(all="ignore"):
+        th, _, _ = lo.ground_locg(lambda v: a @ v, x0, maxiter=3)
+    assert np.isnan(th)
+
+
+@pytest.mark.parametrize("seed", [0, 5])
+def test_apply_h_rows_equals_full_product(seed):
+    """The per-row restatement (for full-size subspaces) against the whole
+    cached-source product on a small problem, including a complex Hamiltonian
+    and rows whose targets fall before the first and after the last row."""
+    p = small_problem(seed, n=11, terms=80, nstates=700, full=bool(seed))
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    full = p.matvec(v)
+    rows = np.concatenate([[0, p.keys.size - 1], rng.choice(p.keys.size, 100, replace=False)])
+    got = so.apply_h_rows(rows, v, p.x, p.z, p.c, p.packed)
+    assert np.array_equal(got, full[rows])
