@@ -144,9 +144,11 @@ def test_invariant_start_is_flagged_and_restarted(gpu):
     ev_nan = sqd((p.labels, p.coeffs), states, return_eigvec=False, restart=None)
     assert np.isnan(ev_nan)
     # the fused mode with the diagonal preconditioner restarts the same way
+    # (sqd keeps the reference's stopping rule |r| < tol 10 N (|Ax| - theta),
+    # loose at small N: a tight tol makes the energy comparable)
     ev2 = sqd((p.labels, p.coeffs), states, return_eigvec=False, mode="fused",
-              precision="c64", tol=1e-7, precond="diag")
-    assert abs(ev2 - e0) < 1e-6 * abs(e0), (ev2, e0)
+              precision="c128", tol=1e-13, precond="diag")
+    assert abs(ev2 - e0) < 1e-9 * abs(e0), (ev2, e0)
 
 
 def test_sqd_judge_keeps_the_lower_energy(gpu):
