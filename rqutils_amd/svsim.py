@@ -47,36 +47,38 @@ class CircuitXZ:
     num_qubits: int
 
 
+def _qiskit_specs(circuit):
+    """Gate specifiers of a circuit given as a sequence of instructions (a
+    qiskit ``QuantumCircuit`` or anything with its ``data``, ``find_bit`` and
+    ``num_qubits``): each instruction's name, qubit indices and first
+    parameter. ``cz`` has no rotation form in the gate table, so it becomes
+    rzz(pi/2) rz(-pi/2) rz(-pi/2) on its two qubits, which equals
+    e^{i pi/4} CZ; the global phase is dropped, as the reference drops it
+    (rqutils/svsim.py:147-152)."""
+    specs = []
+    for inst in circuit.data:
+        name = inst.operation.name
+        where = [circuit.find_bit(q).index for q in inst.qubits]
+        if name == "cz":
+            a, b = where
+            specs.append(("rzz", [a, b], np.pi / 2.0))
+            specs.append(("rz", [a], -np.pi / 2.0))
+            specs.append(("rz", [b], -np.pi / 2.0))
+        else:
+            specs.append((name, where) + tuple(inst.operation.params[:1]))
+    return specs, int(circuit.num_qubits)
+
+
 def to_circuitxz(circuit) -> CircuitXZ:
     """Gate specifiers to X/Z masks and (cos, sin) of half the negated angle
     (rqutils/svsim.py:130-194); fixed gates x, y, z get (0, -1). The number of
-    qubits is one more than the largest index used. A qiskit QuantumCircuit
-    is accepted when qiskit is importable (rqutils/svsim.py:138-160)."""
+    qubits is one more than the largest index used, or the circuit's own for
+    a qiskit-style circuit (``_qiskit_specs``; rqutils/svsim.py:138-160)."""
     if isinstance(circuit, CircuitXZ):
         return circuit
     num_qubits = None
-    try:  # optional, as in the reference
-        from qiskit.circuit import QuantumCircuit
-    except ImportError:
-        QuantumCircuit = None
-    if QuantumCircuit is not None and isinstance(circuit, QuantumCircuit):
-        def qidx(qubits):
-            if isinstance(qubits, tuple):
-                return np.array([circuit.qregs[0].index(q) for q in qubits])
-            return np.array([circuit.qregs[0].index(qubits)])
-        specs = []
-        for datum in circuit.data:
-            op = datum.operation
-            if op.name == "cz":
-                specs += [("rzz", qidx(datum.qubits), np.pi / 2.0),
-                          ("rz", qidx(datum.qubits[0]), -np.pi / 2.0),
-                          ("rz", qidx(datum.qubits[1]), -np.pi / 2.0)]
-            elif op.params:
-                specs.append((op.name, qidx(datum.qubits), op.params[0]))
-            else:
-                specs.append((op.name, qidx(datum.qubits)))
-        num_qubits = circuit.num_qubits
-        circuit = specs
+    if hasattr(circuit, "data") and hasattr(circuit, "find_bit"):
+        circuit, num_qubits = _qiskit_specs(circuit)
     g = len(circuit)
     x = np.zeros(g, np.int64)
     z = np.zeros(g, np.int64)

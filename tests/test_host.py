@@ -163,3 +163,62 @@ def test_sqd_argument_errors():
         sqd((["XX", "ZZ"], [1.0, 0.5]), states, states_size=1)
     with pytest.raises(ValueError):
         sqd((["XX", "ZZ"], [1.0, 0.5]), states, cache_level=(2, 0))
+
+
+class _Bit:
+    pass
+
+
+class _Op:
+    def __init__(self, name, params):
+        self.name, self.params = name, list(params)
+
+
+class _Inst:
+    def __init__(self, name, qubits, params=()):
+        self.operation, self.qubits = _Op(name, params), qubits
+
+
+class _Circuit:
+    """The parts of a qiskit QuantumCircuit that the conversion reads."""
+
+    def __init__(self, n):
+        self.num_qubits = n
+        self.qubits = [_Bit() for _ in range(n)]
+        self.data = []
+
+    def find_bit(self, bit):
+        class _Loc:
+            index = self.qubits.index(bit)
+        return _Loc()
+
+    def add(self, name, qs, *params):
+        self.data.append(_Inst(name, [self.qubits[q] for q in qs], params))
+
+
+def test_svsim_qiskit_style_circuit_conversion():
+    """A qiskit-shaped circuit converts to the same symplectic gates as the
+    equivalent specifier list, and cz becomes e^{i pi/4} CZ (checked on the
+    dense matrices of the converted gates; oracle conventions)."""
+    from oracle import svsim_oracle as sv
+    from rqutils_amd.svsim import to_circuitxz
+    c = _Circuit(4)
+    c.add("rx", [0], 0.3)
+    c.add("cz", [1, 3])
+    c.add("x", [2])
+    c.add("rzz", [0, 2], -0.7)
+    got = to_circuitxz(c)
+    want = to_circuitxz([("rx", [0], 0.3), ("rzz", [1, 3], np.pi / 2), ("rz", [1], -np.pi / 2),
+                         ("rz", [3], -np.pi / 2), ("x", [2]), ("rzz", [0, 2], -0.7)])
+    assert got.num_qubits == 4
+    for f in ("x", "z", "cos", "sin"):
+        assert np.array_equal(getattr(got, f), getattr(want, f)), f
+    # the cz expansion is CZ up to the global phase e^{i pi/4}
+    m = np.eye(16, dtype=complex)
+    for name, qs, ang in [("rzz", [1, 3], np.pi / 2), ("rz", [1], -np.pi / 2),
+                          ("rz", [3], -np.pi / 2)]:
+        m = sv.gate_matrix_dense(name, qs, ang, 4) @ m
+    idx = np.arange(16)
+    cz = np.diag(np.where(((idx >> 1) & 1) & ((idx >> 3) & 1), -1.0, 1.0)).astype(complex)
+    ph = m[0, 0] / abs(m[0, 0])
+    assert np.allclose(m, ph * cz, atol=1e-12)
