@@ -508,3 +508,32 @@ def test_apply_h_rows_equals_full_product(seed):
     rows = np.concatenate([[0, p.keys.size - 1], rng.choice(p.keys.size, 100, replace=False)])
     got = so.apply_h_rows(rows, v, p.x, p.z, p.c, p.packed)
     assert np.array_equal(got, full[rows])
+
+
+# ---- the C restatement of the reference's dataflow (CPU baseline) -----------------
+
+@pytest.mark.parametrize("seed,full", [(0, False), (6, True)])
+def test_c_restatement_matches_numpy_oracle(seed, full):
+    """ref_dataflow.c against the NumPy restatement: source indices bit for
+    bit for every group, the product of all groups within 1e-12 on one and on
+    four threads, with and without precomputed source indices."""
+    from oracle import ref_dataflow as rd
+    p = small_problem(seed, n=11, terms=90, nstates=900, full=full)
+    t = rd.RefTables(p.labels, p.coeffs, p.n)
+    for g in range(p.x.shape[0]):
+        assert np.array_equal(rd.xsource(p.keys, p.n, t.xmask[g]), p.xsources[g]), g
+    rng = np.random.default_rng(seed)
+    v = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    want = p.matvec(v)
+    for threads in (1, 4):
+        got, used = rd.matvec(t, p.keys, v, nthreads=threads)
+        assert used == threads
+        assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    got, _ = rd.matvec(t, p.keys, v, xsources=p.xsources, nthreads=2)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+    # a group subset is the sum of those groups only
+    sub = np.array([0, 3, 5], np.int32)
+    got, _ = rd.matvec(t, p.keys, v, groups=sub, nthreads=2)
+    part = sum(so.apply_xgrp(p.xsources[g], so.get_diagonal(p.z[g], p.c[g], p.packed), v)
+               for g in sub)
+    assert np.linalg.norm(got - part) <= 1e-12 * np.linalg.norm(part)
