@@ -105,13 +105,16 @@ def project_out(basis, vector):
     return vector * (1.0 if _norm(vector) >= 0.99 else 0.0)
 
 
-def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=None):
+def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=None,
+                rel_tol=None):
     """Single-vector LOBPCG -- ground_locg.py:191-377 (callable path).
 
     ``n_for_tol`` is the vector length used in the convergence threshold
     (``xcurr.shape[0]`` in the reference, i.e. the padded length in sqd).
     If ``history`` is a list, ``(niter, theta, |r|, reltol)`` tuples are
-    appended once per loop iteration. Returns ``(theta, x, niter)``.
+    appended once per loop iteration. ``rel_tol`` replaces the reference's
+    stopping rule by ``|r| <= rel_tol |theta|`` (the benchmark's residual).
+    Returns ``(theta, x, niter)``.
     """
     x = np.array(xinit, dtype=np.result_type(xinit.dtype, np.float64))
     if tol is None:
@@ -150,7 +153,8 @@ def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=N
         rn = axn - xn * theta
         reltol = (_norm(axn) - theta) * nlen * 10
         nrn = _norm(rn)
-        converged = bool(nrn < tol * reltol)
+        converged = bool(nrn < tol * reltol) if rel_tol is None else \
+            bool(nrn <= rel_tol * abs(theta))
         niter += 1
         x, y, r = xn, yn, rn
         if history is not None:

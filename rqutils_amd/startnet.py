@@ -75,12 +75,14 @@ def mix_gaussian(x: torch.Tensor, mix: float, seed: int = 0) -> torch.Tensor:
     return x
 
 
-def ground_targets(prob, *, tol: float = 1e-10, maxiter: int = 5000, seed: int = 0):
+def ground_targets(prob, *, tol: float = 1e-10, maxiter: int = 5000, seed: int = 0,
+                   precond: str | None = None):
     """Ground vector of an ``SQDProblem`` from the FP64 fused LOBPCG (Gaussian
-    start) and its training targets. Returns (targets, vector, info)."""
+    start; ``precond="diag"`` for the preconditioned iteration) and its
+    training targets. Returns (targets, vector, info)."""
     x0 = gaussian_start(prob.n, prob.space.keys.device, seed)
     vec, info = prob.solve(x0, mode="fused", precision="c128", criterion="relative", tol=tol,
-                           maxiter=maxiter)
+                           maxiter=maxiter, precond=precond)
     return targets_from_vector(vec), vec, info
 
 
