@@ -783,6 +783,10 @@ C5_STARTS = {
     "min_diag_mix_precond": ("min_diag", 0.1, "diag"),
     "network": ("net", None, None),
     "network_precond": ("net", None, "diag"),
+    # 30 inputs: the bits plus two diagonal-energy features (not the 28-input
+    # configuration; shown beside it)
+    "network_features": ("net_features", None, None),
+    "network_features_precond": ("net_features", None, "diag"),
 }
 C5_UNGUARDED = {
     # for comparison only: starts without a Gaussian part
@@ -791,13 +795,14 @@ C5_UNGUARDED = {
 }
 
 
-def c5_block(dev, n_train, n_test, steps, features=False):
+def c5_block(dev, n_train, n_test, steps, features=True):
     """Network-started LOBPCG on the 28-qubit chain family at 2^20 states: a
-    28-input BF16 MLP (bits as +-1; ``features`` adds the two diagonal-energy
-    inputs) trained with the library's MFMA kernels on the FP64 ground vectors
-    of ``n_train`` Hamiltonians, then used as the start on ``n_test`` held-out
-    ones. Every start is judged against the FP64 energy of plain LOBPCG from a
-    Gaussian start (fused complex128, 1e-10)."""
+    28-input BF16 MLP (bits as +-1) trained with the library's MFMA kernels on
+    the FP64 ground vectors of ``n_train`` Hamiltonians, then used as the start
+    on ``n_test`` held-out ones; with ``features`` also a 30-input network
+    (bits plus two diagonal-energy inputs) beside it. Every start is judged
+    against the FP64 energy of plain LOBPCG from a Gaussian start (fused
+    complex128, 1e-10)."""
     import multiprocessing as mp
     import torch
     from rqutils_amd.sqd import SQDProblem, start_vector
@@ -822,27 +827,34 @@ def c5_block(dev, n_train, n_test, steps, features=False):
             log(f"[netstart] {j + 1}/{n_train} ground vectors")
     sync()
     ground_s = time.time() - t
-    net = StartNet(28, device=dev, seed=0, n_extra=2 if features else 0)
+    net = StartNet(28, device=dev, seed=0)
     sync()
     t = time.time()
-    losses = net.train(keys, tgts, feats if features else None, steps=steps, lr=3e-3)
+    losses = net.train(keys, tgts, steps=steps, lr=3e-3)
     sync()
     train_s = time.time() - t
+    net_f = None
+    if features:
+        net_f = StartNet(28, device=dev, seed=0, n_extra=2)
+        losses_f = net_f.train(keys, tgts, feats, steps=steps, lr=3e-3)
+        sync()
     flop_per_sample = 2 * 3 * (net.kpad * 256 + 256 * 256 + 256 * 2)
     log(f"[netstart] {n_train} ground vectors ({ground_s:.1f} s); {steps} Adam steps "
         f"({train_s:.1f} s), loss {losses[0]:.4f} -> {losses[-1]:.4f}")
     del keys, tgts, feats
-    starts = {**C5_STARTS, **C5_UNGUARDED}
+    starts = {k: v for k, v in {**C5_STARTS, **C5_UNGUARDED}.items()
+              if features or v[0] != "net_features"}
     per = {k: {"iterations": [], "seconds": [], "energy_rel_err": [], "converged": 0,
-               "status4": 0} for k in starts}
+               "status4": 0, "overlap": []} for k in starts}
+    nets = {"net": net, "net_features": net_f}
     for w in ws[n_train:]:
         p = SQDProblem(w.hamiltonian, w.keys, 28, device=dev)
-        _, _, ref = ground_targets(p, seed=w.seed)
+        _, psi, ref = ground_targets(p, seed=w.seed)
         e_ref = ref.eigval
         for name, (st, mix, precond) in starts.items():
             sync()
             t0 = time.perf_counter()
-            x0 = start_vector(p.op, net if st == "net" else st, mix=mix, seed=w.seed)
+            x0 = start_vector(p.op, nets.get(st, st), mix=mix, seed=w.seed)
             _, inf = p.solve(x0, mode="fused", precision="c64", criterion="relative", tol=1e-6,
                              maxiter=3000, precond=precond)
             sync()
@@ -852,6 +864,8 @@ def c5_block(dev, n_train, n_test, steps, features=False):
             r["energy_rel_err"].append(abs(inf.eigval - e_ref) / abs(e_ref))
             r["converged"] += int(inf.converged)
             r["status4"] += int(inf.status == 4)
+            r["overlap"].append(abs(torch.vdot(psi, x0.to(psi.dtype)).item())
+                                / (psi.norm().item() * x0.norm().item()))
         log(f"[netstart] seed {w.seed}: " + ", ".join(
             f"{k} {per[k]['iterations'][-1]} it/{per[k]['energy_rel_err'][-1]:.0e}"
             for k in starts))
@@ -861,6 +875,8 @@ def c5_block(dev, n_train, n_test, steps, features=False):
            "train_s": round(train_s, 2), "adam_steps": steps,
            "train_tflops": round(flop_per_sample * 4096 * steps / train_s / 1e12, 3),
            "loss_first_last": [round(losses[0], 5), round(losses[-1], 5)],
+           "features_network_loss_first_last": None if net_f is None else
+           [round(losses_f[0], 5), round(losses_f[-1], 5)],
            "judge": "FP64 fused LOBPCG from a Gaussian start to 1e-10",
            "wrong_state_threshold": 1e-6, "guarded_starts": {}, "unguarded_for_comparison": {}}
     for k, r in per.items():
@@ -870,7 +886,8 @@ def c5_block(dev, n_train, n_test, steps, features=False):
                "seconds_mean": round(float(np.mean(r["seconds"])), 4),
                "energy_rel_err_max": float(errs.max()),
                "wrong_state": int(np.count_nonzero(errs > 1e-6)),
-               "converged": r["converged"], "invariant_start_status": r["status4"]}
+               "converged": r["converged"], "invariant_start_status": r["status4"],
+               "overlap_mean": round(float(np.mean(r["overlap"])), 4)}
         (out["guarded_starts"] if k in C5_STARTS else out["unguarded_for_comparison"])[k] = rec
     gs = out["guarded_starts"]
     for k, rec in list(gs.items()) + list(out["unguarded_for_comparison"].items()):
@@ -906,8 +923,8 @@ def main():
     ap.add_argument("--net-train", type=int, default=200, help="training Hamiltonians")
     ap.add_argument("--net-test", type=int, default=50, help="held-out Hamiltonians")
     ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps")
-    ap.add_argument("--net-features", action="store_true",
-                    help="network inputs: bits plus two diagonal-energy features")
+    ap.add_argument("--no-net-features", action="store_true",
+                    help="skip the 30-input network (bits plus two diagonal-energy features)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -929,7 +946,7 @@ def main():
             res, key = c4_block(args, dev), "c4_batch64"
         elif args.workload == "netstart":
             res, key = c5_block(dev, args.net_train, args.net_test, args.net_steps,
-                                args.net_features), "c5_network_start"
+                                not args.no_net_features), "c5_network_start"
         else:
             res, key = c1_block(dev), "c1_small16"
         print(json.dumps({"metric": METRIC, "value": None, "unit": None, "n_gpus": 1,
@@ -997,7 +1014,7 @@ def main():
             record["c4_batch64"] = c4_block(args, dev)
         if not args.no_net:
             record["c5_network_start"] = c5_block(dev, args.net_train, args.net_test,
-                                                  args.net_steps, args.net_features)
+                                                  args.net_steps, not args.no_net_features)
         if not args.no_svsim:
             record["svsim"] = svsim_block(dev, cpu=not args.no_cpu)
     print(json.dumps(record), flush=True)
