@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 15
+#define SQDK_ABI_VERSION 16
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -330,6 +330,70 @@ int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slice_rows, uint64
 int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision, const uint64_t* slice_base,
                           uint32_t* col, void* coef, void* diag, uint32_t* row_len,
                           void* stream);
+
+/* ---- operator functions on the reference's own arrays ------------------- *
+ * For callers that hold the reference's packed arrays. A packed padded row of
+ * row_bytes <= 8 bytes (rqutils/sqd.py:237; a packed X or Z signature likewise,
+ * rqutils/paulis/symplectic.py:117-124) is read as one big-endian integer, its
+ * "row key": row-key order is the reference's lexicographic row order (0xFF
+ * fill rows last), XOR and AND act on row keys exactly as on the packed bytes.
+ */
+
+/* keys[i] = big-endian integer of rows[i*row_bytes .. +row_bytes). */
+int sqdk_rows_to_keys(const uint8_t* rows, int64_t n, int32_t row_bytes, uint64_t* keys,
+                      void* stream);
+/* rows[i] = the big-endian bytes of keys[i] for i < n_keys, 0xFF bytes for
+ * n_keys <= i < n_rows: the fill rows of uniquify_states (sqd.py:441-445). */
+int sqdk_keys_to_rows(const uint64_t* keys, int64_t n_keys, int64_t n_rows, int32_t row_bytes,
+                      uint8_t* rows, void* stream);
+
+enum sqdk_coef_form {
+  SQDK_COEF_TERMS = 0, /* C_g from the group's Z signatures and coefficients (get_diagonal) */
+  SQDK_COEF_DIAG = 1,  /* C_g given per row (cache_level[1] == 2: diagonals) */
+  SQDK_COEF_SIGNS = 2  /* C_g from cached sign bits and the coefficients (cache_level[1] == 1) */
+};
+
+/* The reference's group arrays for one product (device pointers). */
+typedef struct sqdk_groups {
+  const uint64_t* keys;    /* [n] row keys of the states, sorted, fill rows (pad bit set) last */
+  int64_t n;               /* rows, fill rows included (< 2^31) */
+  int64_t n_real;          /* rows before the first fill row */
+  int32_t n_groups;        /* J */
+  int32_t coef_form;       /* sqdk_coef_form */
+  const uint64_t* xmask;   /* [J] X signatures as row keys: used when xsource is NULL */
+  const int32_t* xsource;  /* [J][ld] cached source indices (get_xsource) or NULL: each
+                              (row, group) target is found by a binary search of keys */
+  int64_t ld;              /* row stride of xsource, diag and signs (>= n) */
+  const uint64_t* zmask;   /* TERMS: [J][kmax] Z signatures as row keys */
+  const double* coef;      /* TERMS, SIGNS: [J][kmax] complex128 phased coefficients,
+                              zero-padded; zero terms are skipped (the reference stops at
+                              the first one, rqutils/sqd.py:562-564) */
+  int32_t kmax;
+  int32_t sign_bytes;      /* SIGNS: bytes per (group, row), >= kmax / 8 */
+  const double* diag;      /* DIAG: [J][ld] complex128 */
+  const uint8_t* signs;    /* SIGNS: [J][ld][sign_bytes], bit 7 - k % 8 of byte k / 8 */
+} sqdk_groups;
+
+/* out[i] = sum_g C_g(s_i) v[source_g(i)] in group order with FP64 sums; fill
+ * rows get 0. Replaces apply_h (xsource NULL, TERMS; rqutils/sqd.py:590-605),
+ * apply_h_s_cached (NULL, SIGNS; :608-622), apply_h_z_cached (NULL, DIAG;
+ * :625-637), apply_h_x_cached (given, TERMS; :640-654), apply_h_xs_cached
+ * (given, SIGNS; :657-670) and apply_h_xz_cached (given, DIAG; :673-684).
+ * v and out: n complex elements of `precision`. */
+int sqdk_groups_matvec(const sqdk_groups* g, int32_t precision, const void* v, void* out,
+                       void* stream);
+
+/* out[i*sign_bytes + k/8] bit 7 - k%8 = parity of zmask[k] & keys[i] for
+ * k < n_terms (one group's terms). Replaces get_diag_signs (sqd.py:506-528). */
+int sqdk_diag_signs(const uint64_t* zmask, int32_t n_terms, const uint64_t* keys, int64_t n,
+                    int32_t sign_bytes, uint8_t* out, void* stream);
+
+/* out[i] (complex128) = sum_k coef_k (1 - 2 bit_k(i)) over one group's terms.
+ * Replaces compute_diagonal (sqd.py:531-551) with the bit of term k read at
+ * 7 - k % 8 (the reference reads 7 - (k & 255): +1 for every term from the
+ * ninth on). */
+int sqdk_compute_diagonal(const uint8_t* signs, int64_t n, int32_t sign_bytes, const double* coef,
+                          int32_t n_terms, double* out, void* stream);
 
 /* ---- single-vector LOBPCG ---------------------------------------------- */
 

@@ -17,6 +17,7 @@ __all__ = [
     "from_paulisum", "pack_states", "uniquify_states", "get_xsource",
     "xsource_by_lookup", "get_diagonal", "apply_xgrp", "apply_h_x_cached",
     "apply_h", "apply_h_rows", "vinit_from_min_diag", "hproj_coo", "packed_to_keys",
+    "get_diag_signs", "compute_diagonal",
     "PHASES",
 ]
 
@@ -131,6 +132,42 @@ def xsource_by_lookup(xsig, states):
 
 
 _POPCOUNT8 = np.array([bin(i).count("1") for i in range(256)], dtype=np.uint8)
+
+
+def get_diag_signs(zsigs, states):
+    """Packed sign bits of one group's terms -- rqutils/sqd.py:506-528.
+
+    Term k's parity of ``states & z_k`` goes to bit ``7 - k % 8`` of byte
+    ``k // 8`` (bits and bytes counted from the left). Returns
+    ``uint8[N, ceil(K/8)]``.
+    """
+    k = zsigs.shape[0]
+    out = np.zeros((states.shape[0], -(-k // 8)), dtype=np.uint8)
+    for t in range(k):
+        par = (np.bitwise_count(states & zsigs[t]).sum(axis=1) & 1).astype(np.uint8)
+        out[:, t // 8] += par << np.uint8(7 - t % 8)
+    return out
+
+
+def compute_diagonal(diag_signs, coeffs, reference_bit_index=False):
+    """Diagonal from sign bits and coefficients -- rqutils/sqd.py:531-551.
+
+    Stops at the first zero coefficient as the reference does. The reference
+    reads term k's bit at ``7 - (k & 255)`` (sqd.py:544), which for k >= 8
+    shifts by a negative amount; XLA's shift then yields 0, i.e. a +1 sign
+    (quirk Q1). ``reference_bit_index=True`` reproduces that; the default
+    reads ``7 - k % 8``, the bit ``get_diag_signs`` wrote.
+    """
+    out = np.zeros(diag_signs.shape[0], dtype=np.result_type(coeffs.dtype, np.float64))
+    for t in range(coeffs.shape[0]):
+        if coeffs[t] == 0:
+            break
+        if reference_bit_index and t >= 8:
+            bit = np.zeros(diag_signs.shape[0], dtype=np.int64)
+        else:
+            bit = (diag_signs[:, t // 8] >> (7 - t % 8)) & 1
+        out = out + coeffs[t] * (1.0 - 2.0 * bit)
+    return out
 
 
 def get_diagonal(zsigs, coeffs, states, stop_at_zero=False):

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 15
+ABI_VERSION = 16
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -93,6 +93,19 @@ class SqdkLocgResult(C.Structure):
         ("eigval", C.c_double), ("resid_norm", C.c_double), ("ax_norm", C.c_double),
         ("niter", C.c_int32), ("converged", C.c_int32), ("matvecs", C.c_int32),
         ("status", C.c_int32),
+    ]
+
+
+COEF_TERMS, COEF_DIAG, COEF_SIGNS = 0, 1, 2
+
+
+class SqdkGroups(C.Structure):
+    _fields_ = [
+        ("keys", C.c_void_p), ("n", C.c_int64), ("n_real", C.c_int64),
+        ("n_groups", C.c_int32), ("coef_form", C.c_int32), ("xmask", C.c_void_p),
+        ("xsource", C.c_void_p), ("ld", C.c_int64), ("zmask", C.c_void_p),
+        ("coef", C.c_void_p), ("kmax", C.c_int32), ("sign_bytes", C.c_int32),
+        ("diag", C.c_void_p), ("signs", C.c_void_p),
     ]
 
 
@@ -189,6 +202,15 @@ SIGNATURES = {
     "sqdk_adam": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                             C.c_void_p, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int32,
                             C.c_void_p]),
+    "sqdk_rows_to_keys": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]),
+    "sqdk_keys_to_rows": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.c_int32, C.c_void_p,
+                                    C.c_void_p]),
+    "sqdk_groups_matvec": (C.c_int, [C.POINTER(SqdkGroups), C.c_int32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
+    "sqdk_diag_signs": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_int32,
+                                  C.c_void_p, C.c_void_p]),
+    "sqdk_compute_diagonal": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_int32,
+                                        C.c_void_p, C.c_void_p]),
     "sqdk_svsim": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int64,
                              C.c_void_p, C.c_void_p]),
 }

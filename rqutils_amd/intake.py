@@ -4,10 +4,12 @@ The reference packs sampled state rows behind a pad bit, sorts them with a
 multi-key ``lax.sort`` and removes duplicates (``uniquify_states``), filling
 the tail with 0xFF rows; ``sqd`` unpacks the result into ``basis_states``.
 Here rows become uint64 keys on the device (``pack_states``), an occupancy
-bitmap (keys of at most 36 bits) or a stable radix sort plus compaction gives
-the sorted distinct keys (``unique_keys``; the fill rows are not
-materialised: ``states_size`` only enters the stopping rule), and
-``unpack_keys`` turns keys back into uint8 rows. All three run in the HIP
+bitmap (keys of at most 28 bits whose bitmap is small enough) or a stable
+radix sort plus compaction gives the sorted distinct keys (``unique_keys``;
+``sqd`` does not materialise the fill rows: ``states_size`` only enters the
+stopping rule), and ``unpack_keys`` turns keys back into uint8 rows. The
+reference-signature ``uniquify_states(states_p, states_size)`` on packed rows,
+fill rows included, is in :mod:`rqutils_amd.sqd_arrays`. All three run in the HIP
 library (``sqdk_intake_*``) and are bit-exact.
 """
 from __future__ import annotations
@@ -19,7 +21,7 @@ import torch
 
 from . import _lib
 
-__all__ = ["pack_states", "unique_keys", "uniquify_states", "unpack_keys"]
+__all__ = ["pack_states", "unique_keys", "unique_state_keys", "unpack_keys"]
 
 
 def _dev(device):
@@ -78,9 +80,9 @@ def unique_keys(keys: torch.Tensor, key_bits: int = 64) -> torch.Tensor:
     return out[:m]
 
 
-def uniquify_states(states, device=None) -> torch.Tensor:
-    """Sorted distinct keys of sampled state rows (the distinct rows of
-    ``uniquify_states``, rqutils/sqd.py:414-445)."""
+def unique_state_keys(states, device=None) -> torch.Tensor:
+    """Sorted distinct keys of sampled binary state rows ``[N, n]`` (the
+    distinct rows of ``uniquify_states``, rqutils/sqd.py:414-445)."""
     nq = states.shape[1]
     return unique_keys(pack_states(states, device), nq)
 

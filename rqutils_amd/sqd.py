@@ -26,12 +26,14 @@ from scipy.sparse import coo_array, csr_array
 
 from . import _lib
 from .ground_locg import STATUS_INVARIANT, solve
-from .intake import uniquify_states, unpack_keys
+from .intake import unique_state_keys, unpack_keys
 from .operator import PauliOperator, Subspace, keys_to_states, states_to_keys
 from .paulis.symplectic import PauliSumXZ, _bits_to_u64, unpack_rows
 
 __all__ = ["sqd", "hproj", "get_xsource", "get_diagonal", "SQDProblem", "start_vector",
-           "vinit_index"]
+           "vinit_index", "uniquify_states", "get_diag_signs", "compute_diagonal", "apply_xgrp",
+           "apply_h", "apply_h_s_cached", "apply_h_z_cached", "apply_h_x_cached",
+           "apply_h_xs_cached", "apply_h_xz_cached"]
 
 LOG = logging.getLogger(__name__)
 
@@ -225,7 +227,7 @@ def sqd(hamiltonian, states, states_size: Optional[int] = None, return_eigvec: b
         hamiltonian = PauliSumXZ.from_paulisum(hamiltonian, force_real=True, add_padding=True)
     num_qubits = states.shape[1]
     # pack + sort + unique on the device (the distinct rows of uniquify_states)
-    keys = uniquify_states(states, device)
+    keys = unique_state_keys(states, device)
     t_start = time.time()
     prob = SQDProblem(hamiltonian, keys, num_qubits, device=device,
                       cache="auto" if cache_level[0] == 1 else False)
@@ -367,3 +369,10 @@ def get_diagonal(zsignatures: np.ndarray, coeffs: np.ndarray, states: np.ndarray
                                         keys.data_ptr(), n, out.data_ptr(), _lib.stream_ptr()),
                "sqdk_diagonal")
     return out
+
+
+# The operator functions on the reference's own packed arrays (rqutils/sqd.py:
+# 414-684), importable from here as from the reference's module.
+from .sqd_arrays import (  # noqa: E402,F401
+    apply_h, apply_h_s_cached, apply_h_x_cached, apply_h_xs_cached, apply_h_xz_cached,
+    apply_h_z_cached, apply_xgrp, compute_diagonal, get_diag_signs, uniquify_states)

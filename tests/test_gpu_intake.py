@@ -16,7 +16,7 @@ import pytest
 import torch
 
 from rqutils_amd import _lib
-from rqutils_amd.intake import pack_states, unique_keys, uniquify_states, unpack_keys
+from rqutils_amd.intake import pack_states, unique_keys, unique_state_keys, unpack_keys
 from rqutils_amd.operator import keys_to_states, states_to_keys
 
 pytestmark = pytest.mark.gpu
@@ -83,7 +83,7 @@ def test_unpack_and_uniquify_roundtrip(gpu):
     nq = 23
     base = rng.integers(0, 2, size=(5000, nq)).astype(np.uint8)
     states = base[rng.integers(0, 5000, size=20000)]
-    keys = uniquify_states(states, gpu)
+    keys = unique_state_keys(states, gpu)
     ref = np.unique(states_to_keys(states))
     assert np.array_equal(_u64(keys), ref)
     rows = unpack_keys(keys, nq).cpu().numpy()

@@ -537,3 +537,26 @@ def test_c_restatement_matches_numpy_oracle(seed, full):
     part = sum(so.apply_xgrp(p.xsources[g], so.get_diagonal(p.z[g], p.c[g], p.packed), v)
                for g in sub)
     assert np.linalg.norm(got - part) <= 1e-12 * np.linalg.norm(part)
+
+
+@pytest.mark.parametrize("seed", [2, 3])
+def test_sign_bit_cache_composes_the_diagonal(seed):
+    """get_diag_signs then compute_diagonal gives get_diagonal for groups of
+    more than 8 terms (an independent path through the sign bits); with the
+    reference's bit index (quirk Q1) it differs exactly when a term from the
+    ninth on has an odd parity, and agrees for groups of at most 8 terms."""
+    p = small_problem(seed, n=9, terms=300, nstates=250)
+    kmax = p.c.shape[1]
+    assert kmax > 8
+    seen_diff = False
+    for g in range(p.x.shape[0]):
+        signs = so.get_diag_signs(p.z[g], p.packed)
+        d = so.get_diagonal(p.z[g], p.c[g], p.packed)
+        assert np.allclose(so.compute_diagonal(signs, p.c[g]), d, rtol=0, atol=1e-12)
+        dq = so.compute_diagonal(signs, p.c[g], reference_bit_index=True)
+        nterms = int(np.count_nonzero(p.c[g]))
+        if nterms <= 8:
+            assert np.allclose(dq, d, rtol=0, atol=1e-12)
+        elif not np.allclose(dq, d):
+            seen_diff = True
+    assert seen_diff
