@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 16
+#define SQDK_ABI_VERSION 17
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -431,6 +431,19 @@ typedef struct sqdk_locg_opts {
   int32_t refresh_every;   /* FUSED: recompute A x and A p explicitly every k iterations */
   int32_t use_graph;       /* capture check_every iterations as one HIP graph */
   int32_t precond;         /* sqdk_locg_precond (FUSED only) */
+  int32_t reserved;
+  double* debug;           /* REFERENCE only, or NULL: device buffer of (maxiter + 2) x 30
+                              doubles that receives one diagnostics record after the
+                              first step, after the 2x2 step and after every loop
+                              iteration (ground_locg.py:211-233, 264-286, 359-364):
+                              theta, rho = Re <x, A x>, reltol = (|Ax| - theta) 10 N,
+                              converged (0/1), |r|, 0, kappa[3] (complex), then S^H A S
+                              of (x, y, r) (3x3 complex, row-major), at the cost of
+                              three more products per record. As in the reference's
+                              diagnostics mode every one of maxiter iterations runs;
+                              the convergence test is recorded, not acted on */
+  void* debug_vectors;     /* or NULL (needs debug): (maxiter + 2) x 3 x n elements of the
+                              solver precision receiving x, y, r at every record */
 } sqdk_locg_opts;
 
 typedef struct sqdk_locg_result {

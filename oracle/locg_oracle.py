@@ -105,8 +105,18 @@ def project_out(basis, vector):
     return vector * (1.0 if _norm(vector) >= 0.99 else 0.0)
 
 
+def _diagnostics(matvec, x, y, r, theta, kappa=None, reltol=0.0, converged=False):
+    """One diagnostics record -- ground_locg.py:211-233: S^H A S of (x, y, r),
+    rho = Re <x, A x>, kappa (zeros when absent), reltol, converged."""
+    sas = compute_sas(matvec, x, y, r)
+    rho = float(np.real(_vdot(x, matvec(x))))
+    return {"x": x, "y": y, "r": r, "theta": float(np.real(theta)), "rho": rho,
+            "kappa": np.zeros(3, complex) if kappa is None else np.asarray(kappa, complex),
+            "sas": sas, "reltol": float(reltol), "converged": bool(converged)}
+
+
 def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=None,
-                rel_tol=None):
+                rel_tol=None, debug=False):
     """Single-vector LOBPCG -- ground_locg.py:191-377 (callable path).
 
     ``n_for_tol`` is the vector length used in the convergence threshold
@@ -114,7 +124,9 @@ def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=N
     If ``history`` is a list, ``(niter, theta, |r|, reltol)`` tuples are
     appended once per loop iteration. ``rel_tol`` replaces the reference's
     stopping rule by ``|r| <= rel_tol |theta|`` (the benchmark's residual).
-    Returns ``(theta, x, niter)``.
+    Returns ``(theta, x, niter)``; with ``debug`` the reference's diagnostics
+    mode (ground_locg.py:358-376): every one of ``maxiter`` iterations runs
+    and a list of per-step records (``_diagnostics``) is returned fourth.
     """
     x = np.array(xinit, dtype=np.result_type(xinit.dtype, np.float64))
     if tol is None:
@@ -126,6 +138,7 @@ def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=N
     ax = matvec(x)
     rho = np.real(_vdot(x, ax))
     r = ax - rho * x
+    records = [_diagnostics(matvec, x, np.zeros_like(x), r, rho)] if debug else None
 
     # body_iter1 -- ground_locg.py:274-286
     nr = _norm(r)
@@ -137,10 +150,13 @@ def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=N
     yn = t / _norm(t)
     rn = matvec(xn) - theta * xn
     x, y, r = xn, yn, rn
+    if debug:
+        records.append(_diagnostics(matvec, x, y, r, theta, np.insert(kappa, 1, 0.0)))
 
-    # while_loop over body -- ground_locg.py:288-341, 366-370
+    # while_loop over body -- ground_locg.py:288-341, 366-370 (a scan over
+    # maxiter steps in the diagnostics mode, :358-364)
     niter, converged = 0, False
-    while niter < maxiter and not converged:
+    while niter < maxiter and (debug or not converged):
         p = project_out((x, y), r)
         theta, kappa = eigenpair_3x3(compute_sas(matvec, x, y, p))
         s = y * kappa[1] + p * kappa[2]
@@ -159,4 +175,8 @@ def ground_locg(matvec, xinit, maxiter=1000, tol=None, n_for_tol=None, history=N
         x, y, r = xn, yn, rn
         if history is not None:
             history.append((niter, float(np.real(theta)), nrn, float(reltol)))
+        if debug:
+            records.append(_diagnostics(matvec, x, y, r, theta, kappa, reltol, converged))
+    if debug:
+        return float(np.real(theta)), x, niter, records
     return float(np.real(theta)), x, niter

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 16
+ABI_VERSION = 17
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -84,7 +84,8 @@ class SqdkLocgOpts(C.Structure):
         ("mode", C.c_int32), ("precision", C.c_int32), ("criterion", C.c_int32),
         ("maxiter", C.c_int32), ("tol", C.c_double), ("n_for_tol", C.c_int64),
         ("check_every", C.c_int32), ("refresh_every", C.c_int32), ("use_graph", C.c_int32),
-        ("precond", C.c_int32),
+        ("precond", C.c_int32), ("reserved", C.c_int32), ("debug", C.c_void_p),
+        ("debug_vectors", C.c_void_p),
     ]
 
 

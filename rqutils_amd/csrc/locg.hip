@@ -56,7 +56,13 @@ struct ScalarParams {
   int64_t n;       // vector length of the solve
   int criterion;
   int precision;
+  double2* dbg;    // REFERENCE with diagnostics: kDbgRecord double2 per record, else NULL
 };
+
+// One diagnostics record of the reference iteration (the dict of
+// ground_locg.py:211-233): (theta, rho), (reltol, converged), (|r|, 0),
+// kappa[3], then S^H A S of (x, y, r) row-major.
+constexpr int kDbgRecord = 15;
 
 // Status flags (sqdk_locg_result.status).
 constexpr int kStatusBreakdown = 2;    // Rayleigh-Ritz breakdown (fused)
@@ -91,6 +97,7 @@ enum Step : int {
   kStepRefreshRho,     // fused refresh: theta = Re D1 / D0, C0 = -theta
   kStepVerify,         // D0 = |Ax|^2, D1 = |r|^2 after refresh: info only
   kStepZeroResid,      // D0 = |r|^2 of the start: exactly 0 -> converged at rho
+  kStepDebug,          // D0..D5 = S^H A S of (x, y, r) as in kStepEig3: write record
 };
 
 __device__ bool criterion_met(const ScalarParams& p, double rnorm, double axnorm, double theta,
@@ -110,7 +117,7 @@ __device__ bool criterion_met(const ScalarParams& p, double rnorm, double axnorm
 }
 
 __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __restrict__ F,
-                              ScalarParams prm, int use_skip) {
+                              ScalarParams prm, int use_skip, int arg) {
   if (threadIdx.x != 0) return;
   if (use_skip && F[kConv] != 0) return;
   double2* D = S + kD;
@@ -245,7 +252,10 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       const double axn = sqrt(D[0].x), rn = sqrt(D[1].x);
       const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S);
       F[kNiter] += 1;
-      if (conv) F[kConv] = 1;
+      // with diagnostics every iteration runs, as the reference's scan over
+      // maxiter steps does (ground_locg.py:358-364); the flag is recorded
+      S[kInfo2].y = conv ? 1.0 : 0.0;
+      if (conv && !prm.dbg) F[kConv] = 1;
       // The reference iterates on NaN until maxiter; a non-finite residual,
       // norm or theta ends the solve with status 3 instead.
       if (!isfinite(axn) || !isfinite(rn) || !isfinite(S[kTheta].x)) stop_nonfinite(F);
@@ -292,6 +302,34 @@ __global__ void scalar_kernel(int step, double2* __restrict__ S, int32_t* __rest
       // a zero residual is an exact eigenpair even where the threshold is 0
       const bool conv = criterion_met(prm, rn, axn, S[kTheta].x, S) || rn == 0.0;
       F[kConv] = conv ? 1 : 0;
+      break;
+    }
+    case kStepDebug: {
+      // arg = record index; kappa of the record: kKappa..+2 (zeros for the
+      // first record, (k0, 0, k1) after the 2x2 step)
+      double2* r = prm.dbg + (int64_t)arg * kDbgRecord;
+      const double theta = S[kTheta].x;
+      r[0] = make_double2(theta, D[3].x);
+      const double reltol = arg >= 2 ? (S[kInfo].y - theta) * prm.nlen * 10.0 : 0.0;
+      r[1] = make_double2(reltol, arg >= 2 ? S[kInfo2].y : 0.0);
+      r[2] = make_double2(arg >= 2 ? S[kInfo].x : 0.0, 0.0);
+      for (int i = 0; i < 3; ++i)
+        r[3 + i] = arg == 0 ? make_double2(0.0, 0.0)
+                   : arg == 1 ? (i == 0 ? S[kKappa] : i == 1 ? make_double2(0.0, 0.0)
+                                                             : S[kKappa + 1])
+                              : S[kKappa + i];
+      double2 m[3][3];
+      m[1][0] = D[0];
+      m[2][0] = D[1];
+      m[2][1] = D[2];
+      m[0][1] = cconj(D[0]);
+      m[0][2] = cconj(D[1]);
+      m[1][2] = cconj(D[2]);
+      m[0][0] = D[3];
+      m[1][1] = D[4];
+      m[2][2] = D[5];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[6 + 3 * i + j] = m[i][j];
       break;
     }
     default:
@@ -390,8 +428,8 @@ struct Solver {
     return lincomb_impl(n, prec, out, k, ins, ci, reinterpret_cast<const double*>(S), s,
                         skip ? F + kConv : nullptr);
   }
-  int scalar(int step, bool skip) {
-    scalar_kernel<<<1, 64, 0, s>>>(step, S, F, prm, skip ? 1 : 0);
+  int scalar(int step, bool skip, int arg = 0) {
+    scalar_kernel<<<1, 64, 0, s>>>(step, S, F, prm, skip ? 1 : 0, arg);
     SQDK_LAUNCHED("scalar_kernel");
     return SQDK_OK;
   }
@@ -416,6 +454,25 @@ int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_h
   char* ax = sv.vec[3];
   char* ay = sv.vec[4];
   char* ap = sv.vec[5];
+  // Diagnostics record k (ground_locg.py:211-233): S^H A S of (x, y, r) from
+  // three extra products into the scratch vectors ax, ay, ap (free at every
+  // record point), then optionally the three vectors themselves.
+  const size_t vb = (size_t)sv.n * elem_bytes(sv.prec);
+  auto record = [&](int k, bool skip) -> int {
+    if (!sv.prm.dbg) return SQDK_OK;
+    SQDK_TRY(sv.apply(x, ax, skip));
+    SQDK_TRY(sv.apply(y, ay, skip));
+    SQDK_TRY(sv.apply(r, ap, skip));
+    SQDK_TRY(sv.dots({{y, ax}, {r, ax}, {r, ay}, {x, ax}, {y, ay}, {r, ap}}, skip));
+    SQDK_TRY(sv.scalar(kStepDebug, skip, k));
+    if (o.debug_vectors) {
+      char* dst = static_cast<char*>(o.debug_vectors) + (size_t)k * 3 * vb;
+      SQDK_HIP(hipMemcpyAsync(dst, x, vb, hipMemcpyDeviceToDevice, sv.s));
+      SQDK_HIP(hipMemcpyAsync(dst + vb, y, vb, hipMemcpyDeviceToDevice, sv.s));
+      SQDK_HIP(hipMemcpyAsync(dst + 2 * vb, r, vb, hipMemcpyDeviceToDevice, sv.s));
+    }
+    return SQDK_OK;
+  };
   // xinit /= |xinit|  (ground_locg.py:346)
   SQDK_TRY(sv.dots({{x, x}}, false));
   SQDK_TRY(sv.scalar(kStepInitNorm, false));
@@ -425,6 +482,10 @@ int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_h
   SQDK_TRY(sv.dots({{x, ax}}, false));
   SQDK_TRY(sv.scalar(kStepRho, false));
   SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, false));
+  if (sv.prm.dbg) {
+    SQDK_TRY(sv.lin(y, {x}, {kZero}, false));   // y = 0 in the first record
+    SQDK_TRY(record(0, false));
+  }
   // body_iter1 (:274-286); skipped when the start is an exact eigenvector
   SQDK_TRY(sv.dots({{r, r}}, false));
   SQDK_TRY(sv.scalar(kStepZeroResid, false));
@@ -443,8 +504,10 @@ int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_h
   SQDK_TRY(sv.apply(x, ax, true));
   SQDK_TRY(sv.scalar(kStepNegTheta, true));
   SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, true));
+  SQDK_TRY(record(1, true));
 
   int done = 0;
+  int rec = 2;
   while (done < o.maxiter) {
     const int chunk = std::min(o.check_every, o.maxiter - done);
     for (int it = 0; it < chunk; ++it) {
@@ -489,6 +552,7 @@ int run_reference(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_h
       SQDK_TRY(sv.lin(r, {ax, x}, {kOne, C0}, true));
       SQDK_TRY(sv.dots({{ax, ax}, {r, r}}, true));
       SQDK_TRY(sv.scalar(kStepConv, true));
+      SQDK_TRY(record(rec++, true));
       if (sv.op->fn && it + 1 < chunk) break;  // callbacks cannot be skipped on device
     }
     SQDK_TRY(sv.read_flags(flags_host));
@@ -633,6 +697,9 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   SQDK_REQUIRE(opts->precond == SQDK_PRECOND_NONE ||
                    (opts->mode == SQDK_LOCG_FUSED && !op->fn),
                "the diagonal preconditioner needs the FUSED mode and the built-in operator");
+  SQDK_REQUIRE(!opts->debug || opts->mode == SQDK_LOCG_REFERENCE,
+               "diagnostics are recorded by the REFERENCE mode only");
+  SQDK_REQUIRE(!opts->debug_vectors || opts->debug, "debug_vectors needs debug");
   size_t need = 0;
   SQDK_TRY(sqdk_locg_workspace(op, n, opts, &need));
   SQDK_REQUIRE(work_bytes >= need, "workspace of %zu bytes, %zu needed", work_bytes, need);
@@ -671,6 +738,7 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   sv.prm.tol = o.tol;
   sv.prm.nlen = (double)(o.n_for_tol > 0 ? o.n_for_tol : n);
   sv.prm.n = n;
+  sv.prm.dbg = reinterpret_cast<double2*>(o.debug);
   sv.prm.criterion = o.criterion;
   sv.prm.precision = o.precision;
 

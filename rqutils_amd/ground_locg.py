@@ -57,6 +57,26 @@ class LocgInfo:
     resid_norm: float      # |r| as tracked by the solver (explicit at the end for "fused")
     ax_norm: float
     status: int
+    diagnostics: dict | None = None   # per-step records of a debug solve
+
+
+def diagnostics_dict(buf: torch.Tensor, count: int, vectors=None, real: bool = False) -> dict:
+    """The reference's diagnostics (rqutils/ground_locg.py:211-233) from the
+    device records: arrays over the first step, the 2x2 step and each loop
+    iteration: theta, rho, kappa [3], sas [3, 3] (S^H A S of x, y, r), reltol,
+    converged, and the residual norm; with ``vectors`` also x, y, r."""
+    rec = buf.cpu().numpy().reshape(-1, 15, 2)[:count]
+    cplx = rec[..., 0] + 1j * rec[..., 1]
+    out = {"theta": rec[:, 0, 0].copy(), "rho": rec[:, 0, 1].copy(),
+           "reltol": rec[:, 1, 0].copy(), "converged": rec[:, 1, 1] != 0,
+           "resid_norm": rec[:, 2, 0].copy(), "kappa": cplx[:, 3:6].copy(),
+           "sas": cplx[:, 6:15].reshape(-1, 3, 3).copy()}
+    if vectors is not None:
+        v = vectors[:count].cpu().numpy()
+        if real:
+            v = v.real
+        out["x"], out["y"], out["r"] = v[:, 0], v[:, 1], v[:, 2]
+    return out
 
 
 def _precision_of(dtype: torch.dtype) -> int:
@@ -110,7 +130,8 @@ class _Callback:
 def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = None,
           criterion: str = "relative", tol: float = 1e-6, maxiter: int = 1000,
           n_for_tol: int | None = None, check_every: int = 4, refresh_every: int = 16,
-          real: bool = False, precond: str | None = None) -> tuple[torch.Tensor, LocgInfo]:
+          real: bool = False, precond: str | None = None,
+          debug: bool | str = False) -> tuple[torch.Tensor, LocgInfo]:
     """Lowest eigenpair of ``op`` from start vector ``x0`` (device tensor).
 
     ``op`` is a PauliOperator or a callable on device tensors. Returns the
@@ -119,6 +140,11 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     residual scaled by ``1 / max(d - min(theta, min d), floor)``, ``d`` the
     diagonal from the operator's X = 0 terms and ``floor = max(0.02 (mean d -
     min d), 1e-3 |theta|)``.
+
+    ``debug`` (reference mode): record the reference's per-step diagnostics
+    (``LocgInfo.diagnostics``, see ``diagnostics_dict``); ``"vectors"`` also
+    keeps x, y and r of every step. Every one of ``maxiter`` iterations then
+    runs, as in the reference's diagnostics scan.
     """
     L = _lib.lib()
     if mode not in _MODES:
@@ -143,6 +169,15 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     if precond not in _PRECONDS:
         raise ValueError(f"precond must be one of {sorted(_PRECONDS, key=str)}")
     opts.precond = _PRECONDS[precond]
+    dbg = dbg_vec = None
+    if debug:
+        if mode != "reference":
+            raise ValueError("debug diagnostics are recorded by mode='reference' only")
+        dbg = torch.zeros((maxiter + 2) * 30, dtype=torch.float64, device=x.device)
+        opts.debug = dbg.data_ptr()
+        if debug == "vectors":
+            dbg_vec = torch.zeros((maxiter + 2, 3, n), dtype=x.dtype, device=x.device)
+            opts.debug_vectors = dbg_vec.data_ptr()
     if opts.precond and (mode != "fused" or not isinstance(op, PauliOperator)):
         raise ValueError("precond='diag' needs mode='fused' and a PauliOperator")
     cop = _lib.SqdkOp()
@@ -180,6 +215,8 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     _lib.check(rc, "sqdk_locg_solve")
     info = LocgInfo(res.eigval, res.niter, bool(res.converged), res.matvecs, res.resid_norm,
                     res.ax_norm, res.status)
+    if dbg is not None:
+        info.diagnostics = diagnostics_dict(dbg, res.niter + 2, dbg_vec, real)
     if info.status == STATUS_NONFINITE:
         LOG.warning("LOBPCG stopped at iteration %d: NaN or Inf in the operator or vectors",
                     info.niter)
@@ -196,7 +233,7 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
                 vspace=None, log_level: int = logging.WARNING, *, mode: str = "reference",
                 precision: str | None = None, criterion: str = "reference",
                 check_every: int | None = None, refresh_every: int = 16,
-                device=None, precond: str | None = None):
+                device=None, precond: str | None = None, debug: bool | str = False):
     r"""Single-vector LOBPCG (rqutils/ground_locg.py:128-157).
 
     Args:
@@ -216,6 +253,11 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
         precond: ``"diag"`` for the Jacobi-preconditioned fused iteration
             (``mode="fused"`` with a ``PauliOperator``; the reference has no
             preconditioner).
+        debug: Record the reference's diagnostics (its internal
+            ``_ground_locg_callable(debug=True)``, ground_locg.py:211-233,
+            358-376): every one of ``maxiter`` iterations runs and a fourth
+            result, a dict of per-step arrays, is returned; ``"vectors"`` adds
+            x, y and r of every step.
 
     Returns:
         ``(eigval, eigvec, niter)``: the Rayleigh-Ritz value as a float, the
@@ -288,12 +330,14 @@ def ground_locg(mat, xinit, args: tuple = (), maxiter: int = 1000, tol: float | 
         check_every = 4
     vec, info = solve(op, x0, mode=mode, precision=precision, criterion=criterion, tol=tol,
                       maxiter=maxiter, check_every=check_every, refresh_every=refresh_every,
-                      real=real, precond=precond)
+                      real=real, precond=precond, debug=debug)
     if log_level <= logging.DEBUG:
         LOG.debug("LOBPCG: eigval %s after %d iterations (converged=%s, |r|=%g)",
                   info.eigval, info.niter, info.converged, info.resid_norm)
     if real:
         vec = vec.real.contiguous()
+    if debug:
+        return info.eigval, vec, info.niter, info.diagnostics
     return info.eigval, vec, info.niter
 
 

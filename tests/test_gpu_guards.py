@@ -127,8 +127,8 @@ def _isolated_min_diag_problem():
 def test_invariant_start_is_flagged_and_restarted(gpu):
     """The reference's default one-hot on an isolated state has a zero
     residual: the device solve stops with status 4 (not a ground state),
-    sqd() solves again from a Gaussian start by default and returns NaN, as
-    the reference does, with restart=None."""
+    sqd() solves again with a Gaussian part added by default and returns NaN,
+    as the reference does, with restart=None."""
     from rqutils_amd.ground_locg import STATUS_INVARIANT
     from rqutils_amd.operator import keys_to_states
     from rqutils_amd.sqd import SQDProblem, sqd
@@ -163,3 +163,41 @@ def test_sqd_judge_keeps_the_lower_energy(gpu):
     ev = sqd((p.labels, p.coeffs), keys_to_states(p.keys, p.n), return_eigvec=False,
              judge=True, info=out)
     assert abs(ev - e0) < 1e-9 * abs(e0) and "judged" in out
+
+
+def test_debug_diagnostics_match_oracle(gpu):
+    """ground_locg(debug=...) records the reference's diagnostics
+    (rqutils/ground_locg.py:211-233, 358-376) on the device: the first step,
+    the 2x2 step and every one of maxiter iterations, against the oracle's
+    restatement on the same operator and start."""
+    from rqutils_amd.ground_locg import ground_locg
+    from tests.helpers import small_problem
+    p = small_problem(17, n=10, terms=70, nstates=400)
+    op = _problem_op(p)
+    rng = np.random.default_rng(3)
+    x0 = rng.normal(size=p.keys.size) + 1j * rng.normal(size=p.keys.size)
+    maxiter = 12
+    th_o, _, it_o, recs = lo.ground_locg(p.matvec, x0, maxiter=maxiter, debug=True)
+    ev, vec, it, diag = ground_locg(op, torch.from_numpy(x0).cuda(), maxiter=maxiter,
+                                    debug="vectors")
+    assert it == it_o == maxiter and len(recs) == maxiter + 2
+    assert abs(ev - th_o) < 1e-10 * abs(th_o)
+    for k, rec in enumerate(recs):
+        scale = abs(rec["theta"]) + 1.0
+        assert abs(diag["theta"][k] - rec["theta"]) < 1e-9 * scale, k
+        assert abs(diag["rho"][k] - rec["rho"]) < 1e-9 * scale, k
+        assert np.abs(diag["sas"][k] - rec["sas"]).max() < 1e-8 * scale, k
+        assert abs(diag["reltol"][k] - rec["reltol"]) <= 1e-8 * max(1.0, abs(rec["reltol"])), k
+        assert bool(diag["converged"][k]) == rec["converged"], k
+        # kappa and the vectors up to the rounding the two iterations accumulate
+        assert np.abs(np.abs(diag["kappa"][k]) - np.abs(rec["kappa"])).max() < 1e-6, k
+        for name in ("x", "r"):
+            a, b = diag[name][k], rec[name]
+            assert np.linalg.norm(a - b) <= 1e-6 * max(1e-12, np.linalg.norm(b)) + 1e-12, (k, name)
+    with pytest.raises(ValueError):
+        ground_locg(op, torch.from_numpy(x0).cuda(), maxiter=3, debug=True, mode="fused")
+
+
+def _problem_op(p):
+    from rqutils_amd.sqd import SQDProblem
+    return SQDProblem((p.labels, p.coeffs), p.keys, p.n).op
