@@ -90,10 +90,17 @@ def test_products_on_reference_arrays(gpu, n, full):
 
 
 def test_real_vectors_stay_real(gpu):
+    """Real coefficients (no Y) and a real vector give a real product, as the
+    reference's do."""
     from rqutils_amd import sqd as S
-    p, rows, su, nu, rng = _case(9, 5, terms=40)
-    if np.iscomplexobj(p.c):
-        pytest.skip("this seed drew an odd-Y term")
+    rng = np.random.default_rng(5)
+    n = 9
+    labels = ["".join("IXZ"[i] for i in rng.integers(0, 3, n)) for _ in range(40)]
+    coeffs = rng.normal(size=40)
+    keys = np.sort(rng.choice(2 ** n, size=300, replace=False)).astype(np.uint64)
+    p = OracleProblem(labels, coeffs, keys, n)
+    assert not np.iscomplexobj(p.c)
+    su, nu = so.uniquify_states(p.packed, p.packed.shape[0] + 5)
     v = rng.normal(size=su.shape[0])
     v[nu:] = 0
     out = S.apply_h(torch.from_numpy(v).cuda(), p.x, p.z, p.c, su)
