@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 17
+#define SQDK_ABI_VERSION 18
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -274,6 +274,17 @@ int sqdk_dots(int64_t n, int32_t precision, int32_t npairs,
               const void* const* a, const void* const* b, double* out,
               void* work, size_t work_bytes, void* stream);
 int sqdk_dots_workspace(int64_t n, int32_t npairs, size_t* bytes);
+
+/* sqdk_dots with no workspace argument and no allocation of any kind: the
+ * per-block partial sums go to a file-scope device array of the library
+ * (128 KiB on every GPU it runs on, held until the process ends), so npairs
+ * is limited to the 16 pairs that array holds (larger requests fail with
+ * SQDK_EINVAL). No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The stream's device must be the
+ * current device; calls that overlap on one GPU share the array and race. */
+int sqdk_dots_noalloc(int64_t n, int32_t precision, int32_t npairs, const void* const* a,
+                      const void* const* b, double* out, void* stream);
 
 /* Convert between complex128 and complex64 vectors (n elements). */
 int sqdk_convert(int64_t n, int32_t from_precision, const void* in,

@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 17
+ABI_VERSION = 18
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -162,6 +162,8 @@ SIGNATURES = {
                             C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_size_t,
                             C.c_void_p]),
     "sqdk_dots_workspace": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_size_t)]),
+    "sqdk_dots_noalloc": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                    C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p]),
     "sqdk_convert": (C.c_int, [C.c_int64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p,
                                C.c_void_p]),
     "sqdk_locg_workspace": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.POINTER(SqdkLocgOpts),

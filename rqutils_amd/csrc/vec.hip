@@ -6,6 +6,8 @@
 // combination order, so results are bitwise reproducible run to run.
 #include "internal.hpp"
 
+#include <mutex>
+
 namespace sqdk {
 
 namespace {
@@ -468,6 +470,70 @@ int sqdk_dots(int64_t n, int32_t precision, int32_t npairs, const void* const* a
   }
   return dots_impl(n, precision, npairs, pl, out, reinterpret_cast<double*>(work),
                    as_stream(stream));
+}
+
+// ---- sqdk_dots without a caller workspace ----------------------------------
+//
+// The per-block partial sums live in this file-scope device array instead of
+// a caller workspace: 2 x 16 pairs x 512 blocks doubles (128 KiB) on every GPU
+// the library runs on, held until the process ends. Host code must not pass
+// the array's name to a kernel (on the host it is a stand-in, not a device
+// address): each device's address comes from hipGetSymbolAddress, taken once
+// per device with that device current and kept in dots_scratch_ptr[]. Every
+// partial a call reads was written by the same call (block b of pair k at
+// k * grid + b), so nothing needs clearing between calls; calls that overlap
+// on one GPU (two streams at once) share the array and race.
+}  // extern "C"
+
+static __device__ __align__(256) double scratch[2 * sqdk::kMaxPairs * sqdk::kDotBlocks];
+
+namespace sqdk {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex dots_scratch_mutex;
+double* dots_scratch_ptr[kMaxDevices] = {};
+
+int dots_scratch(double** out) {
+  int dev = -1;
+  SQDK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(SQDK_ERANGE, "device %d beyond the %d the static scratch tracks", dev,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(dots_scratch_mutex);
+  if (!dots_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SQDK_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                  hipGetErrorString(e));
+    dots_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = dots_scratch_ptr[dev];
+  return SQDK_OK;
+}
+
+}  // namespace
+}  // namespace sqdk
+
+extern "C" {
+
+int sqdk_dots_noalloc(int64_t n, int32_t precision, int32_t npairs, const void* const* a,
+                      const void* const* b, double* out, void* stream) {
+  SQDK_REQUIRE(a && b && out, "NULL pointer argument");
+  SQDK_REQUIRE(npairs >= 1, "npairs %d out of range", npairs);
+  SQDK_REQUIRE(npairs <= kMaxPairs,
+               "npairs %d exceeds the %d pairs the static scratch of sqdk_dots_noalloc holds",
+               npairs, kMaxPairs);
+  PairList pl{};
+  for (int k = 0; k < npairs; ++k) {
+    SQDK_REQUIRE(a[k] && b[k], "NULL vector in pair %d", k);
+    pl.a[k] = a[k];
+    pl.b[k] = b[k];
+  }
+  double* partials = nullptr;
+  SQDK_TRY(dots_scratch(&partials));
+  return dots_impl(n, precision, npairs, pl, out, partials, as_stream(stream));
 }
 
 int sqdk_convert(int64_t n, int32_t from_precision, const void* in, int32_t to_precision,
