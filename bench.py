@@ -386,19 +386,20 @@ def bench_next_rows(args, dev):
 
 
 
-def build_problem(w, dev, tune=True):
+def build_problem(w, dev, tune=True, launch=None):
     """SQDProblem of a workload whose keys are already resident on the device.
 
     The autotuner (one-time per problem, memoised) picks the launch first;
     ``setup_s`` then times what every new problem pays with that launch:
     grouping the Pauli sum on the host and building the bucket directories and
-    operator tables on the device."""
+    operator tables on the device. ``launch`` (a dict of ``SQDProblem``
+    keywords) fixes the launch instead of the autotuner."""
     import torch
     from rqutils_amd.sqd import SQDProblem
     keys = torch.from_numpy(np.ascontiguousarray(w.keys).view(np.int64)).to(dev)
     sync()
     tuned = None
-    if tune:
+    if tune and not launch:
         t = time.time()
         tuned = SQDProblem(w.hamiltonian, keys, w.num_qubits, device=dev, tune="auto").tuned
         log(f"[{w.name}] autotune -> cap={tuned.bucket_cap} dense_bits={tuned.dense_bits} "
@@ -406,7 +407,9 @@ def build_problem(w, dev, tune=True):
             f"({time.time() - t:.1f} s)")
     sync()
     t0 = time.perf_counter()
-    if tuned is not None:
+    if launch:
+        prob = SQDProblem(w.hamiltonian, keys, w.num_qubits, device=dev, **launch)
+    elif tuned is not None:
         prob = SQDProblem(w.hamiltonian, keys, w.num_qubits, device=dev,
                           group_split=tuned.group_split, compact=tuned.compact,
                           **tuned.space_kw())
@@ -925,6 +928,13 @@ def main():
     ap.add_argument("--net-steps", type=int, default=3000, help="Adam steps")
     ap.add_argument("--no-net-features", action="store_true",
                     help="skip the 30-input network (bits plus two diagonal-energy features)")
+    ap.add_argument("--bucket-cap", type=int, default=None,
+                    help="fix the launch: largest bucket (search lookup), no autotuning")
+    ap.add_argument("--dense-bits", type=int, default=0,
+                    help="fix the launch: dense lookup at this bucket width")
+    ap.add_argument("--rows-per-thread", type=int, default=4, help="with a fixed launch")
+    ap.add_argument("--group-split", type=int, default=1, help="with a fixed launch")
+    ap.add_argument("--compact", type=int, default=0, help="with a fixed launch (0/1)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -966,7 +976,13 @@ def main():
     t = time.time()
     w = W.make(args.workload)
     log(f"workload {w.name}: {w.n} states, {len(w.labels)} terms ({time.time() - t:.1f} s)")
-    prob, setup_s = build_problem(w, dev)
+    launch = None
+    if args.bucket_cap or args.dense_bits:
+        launch = {"rows_per_thread": args.rows_per_thread, "group_split": args.group_split,
+                  "compact": args.compact}
+        launch.update({"dense": True, "bucket_bits": args.dense_bits} if args.dense_bits
+                      else {"dense": False, "bucket_cap": args.bucket_cap})
+    prob, setup_s = build_problem(w, dev, launch=launch)
     mv, (v, out, o128, hits) = matvec_block(prob, w, args, dev)
     record = {
         "metric": METRIC,
