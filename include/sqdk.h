@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 18
+#define SQDK_ABI_VERSION 19
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -296,17 +296,19 @@ int sqdk_convert(int64_t n, int32_t from_precision, const void* in,
  * group) (cache_level[0] == 1, rqutils/sqd.py:353-371, 448-503). Here only the
  * hits are kept: for every row its (source row, group coefficient C_g(s_i))
  * pairs, found once by the bucket join, plus the X = 0 group as a diagonal.
- * Rows are cut into 64-row slices (the join's row slots); a slice stores its
- * entries column-major (entry k of its l-th row at base + 64 k + l) up to its
+ * The rows of each join chunk are sorted by hit count, most first (equal
+ * counts, and the rows without hits, keep their order; slots past the chunk's
+ * rows come last), and every 64 sorted slots form a slice; a slice stores its
+ * entries column-major (entry k of its lane l at base + 64 k + l) up to its
  * largest hit count, shorter rows padded with column 0xFFFFFFFF and
  * coefficient 0; row_len says where each row's hits end. Building:
- * sqdk_cache_slices (host count of slices), sqdk_cache_count (device: slice
- * rows and the entry offsets, the last one the entry count), allocation by the
- * caller, sqdk_cache_fill. */
+ * sqdk_cache_slices (host count of slices), sqdk_cache_count (device: row
+ * counts, the sort, the entry offsets, the last one the entry count),
+ * allocation by the caller, sqdk_cache_fill. */
 typedef struct sqdk_cache {
   int64_t n;                    /* rows (the subspace size) */
   int64_t n_slices;
-  const uint32_t* slice_rows;   /* [2 n_slices]: first row, rows in the slice (0-64) */
+  const uint32_t* slot_row;     /* [64 n_slices]: the row of each slot, 0xFFFFFFFF = none */
   const uint64_t* slice_base;   /* [n_slices + 1]: first entry; the last = entries */
   const uint32_t* col;          /* [entries] source row, 0xFFFFFFFF = padding */
   const void* coef;             /* [entries] complex coefficient (cache precision) */
@@ -320,14 +322,14 @@ typedef struct sqdk_cache {
 /* Slices of the subspace: n_items x chunk_rows / 64 (chunks of 1024, 2048 or
  * 4096 rows). */
 int sqdk_cache_slices(const sqdk_space* space, int64_t* n_slices);
-/* slice_rows [2 n_slices] and slice_base [n_slices + 1] (device). */
-int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space, uint32_t* slice_rows,
-                     uint64_t* slice_base, void* stream);
+/* slot_row [64 n_slices], slice_base [n_slices + 1] and row_len [n] (device). */
+int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space, uint32_t* slot_row,
+                     uint64_t* slice_base, uint32_t* row_len, void* stream);
 /* col / coef [slice_base[n_slices]], diag [n] (device; coef and diag complex of
- * precision) and row_len [n]. */
+ * precision) at the slots of sqdk_cache_count; row_len [n] is rewritten. */
 int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
-                    const uint64_t* slice_base, uint32_t* col, void* coef, void* diag,
-                    uint32_t* row_len, void* stream);
+                    const uint64_t* slice_base, const uint32_t* slot_row, uint32_t* col,
+                    void* coef, void* diag, uint32_t* row_len, void* stream);
 /* out = H v through the table (v, out: n complex of the cache's precision);
  * sums in FP64, each row's in group order; a row reads its own hits only. */
 int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* out, void* stream);
@@ -336,11 +338,11 @@ int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* out, void* s
  * = n_items x chunk_rows / 64; count and fill as above. The product is
  * sqdk_cache_matvec; sqdk_batch.cache hands it to the batched solve. */
 int sqdk_batch_cache_slices(const sqdk_batch* batch, int64_t* n_slices);
-int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slice_rows, uint64_t* slice_base,
-                           void* stream);
+int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slot_row, uint64_t* slice_base,
+                           uint32_t* row_len, void* stream);
 int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision, const uint64_t* slice_base,
-                          uint32_t* col, void* coef, void* diag, uint32_t* row_len,
-                          void* stream);
+                          const uint32_t* slot_row, uint32_t* col, void* coef, void* diag,
+                          uint32_t* row_len, void* stream);
 
 /* ---- operator functions on the reference's own arrays ------------------- *
  * For callers that hold the reference's packed arrays. A packed padded row of

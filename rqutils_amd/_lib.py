@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 18
+ABI_VERSION = 19
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -54,7 +54,7 @@ class SqdkInst(C.Structure):
 
 
 class SqdkCache(C.Structure):
-    _fields_ = [("n", C.c_int64), ("n_slices", C.c_int64), ("slice_rows", C.c_void_p),
+    _fields_ = [("n", C.c_int64), ("n_slices", C.c_int64), ("slot_row", C.c_void_p),
                 ("slice_base", C.c_void_p), ("col", C.c_void_p), ("coef", C.c_void_p),
                 ("diag", C.c_void_p), ("row_len", C.c_void_p), ("precision", C.c_int32),
                 ("slices_per_wave", C.c_int32)]
@@ -170,17 +170,17 @@ SIGNATURES = {
                                       C.POINTER(C.c_size_t)]),
     "sqdk_cache_slices": (C.c_int, [C.POINTER(SqdkSpace), C.POINTER(C.c_int64)]),
     "sqdk_cache_count": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace), C.c_void_p,
-                                   C.c_void_p, C.c_void_p]),
+                                   C.c_void_p, C.c_void_p, C.c_void_p]),
     "sqdk_cache_fill": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace), C.c_int32,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  C.c_void_p]),
+                                  C.c_void_p, C.c_void_p]),
     "sqdk_cache_matvec": (C.c_int, [C.POINTER(SqdkCache), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sqdk_batch_cache_slices": (C.c_int, [C.POINTER(SqdkBatch), C.POINTER(C.c_int64)]),
     "sqdk_batch_cache_count": (C.c_int, [C.POINTER(SqdkBatch), C.c_void_p, C.c_void_p,
-                                         C.c_void_p]),
+                                         C.c_void_p, C.c_void_p]),
     "sqdk_batch_cache_fill": (C.c_int, [C.POINTER(SqdkBatch), C.c_int32, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                        C.c_void_p]),
+                                        C.c_void_p, C.c_void_p]),
     "sqdk_locg_solve": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.c_void_p,
                                   C.POINTER(SqdkLocgOpts), C.c_void_p, C.c_size_t,
                                   C.POINTER(SqdkLocgResult), C.c_void_p]),

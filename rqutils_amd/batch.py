@@ -219,27 +219,30 @@ class SQDBatch:
         _lib.check(L.sqdk_batch_cache_slices(C.byref(self.cstruct), C.byref(ns)),
                    "sqdk_batch_cache_slices")
         ns = ns.value
-        srows = torch.empty(2 * ns, dtype=torch.int32, device=dev)
+        slot_row = torch.empty(64 * ns, dtype=torch.int32, device=dev)
         sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+        rlen = torch.empty(self.n, dtype=torch.int32, device=dev)
         stream = _lib.stream_ptr(dev)
-        _lib.check(L.sqdk_batch_cache_count(C.byref(self.cstruct), srows.data_ptr(),
-                                            sbase.data_ptr(), stream), "sqdk_batch_cache_count")
+        _lib.check(L.sqdk_batch_cache_count(C.byref(self.cstruct), slot_row.data_ptr(),
+                                            sbase.data_ptr(), rlen.data_ptr(), stream),
+                   "sqdk_batch_cache_count")
         entries = int(sbase[-1].item())
         cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
         col = torch.empty(max(entries, 1), dtype=torch.int32, device=dev)
         coef = torch.empty(max(entries, 1), dtype=cdt, device=dev)
         diag = torch.empty(self.n, dtype=cdt, device=dev)
-        rlen = torch.empty(self.n, dtype=torch.int32, device=dev)
         _lib.check(L.sqdk_batch_cache_fill(C.byref(self.cstruct), precision, sbase.data_ptr(),
-                                           col.data_ptr(), coef.data_ptr(), diag.data_ptr(),
-                                           rlen.data_ptr(), stream), "sqdk_batch_cache_fill")
+                                           slot_row.data_ptr(), col.data_ptr(), coef.data_ptr(),
+                                           diag.data_ptr(), rlen.data_ptr(), stream),
+                   "sqdk_batch_cache_fill")
         spw = 4 if entries <= 8 * self.n else 1
-        cs = _lib.SqdkCache(self.n, ns, srows.data_ptr(), sbase.data_ptr(), col.data_ptr(),
+        cs = _lib.SqdkCache(self.n, ns, slot_row.data_ptr(), sbase.data_ptr(), col.data_ptr(),
                             coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, spw)
-        self._caches[precision] = (cs, srows, sbase, col, coef, diag, rlen)
+        self._caches[precision] = (cs, slot_row, sbase, col, coef, diag, rlen)
         esz = 16 if precision == _lib.C128 else 8
         self.cache_info[precision] = {"slices": ns, "entries": entries, "slices_per_wave": spw,
-                                      "bytes": entries * (4 + esz) + self.n * (esz + 4) + ns * 16}
+                                      "bytes": entries * (4 + esz) + self.n * (esz + 4)
+                                      + ns * (8 + 64 * 4)}
         return self.cache_info[precision]
 
     def _cache_struct(self, precision: int):

@@ -792,13 +792,18 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
 //
 // The hits of H over the subspace (row i, source row j, coefficient C_g(s_i)
 // of the group g with s_j = s_i ^ x_g), found once by the same bucket join,
-// then applied by a sliced sparse product. A slice is one wave's 64 rows of
-// one row slot (row0 + 64 R w + 64 r + lane, as in bucket_join_kernel); its
-// entries are column-major, entry k of lane l at base + 64 k + l, for k below
-// the slice's width (its largest hit count); shorter rows are padded with
-// column kHitPad and coefficient 0, and the product reads only a row's own
-// entries (row_len). Hits of a row are stored in class and group order. The
-// X = 0 group is kept as a separate diagonal vector.
+// then applied by a sliced sparse product. The rows of each join chunk are
+// sorted by hit count, most first (rows of equal count keep their order, and
+// slots past the chunk's rows come last), and every 64 consecutive sorted
+// slots form a slice: slot_row[64 s + l] is the row of lane l of slice s. A
+// slice's entries are column-major, entry k of lane l at base + 64 k + l, for
+// k below the slice's width (its first row's count); shorter rows are padded
+// with column kHitPad and coefficient 0, and the product reads only a row's
+// own entries (row_len). Sorting keeps the padding small where counts vary
+// (on the 28-qubit chain 2.16x the hits in the join's row order) and puts the
+// rows without hits into zero-width slices, still in row order. Hits of a
+// row are stored in class and group order. The X = 0 group is kept as a
+// separate diagonal vector.
 //
 // (A layout with one step per group and a lane mask, which keeps the join's
 // gather locality, measured slower: 7.6 hits per step on the 28-qubit chain,
@@ -806,9 +811,10 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
 
 constexpr uint32_t kHitPad = 0xFFFFFFFFu;
 
-// MODE 0: each slice's entry count (64 x width) into sbase[sid] and its
-// (first row, rows) into srows. MODE 1: the entries (sbase now holds each
-// slice's first entry), row lengths and the diagonal.
+// MODE 0: every row's hit count into rlen, then the chunk's sort: the row of
+// each slot into slot_row and each slice's entry count (64 x width) into
+// sbase[sid]. MODE 1 (sbase now holds each slice's first entry): the entries
+// at the slots slot_row gives, the padding and the diagonal.
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
 __global__ void __launch_bounds__(kBlock)
 hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
@@ -819,7 +825,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
                  const uint32_t* __restrict__ class_hi, const int32_t* __restrict__ class_off,
                  const sqdk_inst* __restrict__ descs, const uint16_t* __restrict__ item_inst,
                  sqdk_inst single, int dmax, int max_bucket,
-                 uint32_t* __restrict__ srows, uint64_t* __restrict__ sbase,
+                 uint32_t* __restrict__ slot_row, uint64_t* __restrict__ sbase,
                  uint32_t* __restrict__ col, typename CoefT<VT>::type* __restrict__ hcoef,
                  VT* __restrict__ diag, uint32_t* __restrict__ rlen) {
   using CT = typename CoefT<VT>::type;
@@ -857,13 +863,30 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   bool valid[ROWS];
   uint32_t cnt[ROWS];
   uint64_t ebase[ROWS];
+  constexpr int P = ROWS * kBlock;                    // slots of the chunk
+  const int64_t slot0 = (int64_t)w * P;               // its first slot (64 per slice)
+  if constexpr (MODE == 1) {
+    // the sorted slot of each of the chunk's rows (LDS is free before the classes)
+    uint16_t* inv = reinterpret_cast<uint16_t*>(smem);
+    for (int q = tid; q < P; q += kBlock) {
+      const uint32_t rr = slot_row[slot0 + q];
+      if (rr != kHitPad) inv[rr - row0] = (uint16_t)q;
+    }
+    __syncthreads();
+  }
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
     const uint32_t i = row0 + wslot + r * kWave + lane;
     valid[r] = i < row1;
     sl[r] = valid[r] ? (KT)(keys[i] & lowmask) : (KT)0;
     cnt[r] = 0;
-    ebase[r] = MODE == 1 ? sbase[(int64_t)w * (4 * ROWS) + wave * ROWS + r] : 0;
+    ebase[r] = 0;
+    if constexpr (MODE == 1) {
+      if (valid[r]) {
+        const int q = reinterpret_cast<const uint16_t*>(smem)[wslot + r * kWave + lane];
+        ebase[r] = sbase[(slot0 + q) / kWave] + (uint64_t)(q % kWave);
+      }
+    }
   }
   const bool wave_active = __any(valid[0]);
   if (MODE == 1 && wave_active) {
@@ -891,7 +914,8 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
 #pragma unroll
       for (int r = 0; r < ROWS; ++r) {
         if (idx[r] >= 0) {
-          const uint64_t e = ebase[r] + (uint64_t)cnt[r] * kWave + lane;
+          // ebase already holds the slot's lane within its slice
+          const uint64_t e = ebase[r] + (uint64_t)cnt[r] * kWave;
           col[e] = qs + (uint32_t)idx[r];
           CT c;
           c.x = cr[r];
@@ -982,29 +1006,67 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       }
     }
   }
+  if constexpr (MODE == 0) {
+    // Every row's count, then a bitonic sort of the chunk's slots in LDS by
+    // key = (2^20 - 1 - count) << 12 | local row (ascending: most hits first,
+    // equal counts in row order; slots without a row carry count 0 and come
+    // after every row since their local index is larger).
 #pragma unroll
-  for (int r = 0; r < ROWS; ++r) {
-    const int64_t sid = (int64_t)w * (4 * ROWS) + wave * ROWS + r;
-    if constexpr (MODE == 0) {
-      uint32_t wmax = cnt[r];
-#pragma unroll
-      for (int sh = kWave / 2; sh > 0; sh >>= 1) wmax = max(wmax, (uint32_t)__shfl_xor((int)wmax, sh));
-      const uint32_t nval = (uint32_t)__popcll(__ballot(valid[r]));
-      if (lane == 0) {
-        sbase[sid] = (uint64_t)wmax * kWave;
-        srows[2 * sid] = row0 + wslot + r * kWave;
-        srows[2 * sid + 1] = nval;
-      }
-    } else {
+    for (int r = 0; r < ROWS; ++r)
       if (valid[r]) rlen[row0 + wslot + r * kWave + lane] = cnt[r];
+    __syncthreads();  // the class loop is done with the lookup tables
+    uint32_t* sk = reinterpret_cast<uint32_t*>(smem);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      const int loc = wslot + r * kWave + lane;
+      const uint32_t c = valid[r] ? min(cnt[r], 0xFFFFFu) : 0u;
+      sk[loc] = ((0xFFFFFu - c) << 12) | (uint32_t)loc;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int t = tid; t < P / 2; t += kBlock) {
+          const int lo = 2 * t - (t & (stride - 1));
+          const int hi = lo + stride;
+          const bool up = (lo & size) == 0;
+          const uint32_t a0 = sk[lo], a1 = sk[hi];
+          if ((a0 > a1) == up) {
+            sk[lo] = a1;
+            sk[hi] = a0;
+          }
+        }
+        __syncthreads();
+      }
+    }
+    const int nrow = (int)(row1 - row0);
+    for (int q = tid; q < P; q += kBlock) {
+      const int loc = (int)(sk[q] & 0xFFFu);
+      slot_row[slot0 + q] = loc < nrow ? row0 + (uint32_t)loc : kHitPad;
+      if ((q & (kWave - 1)) == 0) {
+        // a slice's width is its first (largest) count
+        const uint32_t width = loc < nrow ? 0xFFFFFu - (sk[q] >> 12) : 0u;
+        sbase[(slot0 + q) / kWave] = (uint64_t)width * kWave;
+      }
+    }
+  } else {
+    CT z;
+    z.x = RT(0);
+    z.y = RT(0);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r)
+      if (valid[r]) rlen[row0 + wslot + r * kWave + lane] = cnt[r];
+    __syncthreads();  // every row's count is written before the padding reads them
+    // padding of every slot of the chunk: rows past their hits, empty slots whole
+    for (int q = tid; q < P; q += kBlock) {
+      const int64_t sid = (slot0 + q) / kWave;
       const uint32_t width = (uint32_t)((sbase[sid + 1] - sbase[sid]) / kWave);
-      CT z;
-      z.x = RT(0);
-      z.y = RT(0);
-      for (uint32_t k = cnt[r]; k < width; ++k) {
-        const uint64_t e = ebase[r] + (uint64_t)k * kWave + lane;
-        col[e] = kHitPad;
-        hcoef[e] = z;
+      if (width == 0) continue;
+      const uint32_t rr = slot_row[slot0 + q];
+      const uint32_t k0 = rr == kHitPad ? 0u : rlen[rr];
+      const uint64_t e0 = sbase[sid] + (uint64_t)(q % kWave);
+      for (uint32_t k = k0; k < width; ++k) {
+        col[e0 + (uint64_t)k * kWave] = kHitPad;
+        hcoef[e0 + (uint64_t)k * kWave] = z;
       }
     }
   }
@@ -1055,7 +1117,7 @@ template <> __device__ inline double2 from_d2<double2>(double2 a) { return a; }
 // alone (slices_per_item slices per chunk, a multiple of S).
 template <typename VT, int S>
 __global__ void __launch_bounds__(kBlock)
-hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
+hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
                   const uint64_t* __restrict__ sbase, const uint32_t* __restrict__ col,
                   const typename CoefT<VT>::type* __restrict__ hcoef, const VT* __restrict__ diag,
                   const uint32_t* __restrict__ rlen, const VT* __restrict__ v,
@@ -1069,10 +1131,10 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
   if (sid0 >= n_slices) return;
   if (inst_skip && inst_skip[item_inst[sid0 / slices_per_item]] != 0) return;
   if constexpr (S == 1) {
-    // one slice: each lane walks its own row's hits, four entries per step
-    const uint32_t first = srows[2 * sid0], rows = srows[2 * sid0 + 1];
-    if ((uint32_t)lane >= rows) return;
-    const int64_t row = (int64_t)first + lane;
+    // one slice: each lane walks its own row's hits, several entries per step
+    const uint32_t slot = slot_row[sid0 * kWave + lane];
+    if (slot == kHitPad) return;
+    const int64_t row = slot;
     const uint64_t b0 = sbase[sid0] + lane;
     const int len = (int)rlen[row];
     const double2 dd = to_d2(diag[row]), xd = to_d2(v[row]);
@@ -1117,9 +1179,9 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ srows,
 #pragma unroll
   for (int u = 0; u < S; ++u) {
     const int64_t sid = min(sid0 + u, n_slices - 1);
-    const uint32_t first = srows[2 * sid], rows = srows[2 * sid + 1];
-    valid[u] = sid0 + u < n_slices && (uint32_t)lane < rows;
-    row[u] = valid[u] ? (int64_t)first + lane : 0;
+    const uint32_t slot = slot_row[sid * kWave + lane];
+    valid[u] = sid0 + u < n_slices && slot != kHitPad;
+    row[u] = valid[u] ? (int64_t)slot : 0;
     b0[u] = sbase[sid] + lane;
   }
   int kmax = 0;
@@ -1175,11 +1237,14 @@ struct HitTables {
 };
 
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
-int launch_hit_table(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint32_t* col,
+int launch_hit_table(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
                      void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
   auto kern = hit_table_kernel<VT, KT, ROWS, DENSE, MODE>;
   const int dmax = DENSE ? t.dense_fine_bits : t.max_fine_bits;
-  const size_t lds = DENSE ? ((size_t)8 << dmax) : lds_layout(dmax, t.max_bucket, sizeof(KT), 0).total;
+  // (the chunk's sort keys, or its slot map, reuse the LDS after the classes)
+  const size_t lds = std::max<size_t>(
+      DENSE ? ((size_t)8 << dmax) : lds_layout(dmax, t.max_bucket, sizeof(KT), 0).total,
+      (size_t)4 * ROWS * kBlock);
   if (lds > 160 * 1024) return fail(SQDK_ERANGE, "LDS use of %zu bytes exceeds 160 KiB", lds);
   if (lds > 65536)
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1187,35 +1252,35 @@ int launch_hit_table(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint3
   kern<<<(unsigned)t.n_items, kBlock, lds, s>>>(
       t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask, t.grp_off, t.zmask,
       reinterpret_cast<const typename CoefT<VT>::type*>(t.coef), t.class_hi, t.class_off,
-      t.descs, t.item_inst, t.single, dmax, t.max_bucket, srows, sbase, col,
+      t.descs, t.item_inst, t.single, dmax, t.max_bucket, slot_row, sbase, col,
       reinterpret_cast<typename CoefT<VT>::type*>(hcoef), reinterpret_cast<VT*>(diag), rlen);
   SQDK_LAUNCHED("hit_table_kernel");
   return SQDK_OK;
 }
 
 template <typename VT, int MODE, int R>
-int hit_table_rows(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint32_t* col,
+int hit_table_rows(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
                    void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
   if (t.has_dense)
-    SQDK_TRY((launch_hit_table<VT, uint32_t, R, true, MODE>(t, srows, sbase, col, hcoef, diag,
+    SQDK_TRY((launch_hit_table<VT, uint32_t, R, true, MODE>(t, slot_row, sbase, col, hcoef, diag,
                                                             rlen, s)));
   if (t.has_search) {
     if (t.narrow)
-      return launch_hit_table<VT, uint32_t, R, false, MODE>(t, srows, sbase, col, hcoef, diag,
+      return launch_hit_table<VT, uint32_t, R, false, MODE>(t, slot_row, sbase, col, hcoef, diag,
                                                             rlen, s);
-    return launch_hit_table<VT, uint64_t, R, false, MODE>(t, srows, sbase, col, hcoef, diag,
+    return launch_hit_table<VT, uint64_t, R, false, MODE>(t, slot_row, sbase, col, hcoef, diag,
                                                           rlen, s);
   }
   return SQDK_OK;
 }
 
 template <typename VT, int MODE>
-int hit_table_dispatch(const HitTables& t, uint32_t* srows, uint64_t* sbase, uint32_t* col,
+int hit_table_dispatch(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
                        void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
   switch (t.chunk_rows / kBlock) {
-    case 4: return hit_table_rows<VT, MODE, 4>(t, srows, sbase, col, hcoef, diag, rlen, s);
-    case 8: return hit_table_rows<VT, MODE, 8>(t, srows, sbase, col, hcoef, diag, rlen, s);
-    case 16: return hit_table_rows<VT, MODE, 16>(t, srows, sbase, col, hcoef, diag, rlen, s);
+    case 4: return hit_table_rows<VT, MODE, 4>(t, slot_row, sbase, col, hcoef, diag, rlen, s);
+    case 8: return hit_table_rows<VT, MODE, 8>(t, slot_row, sbase, col, hcoef, diag, rlen, s);
+    case 16: return hit_table_rows<VT, MODE, 16>(t, slot_row, sbase, col, hcoef, diag, rlen, s);
     default:
       return fail(SQDK_EINVAL, "chunk of %d rows: the hit table needs 1024, 2048 or 4096",
                   t.chunk_rows);
@@ -1371,7 +1436,7 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
   SQDK_REQUIRE(c->precision == precision, "cache precision %d, vectors %d", c->precision,
                precision);
   SQDK_REQUIRE(c->n >= 1 && c->n < (1ll << 31) && c->n_slices >= 1, "cache sizes out of range");
-  SQDK_REQUIRE(c->slice_rows && c->slice_base && c->diag && c->row_len,
+  SQDK_REQUIRE(c->slot_row && c->slice_base && c->diag && c->row_len,
                "cache has NULL tables");
   SQDK_REQUIRE(c->slices_per_wave == 0 || c->slices_per_wave == 1 || c->slices_per_wave == 4,
                "slices_per_wave %d not 1 or 4", c->slices_per_wave);
@@ -1383,7 +1448,7 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
   auto launch = [&](auto kern, auto vt) {
     using VT = decltype(vt);
     kern<<<(unsigned)grid, kBlock, 0, s>>>(
-        c->n_slices, c->slice_rows, c->slice_base, c->col,
+        c->n_slices, c->slot_row, c->slice_base, c->col,
         reinterpret_cast<const typename CoefT<VT>::type*>(c->coef),
         reinterpret_cast<const VT*>(c->diag), c->row_len, reinterpret_cast<const VT*>(v),
         reinterpret_cast<VT*>(out), skip, counter, item_inst, inst_skip,
@@ -1489,38 +1554,41 @@ extern "C" int sqdk_cache_slices(const sqdk_space* space, int64_t* n_slices) {
 }
 
 extern "C" int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space,
-                                uint32_t* slice_rows, uint64_t* slice_base, void* stream) {
+                                uint32_t* slot_row, uint64_t* slice_base, uint32_t* row_len,
+                                void* stream) {
   using namespace sqdk;
   SQDK_TRY(check_hit_space(ham, space));
-  SQDK_REQUIRE(slice_rows && slice_base, "NULL pointer argument");
+  SQDK_REQUIRE(slot_row && slice_base && row_len, "NULL pointer argument");
   SQDK_REQUIRE(space->n_items >= 1, "empty subspace");
   hipStream_t s = as_stream(stream);
   const int64_t ns = (int64_t)space->n_items * (space->chunk_rows / kWave);
   SQDK_HIP(hipMemsetAsync(slice_base + ns, 0, sizeof(uint64_t), s));
   SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_single(ham, space, ham->coef32),
-                                          slice_rows, slice_base, nullptr, nullptr, nullptr,
-                                          nullptr, s)));
+                                          slot_row, slice_base, nullptr, nullptr, nullptr,
+                                          row_len, s)));
   scan_u64_kernel<<<1, 1024, 0, s>>>(slice_base, ns + 1);
   SQDK_LAUNCHED("scan_u64_kernel");
   return SQDK_OK;
 }
 
 extern "C" int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
-                               const uint64_t* slice_base, uint32_t* col, void* coef, void* diag,
-                               uint32_t* row_len, void* stream) {
+                               const uint64_t* slice_base, const uint32_t* slot_row, uint32_t* col,
+                               void* coef, void* diag, uint32_t* row_len, void* stream) {
   using namespace sqdk;
   SQDK_TRY(check_hit_space(ham, space));
-  SQDK_REQUIRE(slice_base && col && coef && diag && row_len, "NULL pointer argument");
+  SQDK_REQUIRE(slice_base && slot_row && col && coef && diag && row_len,
+               "NULL pointer argument");
   hipStream_t s = as_stream(stream);
   uint64_t* sb = const_cast<uint64_t*>(slice_base);
+  uint32_t* sr = const_cast<uint32_t*>(slot_row);
   if (precision == SQDK_C128) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return hit_table_dispatch<double2, 1>(hit_tables_single(ham, space, ham->coef), nullptr, sb,
+    return hit_table_dispatch<double2, 1>(hit_tables_single(ham, space, ham->coef), sr, sb,
                                           col, coef, diag, row_len, s);
   }
   if (precision == SQDK_C64) {
     SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return hit_table_dispatch<float2, 1>(hit_tables_single(ham, space, ham->coef32), nullptr, sb,
+    return hit_table_dispatch<float2, 1>(hit_tables_single(ham, space, ham->coef32), sr, sb,
                                          col, coef, diag, row_len, s);
   }
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
@@ -1542,10 +1610,10 @@ extern "C" int sqdk_batch_cache_slices(const sqdk_batch* batch, int64_t* n_slice
   return SQDK_OK;
 }
 
-extern "C" int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slice_rows,
-                                      uint64_t* slice_base, void* stream) {
+extern "C" int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slot_row,
+                                      uint64_t* slice_base, uint32_t* row_len, void* stream) {
   using namespace sqdk;
-  SQDK_REQUIRE(batch && slice_rows && slice_base, "NULL pointer argument");
+  SQDK_REQUIRE(batch && slot_row && slice_base && row_len, "NULL pointer argument");
   SQDK_REQUIRE(batch->keys && batch->bstart && batch->fdir && batch->items && batch->item_inst &&
                    batch->inst, "batch has NULL subspace tables");
   SQDK_REQUIRE(batch->n_items >= 1, "empty batch");
@@ -1553,25 +1621,28 @@ extern "C" int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slice_r
   SQDK_TRY(sqdk_batch_cache_slices(batch, &ns));
   hipStream_t s = as_stream(stream);
   SQDK_HIP(hipMemsetAsync(slice_base + ns, 0, sizeof(uint64_t), s));
-  SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_batch(batch, batch->coef32), slice_rows,
-                                          slice_base, nullptr, nullptr, nullptr, nullptr, s)));
+  SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_batch(batch, batch->coef32), slot_row,
+                                          slice_base, nullptr, nullptr, nullptr, row_len, s)));
   scan_u64_kernel<<<1, 1024, 0, s>>>(slice_base, ns + 1);
   SQDK_LAUNCHED("scan_u64_kernel");
   return SQDK_OK;
 }
 
 extern "C" int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision,
-                                     const uint64_t* slice_base, uint32_t* col, void* coef,
-                                     void* diag, uint32_t* row_len, void* stream) {
+                                     const uint64_t* slice_base, const uint32_t* slot_row,
+                                     uint32_t* col, void* coef, void* diag, uint32_t* row_len,
+                                     void* stream) {
   using namespace sqdk;
-  SQDK_REQUIRE(batch && slice_base && col && coef && diag && row_len, "NULL pointer argument");
+  SQDK_REQUIRE(batch && slice_base && slot_row && col && coef && diag && row_len,
+               "NULL pointer argument");
   hipStream_t s = as_stream(stream);
   uint64_t* sb = const_cast<uint64_t*>(slice_base);
+  uint32_t* sr = const_cast<uint32_t*>(slot_row);
   if (precision == SQDK_C128)
-    return hit_table_dispatch<double2, 1>(hit_tables_batch(batch, batch->coef), nullptr, sb, col,
+    return hit_table_dispatch<double2, 1>(hit_tables_batch(batch, batch->coef), sr, sb, col,
                                           coef, diag, row_len, s);
   if (precision == SQDK_C64)
-    return hit_table_dispatch<float2, 1>(hit_tables_batch(batch, batch->coef32), nullptr, sb,
+    return hit_table_dispatch<float2, 1>(hit_tables_batch(batch, batch->coef32), sr, sb,
                                          col, coef, diag, row_len, s);
   return fail(SQDK_EINVAL, "unknown precision %d", precision);
 }

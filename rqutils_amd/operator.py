@@ -426,29 +426,31 @@ class PauliOperator:
         ns = C.c_int64(0)
         _lib.check(L.sqdk_cache_slices(C.byref(s), C.byref(ns)), "sqdk_cache_slices")
         ns = ns.value
-        srows = torch.empty(2 * ns, dtype=torch.int32, device=dev)
+        slot_row = torch.empty(64 * ns, dtype=torch.int32, device=dev)
         sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+        rlen = torch.empty(self.space.n, dtype=torch.int32, device=dev)
         stream = _lib.stream_ptr(dev)
-        _lib.check(L.sqdk_cache_count(C.byref(h), C.byref(s), srows.data_ptr(), sbase.data_ptr(),
-                                      stream), "sqdk_cache_count")
+        _lib.check(L.sqdk_cache_count(C.byref(h), C.byref(s), slot_row.data_ptr(),
+                                      sbase.data_ptr(), rlen.data_ptr(), stream),
+                   "sqdk_cache_count")
         entries = int(sbase[-1].item())
         cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
         esz = 16 if precision == _lib.C128 else 8
-        nbytes = entries * (4 + esz) + self.space.n * (esz + 4) + ns * 16
+        nbytes = entries * (4 + esz) + self.space.n * (esz + 4) + ns * (8 + 64 * 4)
         if max_bytes is not None and nbytes > max_bytes:
             return False
         col = torch.empty(max(entries, 1), dtype=torch.int32, device=dev)
         coef = torch.empty(max(entries, 1), dtype=cdt, device=dev)
         diag = torch.empty(self.space.n, dtype=cdt, device=dev)
-        rlen = torch.empty(self.space.n, dtype=torch.int32, device=dev)
         _lib.check(L.sqdk_cache_fill(C.byref(h), C.byref(s), precision, sbase.data_ptr(),
-                                     col.data_ptr(), coef.data_ptr(), diag.data_ptr(),
-                                     rlen.data_ptr(), stream), "sqdk_cache_fill")
+                                     slot_row.data_ptr(), col.data_ptr(), coef.data_ptr(),
+                                     diag.data_ptr(), rlen.data_ptr(), stream), "sqdk_cache_fill")
         # few entries per row: the product is latency-bound, a wave takes 4 slices
         spw = 4 if entries <= 8 * self.space.n else 1
-        cs = _lib.SqdkCache(self.space.n, ns, srows.data_ptr(), sbase.data_ptr(), col.data_ptr(),
-                            coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, spw)
-        self._caches[precision] = (cs, srows, sbase, col, coef, diag, rlen)
+        cs = _lib.SqdkCache(self.space.n, ns, slot_row.data_ptr(), sbase.data_ptr(),
+                            col.data_ptr(), coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(),
+                            precision, spw)
+        self._caches[precision] = (cs, slot_row, sbase, col, coef, diag, rlen)
         self.cache_info[precision] = {"slices": ns, "entries": entries, "bytes": nbytes,
                                       "slices_per_wave": spw}
         return True

@@ -288,21 +288,19 @@ def hproj(hamiltonian, states, unique_states: bool = False) -> csr_array:
     space = Subspace(keys, states.shape[1])
     op = PauliOperator(hamiltonian, space, prune=False)
     # Every (row, off-diagonal group) hit and its coefficient come from the
-    # hit table (one bucket-join pass), unpacked from its 64-row slices:
-    # entry k of the l-th row of slice s sits at slice_base[s] + 64 k + l.
+    # hit table (one bucket-join pass), unpacked from its 64-slot slices: entry
+    # k of the row in slot t sits at slice_base[t // 64] + 64 k + t % 64.
     op.build_cache(_lib.C128)
-    _, srows, sbase, col, coef, diag, rlen = op._caches[_lib.C128]
+    _, slot_row, sbase, col, coef, diag, rlen = op._caches[_lib.C128]
     dev = space.device
-    sr = srows.view(-1, 2).long()
-    first, nrow = sr[:, 0], sr[:, 1]
-    sid = torch.repeat_interleave(torch.arange(first.numel(), device=dev), nrow)
-    lane = torch.arange(sid.numel(), device=dev) - torch.repeat_interleave(
-        torch.cumsum(nrow, 0) - nrow, nrow)
-    row = first[sid] + lane
+    sr = slot_row.view(-1).to(torch.int64) & 0xFFFFFFFF
+    slot = torch.nonzero(sr != 0xFFFFFFFF).view(-1)
+    row = sr[slot]
     ln = rlen[row].long()
     k = torch.arange(int(ln.sum().item()), device=dev) - torch.repeat_interleave(
         torch.cumsum(ln, 0) - ln, ln)
-    e = sbase[torch.repeat_interleave(sid, ln)] + 64 * k + torch.repeat_interleave(lane, ln)
+    e = sbase[torch.repeat_interleave(slot // 64, ln)] + 64 * k + \
+        torch.repeat_interleave(slot % 64, ln)
     rows = [torch.repeat_interleave(row, ln)]
     cols = [col[e].long()]
     vals = [coef[e]]

@@ -26,37 +26,55 @@ def _problem(p, **kw):
 
 def _table_to_csr(op, prec, n):
     """Host CSR of the hit table (padding dropped), its diagonal and its hit
-    count; checks the layout (padding only after a row's hits, lanes without a
-    row all padding, row lengths)."""
+    count; checks the layout: every row in exactly one slot, the slots of a
+    chunk sorted by hit count (most first, equal counts in row order), padding
+    only after a row's hits, slots without a row all padding, slice widths
+    equal to their largest count, row lengths."""
     import scipy.sparse as sps
-    cs, srows, sbase, col, coef, diag, rlen = op._caches[prec]
-    srows = srows.cpu().numpy().view(np.uint32).reshape(-1, 2)
+    cs, slot_row, sbase, col, coef, diag, rlen = op._caches[prec]
+    slot_row = slot_row.cpu().numpy().view(np.uint32).reshape(-1, 64)
     sbase = sbase.cpu().numpy()
     col = col.cpu().numpy().view(np.uint32)
     coef = coef.cpu().numpy()
     rlen = rlen.cpu().numpy()
+    seen = np.zeros(n, bool)
     rows, cols, vals = [], [], []
-    for sid in range(srows.shape[0]):
-        first, nrow = int(srows[sid, 0]), int(srows[sid, 1])
+    per_chunk = op.space.chunk_rows // 64
+    for c0 in range(0, slot_row.shape[0], per_chunk):
+        chunk = slot_row[c0:c0 + per_chunk].reshape(-1)
+        real = chunk[chunk != 0xFFFFFFFF].astype(np.int64)
+        assert np.all(chunk[real.size:] == 0xFFFFFFFF)      # empty slots last
+        order = np.lexsort((real, -rlen[real].astype(np.int64)))
+        assert np.array_equal(order, np.arange(real.size))  # sorted: count desc, row asc
+    for sid in range(slot_row.shape[0]):
         b0, b1 = int(sbase[sid]), int(sbase[sid + 1])
-        assert (b1 - b0) % 64 == 0 and nrow <= 64
+        assert (b1 - b0) % 64 == 0
         width = (b1 - b0) // 64
+        lanes = slot_row[sid]
+        lens = [int(rlen[r]) for r in lanes if r != 0xFFFFFFFF]
+        assert width == (max(lens) if lens else 0)
+        for r in lanes:
+            if r != 0xFFFFFFFF:
+                assert not seen[r]
+                seen[r] = True
         if width == 0:
             continue
         blk_c = col[b0:b1].reshape(width, 64)
         blk_v = coef[b0:b1].reshape(width, 64)
         for lane in range(64):
             real = blk_c[:, lane] != 0xFFFFFFFF
-            if lane >= nrow:
+            r = int(lanes[lane])
+            if r == 0xFFFFFFFF:
                 assert not real.any()
                 continue
             k = int(real.sum())
-            assert rlen[first + lane] == k
+            assert rlen[r] == k
             assert real[:k].all() and not real[k:].any()
             assert np.all(blk_v[k:, lane] == 0)
-            rows += [first + lane] * k
+            rows += [r] * k
             cols += blk_c[:k, lane].tolist()
             vals += blk_v[:k, lane].tolist()
+    assert seen.all()
     m = sps.csr_matrix((np.asarray(vals, np.complex128), (rows, cols)), shape=(n, n))
     return m, diag.cpu().numpy().astype(np.complex128), len(rows)
 
@@ -145,15 +163,22 @@ def test_sqd_cache_levels_agree(gpu):
     assert abs(e0 - e1) < 1e-10 * abs(e0)
 
 
-def test_slices_per_wave_same_product(gpu):
-    """Both launch shapes of the product give the same bits."""
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_slices_per_wave_same_product(gpu, prec):
+    """Both launch shapes of the product give the same bits, in both
+    precisions, on rows with enough hits for the unrolled steps (4 entries
+    per step for complex128, 8 for complex64) and for the remainder loop."""
     from rqutils_amd import _lib as L
     import ctypes as C
-    p = small_problem(9, n=12, terms=120, nstates=3000)
+    p = small_problem(9, n=12, terms=400, nstates=3000)
     prob = _problem(p, cache=True)
-    assert prob.op.build_cache(L.C128)
-    cs = prob.op._caches[L.C128][0]
-    v = torch.randn(p.keys.size, dtype=torch.complex128, device="cuda")
+    pc = L.C128 if prec == "c128" else L.C64
+    assert prob.op.build_cache(pc)
+    cs = prob.op._caches[pc][0]
+    rlen = prob.op._caches[pc][6].cpu().numpy()
+    assert rlen.max() >= 17 and np.any(rlen % 8 != 0)
+    dt = torch.complex128 if prec == "c128" else torch.complex64
+    v = torch.randn(p.keys.size, dtype=dt, device="cuda")
     outs = []
     for spw in (1, 4):
         cs.slices_per_wave = spw
@@ -162,7 +187,8 @@ def test_slices_per_wave_same_product(gpu):
                                           L.stream_ptr(v.device)), "cache matvec")
         outs.append(o.cpu().numpy())
     assert np.array_equal(outs[0], outs[1])
-    assert _rel(outs[0], p.matvec(v.cpu().numpy())) < 1e-13
+    ref = p.matvec(v.cpu().numpy().astype(np.complex128))
+    assert _rel(outs[0].astype(np.complex128), ref) < (1e-13 if prec == "c128" else 3e-7)
 
 
 def test_cache_budget_falls_back(gpu):
