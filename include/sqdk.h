@@ -296,9 +296,10 @@ int sqdk_convert(int64_t n, int32_t from_precision, const void* in,
  * group) (cache_level[0] == 1, rqutils/sqd.py:353-371, 448-503). Here only the
  * hits are kept: for every row its (source row, group coefficient C_g(s_i))
  * pairs, found once by the bucket join, plus the X = 0 group as a diagonal.
- * The rows of each join chunk are sorted by hit count, most first (equal
- * counts, and the rows without hits, keep their order; slots past the chunk's
- * rows come last), and every 64 sorted slots form a slice; a slice stores its
+ * Each window of 256 consecutive rows of a join chunk is sorted by hit count,
+ * most first (equal counts, and the rows without hits, keep their order; slots
+ * past the chunk's rows come last), and every 64 sorted slots form a slice; a
+ * slice stores its
  * entries column-major (entry k of its lane l at base + 64 k + l) up to its
  * largest hit count, shorter rows padded with column 0xFFFFFFFF and
  * coefficient 0; row_len says where each row's hits end. Building:

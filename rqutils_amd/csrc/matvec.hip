@@ -792,17 +792,18 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
 //
 // The hits of H over the subspace (row i, source row j, coefficient C_g(s_i)
 // of the group g with s_j = s_i ^ x_g), found once by the same bucket join,
-// then applied by a sliced sparse product. The rows of each join chunk are
-// sorted by hit count, most first (rows of equal count keep their order, and
-// slots past the chunk's rows come last), and every 64 consecutive sorted
-// slots form a slice: slot_row[64 s + l] is the row of lane l of slice s. A
-// slice's entries are column-major, entry k of lane l at base + 64 k + l, for
-// k below the slice's width (its first row's count); shorter rows are padded
-// with column kHitPad and coefficient 0, and the product reads only a row's
-// own entries (row_len). Sorting keeps the padding small where counts vary
-// (on the 28-qubit chain 2.16x the hits in the join's row order) and puts the
-// rows without hits into zero-width slices, still in row order. Hits of a
-// row are stored in class and group order. The X = 0 group is kept as a
+// then applied by a sliced sparse product. Each window of 256 consecutive
+// rows of a join chunk is sorted by hit count, most first (rows of equal
+// count keep their order, and slots past the chunk's rows come last), and
+// every 64 consecutive sorted slots form a slice: slot_row[64 s + l] is the
+// row of lane l of slice s. A slice's entries are column-major, entry k of
+// lane l at base + 64 k + l, for k below the slice's width (its first row's
+// count); shorter rows are padded with column kHitPad and coefficient 0, and
+// the product reads only a row's own entries (row_len). The sort keeps the
+// padding small where counts vary (on the 28-qubit chain 1.39x the hits, 2.16x
+// in the join's row order) and out of the cache lines of live entries, and
+// puts rows without hits into zero-width slices, still in row order. Hits of
+// a row are stored in class and group order. The X = 0 group is kept as a
 // separate diagonal vector.
 //
 // (A layout with one step per group and a lane mask, which keeps the join's
@@ -810,6 +811,17 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
 // 1.1 on the 40-qubit sum, and each step costs a wave iteration.)
 
 constexpr uint32_t kHitPad = 0xFFFFFFFFu;
+
+// Rows of one sort window: 256 consecutive rows of a chunk are sorted by hit
+// count among themselves. Measured (MI355X, hit product, 64-row slices):
+// windows of 64 / 128 / 256 / 1024 rows give 1.49 / 1.38 / 1.33 / 1.85 ms on
+// the 28-qubit chain (complex64; 1.76 ms unsorted) and 0.141 / 0.137 / 0.140 /
+// 0.145 ms on the 40-qubit sum (0.150 unsorted). Within a slice the sort makes
+// the lanes still walking their hits contiguous, so the padding of shorter
+// rows stops sharing cache lines with live entries; wider windows also cut
+// the padding itself, until rows from far apart in the chunk share a slice
+// and the gathers lose their locality.
+constexpr int kHitSortLog2 = 8;
 
 // MODE 0: every row's hit count into rlen, then the chunk's sort: the row of
 // each slot into slot_row and each slice's entry count (64 x width) into
@@ -824,7 +836,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
                  const typename CoefT<VT>::type* __restrict__ coef,
                  const uint32_t* __restrict__ class_hi, const int32_t* __restrict__ class_off,
                  const sqdk_inst* __restrict__ descs, const uint16_t* __restrict__ item_inst,
-                 sqdk_inst single, int dmax, int max_bucket,
+                 sqdk_inst single, int dmax, int max_bucket, int sort_log2,
                  uint32_t* __restrict__ slot_row, uint64_t* __restrict__ sbase,
                  uint32_t* __restrict__ col, typename CoefT<VT>::type* __restrict__ hcoef,
                  VT* __restrict__ diag, uint32_t* __restrict__ rlen) {
@@ -1008,19 +1020,22 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   }
   if constexpr (MODE == 0) {
     // Every row's count, then a bitonic sort of the chunk's slots in LDS by
-    // key = (2^20 - 1 - count) << 12 | local row (ascending: most hits first,
-    // equal counts in row order; slots without a row carry count 0 and come
-    // after every row since their local index is larger).
+    // key = window << 44 | (2^32 - 1 - count) << 12 | local row, window =
+    // local row >> sort_log2 (ascending: each window of 2^sort_log2 rows on
+    // its own, most hits first, equal counts in row order; slots without a
+    // row carry count 0 and come after every row of their window since their
+    // local index is larger).
 #pragma unroll
     for (int r = 0; r < ROWS; ++r)
       if (valid[r]) rlen[row0 + wslot + r * kWave + lane] = cnt[r];
     __syncthreads();  // the class loop is done with the lookup tables
-    uint32_t* sk = reinterpret_cast<uint32_t*>(smem);
+    uint64_t* sk = reinterpret_cast<uint64_t*>(smem);
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
       const int loc = wslot + r * kWave + lane;
-      const uint32_t c = valid[r] ? min(cnt[r], 0xFFFFFu) : 0u;
-      sk[loc] = ((0xFFFFFu - c) << 12) | (uint32_t)loc;
+      const uint64_t c = valid[r] ? cnt[r] : 0u;
+      sk[loc] = ((uint64_t)(loc >> sort_log2) << 44) | ((0xFFFFFFFFull - c) << 12) |
+                (uint64_t)loc;
     }
     __syncthreads();
     for (int size = 2; size <= P; size <<= 1) {
@@ -1029,7 +1044,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
           const int lo = 2 * t - (t & (stride - 1));
           const int hi = lo + stride;
           const bool up = (lo & size) == 0;
-          const uint32_t a0 = sk[lo], a1 = sk[hi];
+          const uint64_t a0 = sk[lo], a1 = sk[hi];
           if ((a0 > a1) == up) {
             sk[lo] = a1;
             sk[hi] = a0;
@@ -1044,7 +1059,8 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       slot_row[slot0 + q] = loc < nrow ? row0 + (uint32_t)loc : kHitPad;
       if ((q & (kWave - 1)) == 0) {
         // a slice's width is its first (largest) count
-        const uint32_t width = loc < nrow ? 0xFFFFFu - (sk[q] >> 12) : 0u;
+        const uint32_t width =
+            loc < nrow ? (uint32_t)(0xFFFFFFFFull - ((sk[q] >> 12) & 0xFFFFFFFFull)) : 0u;
         sbase[(slot0 + q) / kWave] = (uint64_t)width * kWave;
       }
     }
@@ -1234,6 +1250,7 @@ struct HitTables {
   int n_items, chunk_rows, max_bucket, max_fine_bits, narrow;
   int dense_fine_bits;         // dense-lookup chunks: their largest fine_bits
   bool has_dense, has_search;
+  int sort_log2;               // rows of a sort window: 2^sort_log2 (6 .. log2 chunk rows)
 };
 
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
@@ -1244,7 +1261,7 @@ int launch_hit_table(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, ui
   // (the chunk's sort keys, or its slot map, reuse the LDS after the classes)
   const size_t lds = std::max<size_t>(
       DENSE ? ((size_t)8 << dmax) : lds_layout(dmax, t.max_bucket, sizeof(KT), 0).total,
-      (size_t)4 * ROWS * kBlock);
+      (size_t)8 * ROWS * kBlock);
   if (lds > 160 * 1024) return fail(SQDK_ERANGE, "LDS use of %zu bytes exceeds 160 KiB", lds);
   if (lds > 65536)
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
@@ -1252,15 +1269,17 @@ int launch_hit_table(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, ui
   kern<<<(unsigned)t.n_items, kBlock, lds, s>>>(
       t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask, t.grp_off, t.zmask,
       reinterpret_cast<const typename CoefT<VT>::type*>(t.coef), t.class_hi, t.class_off,
-      t.descs, t.item_inst, t.single, dmax, t.max_bucket, slot_row, sbase, col,
+      t.descs, t.item_inst, t.single, dmax, t.max_bucket, t.sort_log2, slot_row, sbase, col,
       reinterpret_cast<typename CoefT<VT>::type*>(hcoef), reinterpret_cast<VT*>(diag), rlen);
   SQDK_LAUNCHED("hit_table_kernel");
   return SQDK_OK;
 }
 
 template <typename VT, int MODE, int R>
-int hit_table_rows(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
+int hit_table_rows(const HitTables& t0, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
                    void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
+  HitTables t = t0;
+  t.sort_log2 = kHitSortLog2;
   if (t.has_dense)
     SQDK_TRY((launch_hit_table<VT, uint32_t, R, true, MODE>(t, slot_row, sbase, col, hcoef, diag,
                                                             rlen, s)));

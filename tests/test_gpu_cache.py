@@ -40,12 +40,19 @@ def _table_to_csr(op, prec, n):
     seen = np.zeros(n, bool)
     rows, cols, vals = [], [], []
     per_chunk = op.space.chunk_rows // 64
+    win = 256 // 64                                          # slices of a sort window
     for c0 in range(0, slot_row.shape[0], per_chunk):
         chunk = slot_row[c0:c0 + per_chunk].reshape(-1)
         real = chunk[chunk != 0xFFFFFFFF].astype(np.int64)
         assert np.all(chunk[real.size:] == 0xFFFFFFFF)      # empty slots last
-        order = np.lexsort((real, -rlen[real].astype(np.int64)))
-        assert np.array_equal(order, np.arange(real.size))  # sorted: count desc, row asc
+        rows0 = real.min() if real.size else 0
+        for w0 in range(0, per_chunk, win):
+            part = slot_row[c0 + w0:c0 + w0 + win].reshape(-1)
+            part = part[part != 0xFFFFFFFF].astype(np.int64)
+            # the window holds its own 256 rows, sorted: count desc, row asc
+            assert np.all((part - rows0) // 256 == w0 // win)
+            order = np.lexsort((part, -rlen[part].astype(np.int64)))
+            assert np.array_equal(order, np.arange(part.size))
     for sid in range(slot_row.shape[0]):
         b0, b1 = int(sbase[sid]), int(sbase[sid + 1])
         assert (b1 - b0) % 64 == 0
