@@ -62,6 +62,16 @@ def count_hits(prob) -> int:
 
 
 
+def read_limiter(path):
+    """The measured limiter (resource, fraction used) from a committed rocprof
+    summary's ``limiter:`` line, or None."""
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"limiter:\s*(\S+)\s+([0-9.eE+]+)", open(path).read())
+    return {"resource": m.group(1), "frac": float(m.group(2)),
+            "source": os.path.relpath(path, ROOT)} if m else None
+
+
 def read_traffic(path):
     """HBM bytes per launch from a committed rocprof summary (or None)."""
     if not os.path.exists(path):
@@ -1007,7 +1017,9 @@ def main():
                      "unit": "GB/s", "frac": mv["hbm_frac"],
                      "traffic": read_traffic(os.path.join(
                          ROOT, "profiles", f"bucket_join_kernel_{w.name}.md")),
-                     "kernel": mv["kernel"], "bytes_per_launch": mv["bytes_per_launch"]},
+                     "kernel": mv["kernel"], "bytes_per_launch": mv["bytes_per_launch"],
+                     "measured_limiter": read_limiter(os.path.join(
+                         ROOT, "profiles", f"bucket_join_kernel_{w.name}.md"))},
     }
     if not args.no_cache:
         record["hit_table"] = hit_table_block(prob, v, out, o128, hits, args.steps, dev)
