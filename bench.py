@@ -808,6 +808,61 @@ C5_UNGUARDED = {
 }
 
 
+GEMM_SHAPES = (
+    # (name, M, N, K, trans_a, trans_b, ksplit): the start network's products at
+    # a training batch of 4096, and one large square product
+    ("hidden_forward", 4096, 256, 256, 0, 0, 1),
+    ("hidden_input_grad", 4096, 256, 256, 0, 1, 1),
+    ("hidden_weight_grad", 256, 256, 4096, 1, 0, 8),
+    ("square_8192", 8192, 8192, 8192, 0, 0, 1),
+)
+
+
+def gemm_block(dev, reps=20):
+    """BF16 GEMM (sqdk_gemm_bf16, FP32 accumulation) timed with HIP events on
+    the stream it is launched on; TFLOP/s against the 2.5 PFLOP/s dense BF16
+    peak."""
+    import ctypes as C
+    import torch
+    from rqutils_amd import _lib
+    L = _lib.lib()
+    stream = torch.cuda.current_stream(dev)
+    g = torch.Generator(device=dev).manual_seed(5)
+    out = {}
+    for name, M, N, K, ta, tb, ks in GEMM_SHAPES:
+        a = torch.randn((K, M) if ta else (M, K), device=dev, generator=g).to(torch.bfloat16)
+        b = torch.randn((N, K) if tb else (K, N), device=dev, generator=g).to(torch.bfloat16)
+        c = torch.empty((M, N), dtype=torch.float32, device=dev)
+        nb = C.c_size_t(0)
+        _lib.check(L.sqdk_gemm_bf16_workspace(M, N, ks, C.byref(nb)), "gemm workspace")
+        work = torch.empty(max(nb.value, 4), dtype=torch.uint8, device=dev)
+
+        def run():
+            _lib.check(L.sqdk_gemm_bf16(M, N, K, a.data_ptr(), a.shape[1], ta, b.data_ptr(),
+                                        b.shape[1], tb, c.data_ptr(), N, None, ks,
+                                        work.data_ptr() if ks > 1 else None, nb.value,
+                                        stream.cuda_stream), "sqdk_gemm_bf16")
+        run()
+        ref = (a.float().t() if ta else a.float()) @ (b.float().t() if tb else b.float())
+        err = ((c - ref).norm() / ref.norm()).item()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        n = reps if M * N * K < (1 << 34) else 3
+        e0.record(stream)
+        for _ in range(n):
+            run()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / n
+        tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+        out[name] = {"shape": [M, N, K], "trans": [ta, tb], "ksplit": ks, "ms": round(ms, 5),
+                     "tflops": round(tf, 2), "frac_of_2500": round(tf / 2500.0, 4),
+                     "rel_err_vs_fp32": float(f"{err:.2e}")}
+        del a, b, c, ref, work
+    return out
+
+
 def c5_block(dev, n_train, n_test, steps, features=True):
     """Network-started LOBPCG on the 28-qubit chain family at 2^20 states: a
     28-input BF16 MLP (bits as +-1) trained with the library's MFMA kernels on
@@ -887,6 +942,8 @@ def c5_block(dev, n_train, n_test, steps, features=True):
            "generate_s": round(gen_s, 2), "ground_vectors_s": round(ground_s, 2),
            "train_s": round(train_s, 2), "adam_steps": steps,
            "train_tflops": round(flop_per_sample * 4096 * steps / train_s / 1e12, 3),
+           "train_ms_per_step": round(1e3 * train_s / max(steps, 1), 3),
+           "gemm": gemm_block(dev),
            "loss_first_last": [round(losses[0], 5), round(losses[-1], 5)],
            "features_network_loss_first_last": None if net_f is None else
            [round(losses_f[0], 5), round(losses_f[-1], 5)],

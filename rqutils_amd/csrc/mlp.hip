@@ -5,11 +5,11 @@
 // (rqutils/sqd.py:373-398).
 //
 // The dense products run on the matrix cores: gemm_bf16_kernel gives every
-// wavefront one 16x16 output tile and accumulates in FP32 with
-// v_mfma_f32_16x16x16_bf16 over K in steps of 16. Operand transposes are
-// strides, so the same kernel serves the forward pass (X W), the input
-// gradients (dZ W^T) and the weight gradients (H^T dZ, K-split over the batch
-// with a fixed-order reduction of the partial tiles). Elementwise kernels do
+// workgroup a 64x64 output tile, staged through LDS 32 k at a time and
+// multiplied with gfx950's v_mfma_f32_16x16x32_bf16 (FP32 accumulation).
+// Operand transposes are strides, so the same kernel serves the forward pass
+// (X W), the input gradients (dZ W^T) and the weight gradients (H^T dZ,
+// K-split over the batch with a fixed-order reduction of the partial tiles). Elementwise kernels do
 // the encoding, GELU and its derivative, the loss and Adam.
 #include "internal.hpp"
 
@@ -33,57 +33,158 @@ __device__ inline uint16_t f2bf(float f) {
 __device__ inline float bf2f(uint16_t h) { return __uint_as_float(((uint32_t)h) << 16); }
 
 // C = op(A) op(B) (+ bias) ; op(A)(i,k) = transA ? A[k*lda+i] : A[i*lda+k],
-// op(B)(k,j) = transB ? B[j*ldb+k] : B[k*ldb+j]. Wave w handles tile
-// (tm, tn) and K range ks of ksplit. With ksplit > 1 partial tiles go to
-// part[ks][M][N] and gemm_reduce_kernel finishes.
+// op(B)(k,j) = transB ? B[j*ldb+k] : B[k*ldb+j].
+//
+// A workgroup (4 waves in a 2 x 2 grid) computes a (32 WM) x (32 WN) tile of
+// C over its share ks of the K range (ksplit shares; with ksplit > 1 the
+// partial tile goes to part[ks][M][N] and gemm_reduce_kernel finishes in a
+// fixed order). K advances 32 at a time through two LDS buffers: while the
+// waves multiply the slices in one buffer, every thread already holds the
+// next slices' global data in registers (16-byte loads along each operand's
+// contiguous dimension, whatever the transpose) and stores them into the
+// other buffer, so one barrier per step suffices. LDS holds both operands
+// k-contiguous (row stride 40 bf16 = 80 bytes): every lane's 8 k values are
+// one aligned 16-byte read. Each wave multiplies its (16 WM) x (16 WN)
+// quarter as WM x WN tiles of gfx950's v_mfma_f32_16x16x32_bf16 (FP32
+// accumulation). Operand layout of that instruction: lane l feeds row (or
+// column) l % 16 with 8 k values, the same k for both operands (which 8 does
+// not change the sum); the result holds column l % 16, rows 4 (l / 16) + j.
+constexpr int kTileK = 32, kLdsStride = 40;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// 16-byte loads of an R x 32 panel, element (r, k) at p[r*ld + k] when
+// kcontig, else at p[k*ld + r]; rows >= rmax and k >= kmax read as 0. vec: the
+// 16-byte path is aligned (ld a multiple of 8, p 16-byte aligned).
+template <int R>
+__device__ inline void gemm_load_panel(uint4 (&v)[R / 64], const uint16_t* __restrict__ p,
+                                       int64_t ld, bool kcontig, bool vec, int64_t r0,
+                                       int64_t rmax, int64_t k0, int64_t kmax, int tid) {
+#pragma unroll
+  for (int t = 0; t < R / 64; ++t) {
+    const int q = tid + kBlock * t;
+    const int64_t gr = r0 + (kcontig ? (q >> 2) : 8 * (q % (R / 8)));
+    const int64_t gk = k0 + (kcontig ? 8 * (q & 3) : q / (R / 8));
+    const bool full = kcontig ? (gr < rmax && gk + 8 <= kmax) : (gk < kmax && gr + 8 <= rmax);
+    if (vec && full) {
+      v[t] = *reinterpret_cast<const uint4*>(kcontig ? p + gr * ld + gk : p + gk * ld + gr);
+    } else {
+      uint16_t e[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int64_t r = kcontig ? gr : gr + j, k = kcontig ? gk + j : gk;
+        e[j] = (r < rmax && k < kmax) ? p[kcontig ? r * ld + k : k * ld + r] : (uint16_t)0;
+      }
+      v[t] = make_uint4(e[0] | ((uint32_t)e[1] << 16), e[2] | ((uint32_t)e[3] << 16),
+                        e[4] | ((uint32_t)e[5] << 16), e[6] | ((uint32_t)e[7] << 16));
+    }
+  }
+}
+
+// Stores a panel loaded by gemm_load_panel as lds[r][k] (stride kLdsStride).
+template <int R>
+__device__ inline void gemm_store_panel(const uint4 (&v)[R / 64], uint16_t* lds, bool kcontig,
+                                        int tid) {
+#pragma unroll
+  for (int t = 0; t < R / 64; ++t) {
+    const int q = tid + kBlock * t;
+    if (kcontig) {
+      *reinterpret_cast<uint4*>(&lds[(q >> 2) * kLdsStride + 8 * (q & 3)]) = v[t];
+    } else {
+      const int r = 8 * (q % (R / 8)), k = q / (R / 8);
+      const uint32_t w[4] = {v[t].x, v[t].y, v[t].z, v[t].w};
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        lds[(r + j) * kLdsStride + k] = (uint16_t)(w[j >> 1] >> (16 * (j & 1)));
+    }
+  }
+}
+
+template <int WM, int WN>
 __global__ void __launch_bounds__(kBlock)
 gemm_bf16_kernel(int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A, int64_t lda,
                  int transA, const uint16_t* __restrict__ B, int64_t ldb, int transB,
                  float* __restrict__ C, int64_t ldc, const float* __restrict__ bias, int ksplit,
                  float* __restrict__ part) {
-  const int64_t tiles_n = (N + 15) / 16;
-  const int64_t tiles = ((M + 15) / 16) * tiles_n;
-  const int64_t wave = (int64_t)blockIdx.x * (kBlock / 64) + threadIdx.x / 64;
-  if (wave >= tiles * ksplit) return;
-  const int ks = (int)(wave / tiles);
-  const int64_t t = wave % tiles;
-  const int64_t tm = t / tiles_n, tn = t % tiles_n;
-  const int lane = threadIdx.x & 63;
-  const int64_t ksteps = (K + 15) / 16;
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  __shared__ __align__(16) uint16_t As[2][BM * kLdsStride];   // [buffer][i][k]
+  __shared__ __align__(16) uint16_t Bs[2][BN * kLdsStride];   // [buffer][j][k]
+  const int64_t tiles_n = (N + BN - 1) / BN;
+  const int64_t tiles = ((M + BM - 1) / BM) * tiles_n;
+  const int64_t blk = blockIdx.x;
+  const int ks = (int)(blk / tiles);
+  const int64_t t = blk % tiles;
+  const int64_t m0 = (t / tiles_n) * BM, n0 = (t % tiles_n) * BN;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 16 * WM, wn = (wave & 1) * 16 * WN;
+  const int64_t ksteps = (K + kTileK - 1) / kTileK;
   const int64_t per = (ksteps + ksplit - 1) / ksplit;
-  const int64_t k_lo = ks * per * 16;
-  const int64_t k_hi = min(K, (ks + 1) * per * 16);
-  const int64_t ai = tm * 16 + (lane & 15);   // row of op(A) this lane feeds
-  const int64_t bj = tn * 16 + (lane & 15);   // column of op(B) this lane feeds
-  const int kq = (lane >> 4) * 4;             // first of this lane's 4 k values
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  for (int64_t k0 = k_lo; k0 < k_hi; k0 += 16) {
-    bf16x4 a, b;
+  const int64_t k_lo = (int64_t)ks * per * kTileK;
+  const int64_t k_hi = min(K, ((int64_t)ks + 1) * per * kTileK);
+  // op(A) rows are k-contiguous in memory unless transA; op(B) columns unless !transB
+  const bool a_kc = !transA, b_kc = transB != 0;
+  const bool a_vec = (lda % 8 == 0) && ((uintptr_t)A % 16 == 0);
+  const bool b_vec = (ldb % 8 == 0) && ((uintptr_t)B % 16 == 0);
+  f32x4 acc[WM][WN];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t k = k0 + kq + j;
-      uint16_t va = 0, vb = 0;
-      if (k < k_hi) {
-        if (ai < M) va = transA ? A[k * lda + ai] : A[ai * lda + k];
-        if (bj < N) vb = transB ? B[bj * ldb + k] : B[k * ldb + bj];
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  uint4 ra[BM / 64], rb[BN / 64];
+  if (k_lo < k_hi) {
+    gemm_load_panel<BM>(ra, A, lda, a_kc, a_vec, m0, M, k_lo, k_hi, tid);
+    gemm_load_panel<BN>(rb, B, ldb, b_kc, b_vec, n0, N, k_lo, k_hi, tid);
+    gemm_store_panel<BM>(ra, As[0], a_kc, tid);
+    gemm_store_panel<BN>(rb, Bs[0], b_kc, tid);
+  }
+  __syncthreads();
+  const int kq = 8 * (lane >> 4);
+  int cur = 0;
+  for (int64_t k0 = k_lo; k0 < k_hi; k0 += kTileK) {
+    const bool more = k0 + kTileK < k_hi;
+    if (more) {
+      gemm_load_panel<BM>(ra, A, lda, a_kc, a_vec, m0, M, k0 + kTileK, k_hi, tid);
+      gemm_load_panel<BN>(rb, B, ldb, b_kc, b_vec, n0, N, k0 + kTileK, k_hi, tid);
+    }
+    bf16x8 av[WM], bv[WN];
+#pragma unroll
+    for (int a = 0; a < WM; ++a)
+      av[a] = *reinterpret_cast<const bf16x8*>(
+          &As[cur][(wm + 16 * a + (lane & 15)) * kLdsStride + kq]);
+#pragma unroll
+    for (int b = 0; b < WN; ++b)
+      bv[b] = *reinterpret_cast<const bf16x8*>(
+          &Bs[cur][(wn + 16 * b + (lane & 15)) * kLdsStride + kq]);
+#pragma unroll
+    for (int a = 0; a < WM; ++a)
+#pragma unroll
+      for (int b = 0; b < WN; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+    if (more) {
+      // the other buffer was last read before the previous step's barrier
+      gemm_store_panel<BM>(ra, As[cur ^ 1], a_kc, tid);
+      gemm_store_panel<BN>(rb, Bs[cur ^ 1], b_kc, tid);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < WM; ++a)
+#pragma unroll
+    for (int b = 0; b < WN; ++b) {
+      const int64_t cj = n0 + wn + 16 * b + (lane & 15);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t ci = m0 + wm + 16 * a + 4 * (lane >> 4) + j;
+        if (ci < M && cj < N) {
+          if (ksplit == 1)
+            C[ci * ldc + cj] = acc[a][b][j] + (bias ? bias[cj] : 0.f);
+          else
+            part[((int64_t)ks * M + ci) * N + cj] = acc[a][b][j];
+        }
       }
-      a[j] = (short)va;
-      b[j] = (short)vb;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-  }
-  // D layout: lane holds column (lane & 15), rows 4 * (lane >> 4) + j.
-  const int64_t cj = tn * 16 + (lane & 15);
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int64_t ci = tm * 16 + (lane >> 4) * 4 + j;
-    if (ci < M && cj < N) {
-      if (ksplit == 1)
-        C[ci * ldc + cj] = acc[j] + (bias ? bias[cj] : 0.f);
-      else
-        part[((int64_t)ks * M + ci) * N + cj] = acc[j];
-    }
-  }
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -163,14 +264,38 @@ loss_kernel(int64_t n, const float* __restrict__ y, int64_t ldy, const float* __
   if (threadIdx.x == 0) partial[blockIdx.x] = acc[0] / wsum;
 }
 
-// out[j] = sum_i x[i*cols + j] over rows, in row order (one thread per column).
-__global__ void __launch_bounds__(kBlock)
-colsum_kernel(int64_t n, int64_t cols, const float* __restrict__ x, float* __restrict__ out) {
-  const int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (j >= cols) return;
-  double s = 0.0;
-  for (int64_t i = 0; i < n; ++i) s += x[i * cols + j];
-  out[j] = (float)s;
+// out[j] = sum_i x[i*cols + j]. A workgroup of 1024 threads owns cw columns
+// (cw = 16, or the next power of two >= cols when cols < 16) and splits the
+// rows into 1024 / cw interleaved groups; each thread sums its group in FP64
+// (four partial sums in a fixed order) and the groups are combined by a fixed
+// tree in LDS, so the result does not depend on timing.
+constexpr int kColsumThreads = 1024;
+
+__global__ void __launch_bounds__(kColsumThreads)
+colsum_kernel(int64_t n, int64_t cols, int cw, const float* __restrict__ x,
+              float* __restrict__ out) {
+  __shared__ double part[kColsumThreads];
+  const int tid = threadIdx.x;
+  const int c = tid % cw, g = tid / cw, groups = kColsumThreads / cw;
+  const int64_t j = (int64_t)blockIdx.x * cw + c;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  if (j < cols) {
+    int64_t i = g;
+    for (; i + 3 * groups < n; i += 4 * groups) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) s[u] += x[(i + u * groups) * cols + j];
+    }
+    if (i < n) s[0] += x[i * cols + j];
+    if (i + groups < n) s[1] += x[(i + groups) * cols + j];
+    if (i + 2 * groups < n) s[2] += x[(i + 2 * groups) * cols + j];
+  }
+  part[tid] = (s[0] + s[1]) + (s[2] + s[3]);
+  __syncthreads();
+  for (int half = groups / 2; half >= 1; half /= 2) {
+    if (g < half) part[tid] += part[tid + half * cw];
+    __syncthreads();
+  }
+  if (g == 0 && j < cols) out[j] = (float)part[tid];
 }
 
 // Adam (Kingma & Ba) on FP32 master weights, refreshing their bf16 copy.
@@ -326,12 +451,17 @@ int sqdk_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t l
   SQDK_REQUIRE(ksplit >= 1 && ksplit <= 4096, "ksplit %d out of range", ksplit);
   SQDK_REQUIRE(ksplit == 1 || (work && work_bytes >= sizeof(float) * (size_t)ksplit * M * N),
                "GEMM workspace too small for ksplit %d", ksplit);
-  const int64_t waves = ((M + 15) / 16) * ((N + 15) / 16) * ksplit;
-  const int64_t blocks = ceil_div(waves, kBlock / 64);
+  // 128 x 128 tiles where they still give at least four workgroups per CU, else 64 x 64
+  const int64_t big = ceil_div(M, 128) * ceil_div(N, 128) * ksplit;
+  const int64_t blocks = big >= 1024 ? big : ceil_div(M, 64) * ceil_div(N, 64) * ksplit;
   SQDK_REQUIRE(blocks < (1ll << 31), "GEMM grid too large");
   hipStream_t s = as_stream(stream);
-  gemm_bf16_kernel<<<(unsigned)blocks, kBlock, 0, s>>>(M, N, K, A, lda, trans_a, B, ldb, trans_b,
-                                                       C, ldc, bias, ksplit, work);
+  if (big >= 1024)
+    gemm_bf16_kernel<4, 4><<<(unsigned)blocks, kBlock, 0, s>>>(
+        M, N, K, A, lda, trans_a, B, ldb, trans_b, C, ldc, bias, ksplit, work);
+  else
+    gemm_bf16_kernel<2, 2><<<(unsigned)blocks, kBlock, 0, s>>>(
+        M, N, K, A, lda, trans_a, B, ldb, trans_b, C, ldc, bias, ksplit, work);
   SQDK_LAUNCHED("gemm_bf16_kernel");
   if (ksplit > 1) {
     gemm_reduce_kernel<<<grid_for(M * N), kBlock, 0, s>>>(M, N, ksplit, work, C, ldc, bias);
@@ -395,7 +525,10 @@ int sqdk_mlp_loss(int64_t n, const float* y, int64_t ldy, const float* target, d
 
 int sqdk_colsum(int64_t n, int64_t cols, const float* x, float* out, void* stream) {
   SQDK_REQUIRE(x && out && n >= 1 && cols >= 1, "bad colsum arguments");
-  colsum_kernel<<<grid_for(cols), kBlock, 0, as_stream(stream)>>>(n, cols, x, out);
+  int cw = 1;
+  while (cw < 16 && cw < cols) cw *= 2;
+  colsum_kernel<<<(unsigned)ceil_div(cols, cw), kColsumThreads, 0, as_stream(stream)>>>(
+      n, cols, cw, x, out);
   SQDK_LAUNCHED("colsum_kernel");
   return SQDK_OK;
 }

@@ -239,22 +239,27 @@ class StartNet:
                                    dy32.data_ptr(), dy16.data_ptr(), part.data_ptr(), nblocks, s),
                    "sqdk_mlp_loss")
         gw1, gb1, gw2, gb2, gw3, gb3 = self.grads
-        ks = max(1, min(64, n // 512))     # K-split of the batch for the weight gradients
+        def ks(m, cols, k):
+            # K-split of the batch for the weight gradients (m x cols outputs):
+            # about 256 workgroups of 64 x 64, at least 128 samples each
+            tiles = -(-m // 64) * -(-cols // 64)
+            return max(1, min(k // 128, 256 // tiles))
         # output layer: dW3 = H2^T dY, db3 = colsum dY, dH2 = dY W3^T
-        self._gemm(self.h, 2, n, h2, self.h, 1, dy16, 2, 0, gw3, 2, ksplit=ks)
+        self._gemm(self.h, 2, n, h2, self.h, 1, dy16, 2, 0, gw3, 2, ksplit=ks(self.h, 2, n))
         _lib.check(L.sqdk_colsum(n, 2, dy32.data_ptr(), gb3.data_ptr(), s), "sqdk_colsum")
         dh2 = torch.empty((n, self.h), dtype=torch.float32, device=self.device)
         self._gemm(n, self.h, 2, dy16, 2, 0, self.bf16[4], 2, 1, dh2, self.h)
         dz2_16, dz2 = self._act(1, z2, dh2, want16=True, want32=True)
         # hidden layer: dW2 = H1^T dZ2, db2 = colsum dZ2, dH1 = dZ2 W2^T
-        self._gemm(self.h, self.h, n, h1, self.h, 1, dz2_16, self.h, 0, gw2, self.h, ksplit=ks)
+        self._gemm(self.h, self.h, n, h1, self.h, 1, dz2_16, self.h, 0, gw2, self.h,
+                   ksplit=ks(self.h, self.h, n))
         _lib.check(L.sqdk_colsum(n, self.h, dz2.data_ptr(), gb2.data_ptr(), s), "sqdk_colsum")
         dh1 = torch.empty_like(dh2)
         self._gemm(n, self.h, self.h, dz2_16, self.h, 0, self.bf16[2], self.h, 1, dh1, self.h)
         dz1_16, dz1 = self._act(1, z1, dh1, want16=True, want32=True)
         # input layer: dW1 = X^T dZ1, db1 = colsum dZ1
         self._gemm(self.kpad, self.h, n, x, self.kpad, 1, dz1_16, self.h, 0, gw1, self.h,
-                   ksplit=ks)
+                   ksplit=ks(self.kpad, self.h, n))
         _lib.check(L.sqdk_colsum(n, self.h, dz1.data_ptr(), gb1.data_ptr(), s), "sqdk_colsum")
         self._cache = None
         return part.sum()
@@ -280,8 +285,10 @@ class StartNet:
         half uniformly over the concatenated instances; each sample carries the
         importance weight (w + floor) / (w/2 + 1/2) (weights have mean 1), an
         unbiased estimate of the weighted loss plus ``floor`` times a uniform
-        one that keeps every state's amplitude supervised. Returns the losses
-        (host floats) of every ``max(1, log_every)``-th step."""
+        one that keeps every state's amplitude supervised. Samples are drawn 64
+        steps at a time and the losses stay on the device, so the step loop
+        waits on the GPU only where ``log_every`` prints. Returns the losses
+        (host floats) of every ``max(1, log_every)``-th step and the last."""
         keys = torch.cat([k.to(self.device, torch.int64) for k in keys_list]).contiguous()
         tgts = torch.cat([t.to(self.device, torch.float32) for t in targets_list]).contiguous()
         extras = None
@@ -296,24 +303,33 @@ class StartNet:
         total = float(cdf[-1].item())
         gen = torch.Generator(device=self.device).manual_seed(seed)
         half = batch // 2
-        losses = []
         every = max(1, log_every)
-        for step in range(steps):
-            u = torch.rand(half, generator=gen, device=self.device, dtype=torch.float64) * total
-            i1 = torch.searchsorted(cdf, u).clamp_(max=n_all - 1)
-            i2 = torch.randint(0, n_all, (batch - half,), generator=gen, device=self.device)
-            idx = torch.cat([i1, i2])
-            tb = tgts[idx]
-            wi = w_all[idx] / mean_w
-            tb[:, 2] = ((wi + floor) / (0.5 * wi + 0.5)).to(torch.float32)
-            loss = self.loss_grads(keys[idx], tb.contiguous(), wsum=float(batch),
-                                   extra=None if extras is None else extras[idx].contiguous())
-            self.adam_step(lr=lr * 0.5 * (1.0 + math.cos(math.pi * step / steps)))
-            if step % every == 0 or step == steps - 1:
-                losses.append(float(loss.item()))
-                if log_every:
-                    print(f"[startnet] step {step} loss {losses[-1]:.4f}", flush=True)
-        return losses
+        loss_buf = torch.zeros(steps, dtype=torch.float64, device=self.device)
+        chunk = 64
+        for c0 in range(0, steps, chunk):
+            cn = min(chunk, steps - c0)
+            # one draw for a chunk of steps: no host round trip inside the loop
+            u = torch.rand((cn, half), generator=gen, device=self.device,
+                           dtype=torch.float64) * total
+            i1 = torch.searchsorted(cdf, u.reshape(-1)).clamp_(max=n_all - 1).view(cn, half)
+            i2 = torch.randint(0, n_all, (cn, batch - half), generator=gen, device=self.device)
+            idx_all = torch.cat([i1, i2], dim=1)
+            wi_all = w_all[idx_all] / mean_w
+            imp_all = ((wi_all + floor) / (0.5 * wi_all + 0.5)).to(torch.float32)
+            for j in range(cn):
+                step = c0 + j
+                idx = idx_all[j]
+                tb = tgts[idx]
+                tb[:, 2] = imp_all[j]
+                loss = self.loss_grads(
+                    keys[idx], tb, wsum=float(batch),
+                    extra=None if extras is None else extras[idx].contiguous())
+                self.adam_step(lr=lr * 0.5 * (1.0 + math.cos(math.pi * step / steps)))
+                loss_buf[step] = loss
+                if log_every and step % every == 0:
+                    print(f"[startnet] step {step} loss {float(loss.item()):.4f}", flush=True)
+        keep = sorted(set(range(0, steps, every)) | {steps - 1}) if steps else []
+        return loss_buf[keep].tolist()
 
     def infer(self, keys: torch.Tensor, extra: torch.Tensor | None = None) -> torch.Tensor:
         """Outputs y (n x 2, FP32) through the fused inference kernel

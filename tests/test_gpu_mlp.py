@@ -37,14 +37,28 @@ def _f32(a, dev):
     return torch.tensor(np.asarray(a, dtype=np.float32), device=dev)
 
 
-@pytest.mark.parametrize("M,N,K,ksplit", [(16, 16, 16, 1), (37, 45, 29, 1), (64, 2, 256, 1),
-                                          (256, 256, 1000, 5), (5, 300, 64, 2)])
+def _round8(x):
+    return (x + 7) // 8 * 8
+
+
+# (M, N, K, ksplit); the last two shapes take the 128 x 128 tiles (at least
+# 1024 of them), the others the 64 x 64 ones
+_GEMM_SHAPES = [(16, 16, 16, 1), (37, 45, 29, 1), (64, 2, 256, 1), (256, 256, 1000, 5),
+                (5, 300, 64, 2), (4097, 4100, 72, 1), (1000, 1100, 333, 16)]
+
+
+@pytest.mark.parametrize("M,N,K,ksplit", _GEMM_SHAPES)
 @pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_gemm_bf16_matches_numpy(gpu, M, N, K, ksplit, ta, tb):
-    print(f"gemm M={M} N={N} K={K} ksplit={ksplit} ta={ta} tb={tb}", flush=True)
+@pytest.mark.parametrize("aligned", [False, True])
+def test_gemm_bf16_matches_numpy(gpu, M, N, K, ksplit, ta, tb, aligned):
+    """Leading dimensions padded to an odd size take the element-wise loads;
+    multiples of 8 take the 16-byte loads (ragged edges element-wise)."""
+    print(f"gemm M={M} N={N} K={K} ksplit={ksplit} ta={ta} tb={tb} aligned={aligned}", flush=True)
     rng = np.random.default_rng(M * 7 + N * 3 + K + 2 * ta + tb)
     opa, opb = rng.normal(size=(M, K)), rng.normal(size=(K, N))
     lda, ldb, ldc = (M if ta else K) + 3, (K if tb else N) + 1, N + 2
+    if aligned:
+        lda, ldb = _round8(M if ta else K), _round8(K if tb else N)
     a_store = np.zeros((K, lda) if ta else (M, lda))
     b_store = np.zeros((N, ldb) if tb else (K, ldb))
     if ta:
@@ -88,6 +102,27 @@ def test_gemm_bf16_rejects_bad_arguments(gpu):
     with pytest.raises(ValueError):
         _lib.check(L.sqdk_gemm_bf16(0, 16, 16, x.data_ptr(), 16, 0, x.data_ptr(), 16, 0,
                                     y.data_ptr(), 16, None, 1, None, 0, _lib.stream_ptr(gpu)), "g")
+
+
+@pytest.mark.parametrize("n,cols", [(1, 1), (7, 3), (4096, 256), (5000, 2), (1000, 37),
+                                    (33, 100), (4099, 17)])
+def test_colsum_matches_numpy(gpu, n, cols):
+    """Column sums (the bias gradients) against an FP64 sum; a second launch
+    gives the same bits (fixed summation order)."""
+    rng = np.random.default_rng(n + cols)
+    x = rng.normal(size=(n, cols)).astype(np.float32)
+    xt = _f32(x, gpu)
+    outs = []
+    for _ in range(2):
+        out = torch.full((cols + 1,), 5.0, dtype=torch.float32, device=gpu)
+        _lib.check(_lib.lib().sqdk_colsum(n, cols, xt.data_ptr(), out.data_ptr(),
+                                          _lib.stream_ptr(gpu)), "colsum")
+        outs.append(out.cpu().numpy())
+    ref = x.astype(np.float64).sum(axis=0)
+    scale = np.abs(x).astype(np.float64).sum(axis=0)
+    assert np.all(np.abs(outs[0][:cols] - ref) <= 1e-6 * scale + 1e-30)
+    assert outs[0][cols] == 5.0
+    assert np.array_equal(outs[0], outs[1])
 
 
 @pytest.mark.parametrize("n_extra", [0, 2])
