@@ -507,6 +507,39 @@ TTS_RUNS = {
 }
 
 
+GRAPH_RUNS = ("fused_c64", "cached_fused_c64_diag_precond")
+
+
+def graph_block(prob, x0, runs, tol, reps=3):
+    """The fused solve launched directly and with each refresh period replayed
+    as one captured HIP graph (use_graph): best of ``reps`` wall times, and
+    whether the two give the same eigenvalue bits and iteration count."""
+    out = {}
+    for name in runs:
+        mode, prec, precond, cached = TTS_RUNS[name]
+        prob.op.use_cache = cached
+        rec = {}
+        for graph in (False, True):
+            best, info = None, None
+            for _ in range(reps):
+                sync()
+                t0 = time.perf_counter()
+                _, info = prob.solve(x0, mode=mode, precision=prec, precond=precond,
+                                     criterion="relative", tol=tol, maxiter=3000,
+                                     use_graph=graph)
+                sync()
+                dt = time.perf_counter() - t0
+                best = dt if best is None else min(best, dt)
+            rec["graph" if graph else "direct"] = {"solve_s": round(best, 5),
+                                                   "iterations": info.niter,
+                                                   "energy": info.eigval}
+        rec["identical"] = (rec["graph"]["energy"] == rec["direct"]["energy"] and
+                            rec["graph"]["iterations"] == rec["direct"]["iterations"])
+        rec["graph_speedup"] = round(rec["direct"]["solve_s"] / rec["graph"]["solve_s"], 3)
+        out[name] = rec
+    return out
+
+
 def time_to_solution(prob, w, dev, setup_s, runs=None, tol=1e-6):
     """Time to a ``tol`` relative residual from one seeded Gaussian start.
 
@@ -546,6 +579,10 @@ def time_to_solution(prob, w, dev, setup_s, runs=None, tol=1e-6):
                      "resid_rel": info.resid_norm / abs(info.eigval)}
         log(f"[{w.name}] {name}: {total:.3f} s ({dt:.3f} s solve), {info.niter} it, "
             f"{info.matvecs} matvecs, E={info.eigval:.12f}")
+    if all(k in runs for k in GRAPH_RUNS):
+        res["graph_replay"] = graph_block(prob, x0, GRAPH_RUNS, tol)
+        log(f"[{w.name}] graph replay speedup: " + ", ".join(
+            f"{k} {v['graph_speedup']}" for k, v in res["graph_replay"].items()))
     op.use_cache = True
     if _lib.C128 not in op._caches:
         op.build_cache(_lib.C128)
@@ -620,6 +657,10 @@ def c1_block(dev):
         res[name] = {"seconds": round(setup_s + dt, 5), "solve_s": round(dt, 5),
                      "iterations": info.niter, "energy": info.eigval,
                      "converged": info.converged}
+    from rqutils_amd import _lib
+    prob.op.build_cache(_lib.C64)
+    res["graph_replay"] = graph_block(prob, x0, GRAPH_RUNS, 1e-6)
+    prob.op.drop_cache()
     x, z, c, _ = so.from_paulisum(w.labels, w.coeffs, force_real=True, add_padding=True)
     packed = so.pack_states(keys_to_states(w.keys, w.num_qubits))
     t0 = time.perf_counter()

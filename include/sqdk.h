@@ -443,7 +443,12 @@ typedef struct sqdk_locg_opts {
   int64_t n_for_tol;       /* vector length used by SQDK_CRIT_REFERENCE */
   int32_t check_every;     /* host convergence poll period in iterations (>=1) */
   int32_t refresh_every;   /* FUSED: recompute A x and A p explicitly every k iterations */
-  int32_t use_graph;       /* capture check_every iterations as one HIP graph */
+  int32_t use_graph;       /* FUSED with the built-in operator: capture one refresh
+                              period (refresh_every iterations and the refresh) as a
+                              HIP graph, replayed on a private stream after the
+                              caller's stream is synchronised; the host polls once
+                              per period. The batched solve captures refresh_every
+                              iterations the same way */
   int32_t precond;         /* sqdk_locg_precond (FUSED only) */
   int32_t reserved;
   double* debug;           /* REFERENCE only, or NULL: device buffer of (maxiter + 2) x 30

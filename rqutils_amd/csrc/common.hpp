@@ -173,26 +173,52 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
+constexpr int pow2_at_least(int v) { return v <= 1 ? 1 : 2 * pow2_at_least((v + 1) / 2); }
+
 // Block-wide sum of NV doubles in a fixed order; result valid in thread 0.
 // `red` must hold NV * (blockDim.x / 64) doubles of shared memory.
+//
+// Within a wave the NV values (padded to P, a power of two) are summed by
+// recursive halving: at the exchange over lane bit m a lane keeps half of
+// its values (the lower half if that bit is 0) and receives its partner's
+// share of that half, so P/2 + P/4 + ... + 1 shuffles leave lane L with the
+// wave's total of value L / (64 / P), followed by log2(64 / P) plain
+// exchanges; about P shuffles instead of 6 NV. Lanes L with L % (64 / P) == 0
+// then store their value, and thread 0 adds the waves' values in wave order.
 template <int NV>
 __device__ inline void block_sum(double (&v)[NV], double* red) {
+  static_assert(NV >= 1 && NV <= kWave, "block_sum handles 1..64 values");
+  constexpr int P = pow2_at_least(NV);
   const int lane = threadIdx.x & (kWave - 1);
   const int wid = threadIdx.x / kWave;
   const int nw = blockDim.x / kWave;
+  double w[P];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  for (int k = 0; k < P; ++k) w[k] = k < NV ? v[k] : 0.0;
+  int m = kWave / 2;
+#pragma unroll
+  for (int h = P / 2; h >= 1; h /= 2, m /= 2) {
+    const bool upper = (lane & m) != 0;
+#pragma unroll
+    for (int k = 0; k < h; ++k) {
+      const double send = upper ? w[k] : w[k + h];
+      const double keep = upper ? w[k + h] : w[k];
+      w[k] = keep + __shfl_xor(send, m);
+    }
+  }
+  for (; m > 0; m /= 2) w[0] += __shfl_xor(w[0], m);
   __syncthreads();
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < NV; ++k) red[k * nw + wid] = v[k];
+  constexpr int kLanesPer = kWave / P;
+  if (lane % kLanesPer == 0) {
+    const int k = lane / kLanesPer;
+    if (k < NV) red[k * nw + wid] = w[0];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double s = 0.0;
-      for (int w = 0; w < nw; ++w) s += red[k * nw + w];
+      for (int q = 0; q < nw; ++q) s += red[k * nw + q];
       v[k] = s;
     }
   }

@@ -391,6 +391,7 @@ struct Solver {
   size_t mv_bytes = 0;
   ScalarParams prm;
   int matvecs = 0;
+  bool use_graph = false;   // FUSED: replay refresh periods as a captured HIP graph
 
   int apply(const void* in, void* out, bool skip) {
     if (op->fn) {
@@ -604,9 +605,8 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
   // and A W are read after the product.
   bool full_gram = true;
   double* gram_out = reinterpret_cast<double*>(sv.S + kGram);
-  while (true) {
-    const int chunk = std::min(o.check_every, o.maxiter - done);
-    for (int it = 0; it < chunk; ++it) {
+  auto iterate = [&](int count) -> int {
+    for (int it = 0; it < count; ++it) {
       SQDK_TRY(sv.apply(W, AW, true));
       if (full_gram)
         SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, res, gram_out, sv.partials, sv.s,
@@ -628,7 +628,48 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
         SQDK_TRY(sv.lin(R, {AX, X}, {kOne, C0}, true));
         SQDK_TRY(precondition(true));
       }
-      if (sv.op->fn && it + 1 < chunk) break;
+      if (sv.op->fn && it + 1 < count) break;
+    }
+    return SQDK_OK;
+  };
+  // With a graph, one refresh period (refresh_every iterations, the first
+  // with the full Gram pass, the last followed by the explicit refresh) is
+  // captured once and replayed whenever a period starts and fits below
+  // maxiter; the host then polls once per period. Every kernel in it skips
+  // its work once the convergence flag is set.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  struct GraphGuard {
+    hipGraph_t& g;
+    hipGraphExec_t& e;
+    ~GraphGuard() {
+      if (e) hipGraphExecDestroy(e);
+      if (g) hipGraphDestroy(g);
+    }
+  } guard{graph, exec};
+  const bool graphs = sv.use_graph && o.refresh_every > 0 && !sv.op->fn;
+  while (true) {
+    const int left = o.maxiter - done;
+    if (graphs && full_gram && since_refresh == 0 && left >= o.refresh_every) {
+      if (!exec) {
+        SQDK_HIP(hipStreamBeginCapture(sv.s, hipStreamCaptureModeThreadLocal));
+        const int rc = iterate(o.refresh_every);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(sv.s, &g);
+        if (rc != SQDK_OK) {
+          if (g) hipGraphDestroy(g);
+          return rc;
+        }
+        if (e != hipSuccess)
+          return fail(SQDK_EHIP, "graph capture failed: %s", hipGetErrorString(e));
+        graph = g;
+        SQDK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      }
+      SQDK_HIP(hipGraphLaunch(exec, sv.s));
+      since_refresh = 0;   // the period ends with the refresh
+      full_gram = true;
+    } else {
+      SQDK_TRY(iterate(std::min(o.check_every, left)));
     }
     SQDK_TRY(sv.read_flags(flags_host));
     done = flags_host[kNiter];
@@ -712,6 +753,24 @@ int sqdk_locg_solve(const sqdk_op* op, int64_t n, void* x, const sqdk_locg_opts*
   sv.n = n;
   sv.prec = o.precision;
   sv.s = as_stream(stream);
+  sv.use_graph = o.use_graph && o.mode == SQDK_LOCG_FUSED && !op->fn && o.refresh_every > 0 &&
+                 o.maxiter >= o.refresh_every;
+  // Graph capture needs a stream other than the legacy default one: a private
+  // stream, after the caller's has finished what it queued (the start vector).
+  struct OwnStream {
+    hipStream_t s = nullptr;
+    ~OwnStream() {
+      if (s) {
+        hipStreamSynchronize(s);
+        hipStreamDestroy(s);
+      }
+    }
+  } own;
+  if (sv.use_graph) {
+    SQDK_HIP(hipStreamSynchronize(sv.s));
+    SQDK_HIP(hipStreamCreateWithFlags(&own.s, hipStreamNonBlocking));
+    sv.s = own.s;
+  }
   char* w = static_cast<char*>(work);
   sv.S = reinterpret_cast<double2*>(w);
   w += align256(kSlotBytes);

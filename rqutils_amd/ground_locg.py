@@ -131,7 +131,7 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
           criterion: str = "relative", tol: float = 1e-6, maxiter: int = 1000,
           n_for_tol: int | None = None, check_every: int = 4, refresh_every: int = 16,
           real: bool = False, precond: str | None = None,
-          debug: bool | str = False) -> tuple[torch.Tensor, LocgInfo]:
+          debug: bool | str = False, use_graph: bool = False) -> tuple[torch.Tensor, LocgInfo]:
     """Lowest eigenpair of ``op`` from start vector ``x0`` (device tensor).
 
     ``op`` is a PauliOperator or a callable on device tensors. Returns the
@@ -140,6 +140,9 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     residual scaled by ``1 / max(d - min(theta, min d), floor)``, ``d`` the
     diagonal from the operator's X = 0 terms and ``floor = max(0.02 (mean d -
     min d), 1e-3 |theta|)``.
+
+    ``use_graph`` (fused mode, PauliOperator): replay each refresh period of
+    ``refresh_every`` iterations as one captured HIP graph on a private stream.
 
     ``debug`` (reference mode): record the reference's per-step diagnostics
     (``LocgInfo.diagnostics``, see ``diagnostics_dict``); ``"vectors"`` also
@@ -165,7 +168,7 @@ def solve(op, x0: torch.Tensor, *, mode: str = "fused", precision: str | None = 
     opts.n_for_tol = int(n_for_tol if n_for_tol is not None else n)
     opts.check_every = int(check_every)
     opts.refresh_every = int(refresh_every)
-    opts.use_graph = 0
+    opts.use_graph = 1 if use_graph else 0
     if precond not in _PRECONDS:
         raise ValueError(f"precond must be one of {sorted(_PRECONDS, key=str)}")
     opts.precond = _PRECONDS[precond]
