@@ -98,6 +98,12 @@ __device__ inline int remap_xcd(int i, int g) {
 }
 
 // Byte layout of the dynamic LDS tables of one workgroup.
+// The staged key slice is followed by kKeyPad entries of the key type's
+// largest value, so a lifting probe up to 2^kKeyPadLog2 - 1 positions past the
+// slice reads a key above every target (targets are below 2^m, m < the key
+// width) and needs no clamp.
+constexpr int kKeyPadLog2 = 6, kKeyPad = 1 << kKeyPadLog2;
+
 struct LdsLayout {
   size_t pairs, occ, keys, vals, total;
 };
@@ -107,7 +113,7 @@ __host__ __device__ inline LdsLayout lds_layout(int fine_bits, int max_bucket, s
   l.pairs = 0;
   l.occ = ((size_t)4 << fine_bits);
   l.keys = l.occ + 16 * sizeof(int);
-  l.vals = (l.keys + key_bytes * ((size_t)max_bucket + 1) + 15) & ~(size_t)15;
+  l.vals = (l.keys + key_bytes * ((size_t)max_bucket + kKeyPad) + 15) & ~(size_t)15;
   l.total = l.vals + val_bytes * (size_t)max_bucket;
   return l;
 }
@@ -293,6 +299,93 @@ __device__ inline void fold_class(Acc<VT> (&acc)[ROWS], VT (&pacc)[ROWS], VT* __
   }
 }
 
+// LDS address of a generic pointer into the workgroup's shared memory, and
+// back: the search keeps 32-bit LDS addresses, which a read takes as is.
+template <typename T>
+using LdsPtr = const T __attribute__((address_space(3)))*;
+
+template <typename T>
+__device__ inline uint32_t lds_addr(const T* p) {
+  return (uint32_t)(uintptr_t)(LdsPtr<T>)p;
+}
+
+template <typename T>
+__device__ inline T lds_load(uint32_t addr) {
+  return *(LdsPtr<T>)(uintptr_t)addr;
+}
+
+// Search of each row's target tl in the staged key slice. On entry ba holds
+// the LDS address of the first key of the target's sub-bucket (staged in the
+// sub-bucket table); binary lifting over 2^steps positions moves it to the
+// last key <= tl: per step an add, one LDS read, a compare and a select. Keys
+// past the sub-bucket are larger than the target, so ba ends on the target if
+// it is present. Strides of 2^kKeyPadLog2 positions or more (a sub-bucket of
+// more than 64 keys) are clamped to the slice's last key, `last`; the smaller
+// ones end at most 63 positions past the slice, in the padding. hit[r]: the
+// key found is the target.
+template <typename KT, int ROWS>
+__device__ inline void lift_search(uint32_t (&ba)[ROWS], const KT (&tl)[ROWS], int steps,
+                                   uint32_t last, bool (&hit)[ROWS]) {
+  int st = steps - 1;
+  if (st >= kKeyPadLog2) {
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) ba[r] = min(ba[r], last);
+    for (; st >= kKeyPadLog2; --st) {
+      const uint32_t step = (uint32_t)sizeof(KT) << st;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        const uint32_t pa = min(ba[r] + step, last);
+        ba[r] = lds_load<KT>(pa) <= tl[r] ? pa : ba[r];
+      }
+    }
+  }
+  for (; st >= 0; --st) {
+    const uint32_t step = (uint32_t)sizeof(KT) << st;
+    uint32_t pa[ROWS];
+    KT kv[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) pa[r] = ba[r] + step;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) kv[r] = lds_load<KT>(pa[r]);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) ba[r] = kv[r] <= tl[r] ? pa[r] : ba[r];
+  }
+  KT kb[ROWS];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) kb[r] = lds_load<KT>(ba[r]);
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r) hit[r] = kb[r] == tl[r];
+}
+
+// Stages the target bucket's slice for the search: the low key bits, the
+// padding, and per sub-bucket the LDS byte address of its first key; returns
+// the lifting steps, ceil(log2(largest sub-bucket)). Every thread of the
+// workgroup calls it (two barriers).
+template <typename KT, int NT>
+__device__ inline int stage_search(const uint64_t* __restrict__ keys, uint64_t qs, int qn,
+                                   uint64_t lowmask, const uint16_t* __restrict__ fdr,
+                                   uint64_t fbase, int ndir, uint32_t kaddr, KT* lkeys,
+                                   uint32_t* lpair, int* s_occ, int tid) {
+  __syncthreads();  // the previous class is done with the LDS tables
+  for (int i = tid; i < qn; i += NT) lkeys[i] = (KT)(keys[qs + i] & lowmask);
+  for (int i = tid; i < kKeyPad; i += NT) lkeys[qn + i] = (KT)~(KT)0;
+  int occ = 0;
+  for (int e = tid; e < ndir; e += NT) {
+    const uint32_t lo = fdr[fbase + e];
+    const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
+    lpair[e] = kaddr + lo * (uint32_t)sizeof(KT);
+    occ = max(occ, (int)(hi - lo));
+  }
+#pragma unroll
+  for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
+  if ((tid & (kWave - 1)) == 0) s_occ[tid / kWave] = occ;
+  __syncthreads();
+  int maxocc = 0;
+#pragma unroll
+  for (int i = 0; i < NT / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
+  return maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
+}
+
 // Occupancy: at 4 rows per thread the complex64 kernel with 32-bit LDS keys
 // keeps 7 waves per SIMD (at most 72 VGPRs, no spills), the others 5 (at most
 // 96 VGPRs).
@@ -342,6 +435,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   uint32_t* lpair = reinterpret_cast<uint32_t*>(smem + lay.pairs);
   int* s_occ = reinterpret_cast<int*>(smem + lay.occ);
   KT* lkeys = reinterpret_cast<KT*>(smem + lay.keys);
+  const uint32_t kaddr = lds_addr(lkeys);  // LDS address of lkeys[0]
   VT* lv = reinterpret_cast<VT*>(smem + lay.vals);
   // Dense lookup: (occupancy bitmap, first-key offset) of every 32-value
   // sub-bucket of the staged target bucket.
@@ -481,30 +575,13 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
       if (class_hit) fold_class<VT, ROWS, COMPACT>(acc, pacc, lpart, lane);
       (void)qn;
     } else {
-      __syncthreads();  // the previous class is done with the LDS tables
-      for (int i = tid; i < qn; i += NT) lkeys[i] = (KT)(keys[qs + i] & lowmask);
-      if (tid == 0) lkeys[qn] = (KT)0;
+      const int steps = stage_search<KT, NT>(keys, qs, qn, lowmask, fdr, q << d, ndir, kaddr,
+                                             lkeys, lpair, s_occ, tid);
       if (STAGE_V) {
         for (int i = tid; i < qn; i += NT) lv[i] = v[qs + i];
+        __syncthreads();
       }
-      // Sub-bucket ranges as (lo | hi << 16) and the largest occupancy.
-      const uint64_t fbase = q << d;
-      int occ = 0;
-      for (int e = tid; e < ndir; e += NT) {
-        const uint32_t lo = fdr[fbase + e];
-        const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
-        lpair[e] = lo | (hi << 16);
-        occ = max(occ, (int)(hi - lo));
-      }
-#pragma unroll
-      for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
-      if ((tid & (kWave - 1)) == 0) s_occ[tid / kWave] = occ;
-      __syncthreads();
-      int maxocc = 0;
-#pragma unroll
-      for (int i = 0; i < NT / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
-      // Steps of the branch-free lower bound: ceil(log2(maxocc)).
-      const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
+      const uint32_t last = kaddr + (uint32_t)(qn - 1) * (uint32_t)sizeof(KT);
       if (!wave_active) continue;  // the barriers above are still reached by every wave
       const VT* __restrict__ vq = v + qs;
       const __amdgpu_buffer_rsrc_t vr = slice_rsrc(vq, qn);
@@ -528,34 +605,23 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
         if (gt1 <= gt0) continue;  // a group whose terms all had zero coefficients
         const KT xl = (KT)(xm & lowmask);
         KT tl[ROWS];
-        int base[ROWS];
+        uint32_t ba[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          // start of the target's sub-bucket (an empty one at the slice's end
-          // starts on the last key, which cannot equal the target)
-          base[r] = min((int)(lpair[(uint32_t)(tl[r] >> f)] & 0xFFFFu), qn - 1);
+          ba[r] = lpair[(uint32_t)(tl[r] >> f)];
         }
-        // Branch-free search for the last key <= target by binary lifting over
-        // 2^steps >= largest sub-bucket positions, each probe clamped to the
-        // slice's last key: keys past the sub-bucket are larger than the
-        // target, so base ends on the target if it is present. Every read is
-        // in [0, qn).
-        for (int st = steps - 1; st >= 0; --st) {
-#pragma unroll
-          for (int r = 0; r < ROWS; ++r) {
-            const int probe = min(base[r] + (1 << st), qn - 1);
-            base[r] = lkeys[probe] <= tl[r] ? probe : base[r];
-          }
-        }
-        int idx[ROWS];
+        bool hit[ROWS];
+        lift_search<KT, ROWS>(ba, tl, steps, last, hit);
+        // Rows past the chunk's end (sl = 0) may hit too: their products are
+        // formed like the others' and never written.
         bool any_hit = false;
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-          const KT kb = lkeys[base[r]];
-          idx[r] = (valid[r] && kb == tl[r]) ? base[r] : -1;
-          any_hit |= idx[r] >= 0;
-        }
+        for (int r = 0; r < ROWS; ++r) any_hit |= hit[r];
+        int idx[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+          idx[r] = hit[r] ? (int)((ba[r] - kaddr) / (uint32_t)sizeof(KT)) : -1;
         if constexpr (COMPACT) {
           class_hit |= compact_group<VT, KT, ROWS>(idx, lane, lq, lrow, lpart, STAGE_V ? lv : vq,
                                                    zmask, coef, gt0, gt1, bucket_base, narrow);
@@ -737,7 +803,7 @@ int launch_vt(const sqdk_ham* h, const sqdk_space* sp, const void* coef, const v
   args.single.bucket_bits = sp->bucket_bits;
   args.single.fine_bits = sp->fine_bits;
   args.single.fbits_base = 0;
-  SQDK_TRY(launch_common<VT>(t, args, sp->bucket_bits <= 32, rows, stage_v, s));
+  SQDK_TRY(launch_common<VT>(t, args, sp->bucket_bits < 32, rows, stage_v, s));
   return reduce_split<VT>(t, args, s);
 }
 
@@ -784,7 +850,7 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
     ad.counter = nullptr;
     SQDK_TRY(launch_common<VT>(td, ad, 1, rows, 0, s));
   }
-  SQDK_TRY(launch_common<VT>(t, args, b->max_bucket_bits <= 32, rows, stage_v, s));
+  SQDK_TRY(launch_common<VT>(t, args, b->max_bucket_bits < 32, rows, stage_v, s));
   return reduce_split<VT>(t, args, s);
 }
 
@@ -864,6 +930,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   uint32_t* lpair = reinterpret_cast<uint32_t*>(smem + lay.pairs);
   int* s_occ = reinterpret_cast<int*>(smem + lay.occ);
   KT* lkeys = reinterpret_cast<KT*>(smem + lay.keys);
+  const uint32_t kaddr = lds_addr(lkeys);  // LDS address of lkeys[0]
   uint2* ldir = reinterpret_cast<uint2*>(smem);
 
   const uint32_t row0 = items[w];
@@ -970,48 +1037,29 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
         if (__any(any_hit)) record(idx, qs, gt0, gt1);
       }
     } else {
-      __syncthreads();
-      for (int i = tid; i < qn; i += kBlock) lkeys[i] = (KT)(keys[qs + i] & lowmask);
-      const uint64_t fbase = q << d;
-      int occ = 0;
-      for (int e = tid; e < ndir; e += kBlock) {
-        const uint32_t lo = fdir_i[fbase + e];
-        const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdir_i[fbase + e + 1] : (uint32_t)qn;
-        lpair[e] = lo | (hi << 16);
-        occ = max(occ, (int)(hi - lo));
-      }
-#pragma unroll
-      for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
-      if (lane == 0) s_occ[wave] = occ;
-      __syncthreads();
-      int maxocc = 0;
-#pragma unroll
-      for (int i = 0; i < kBlock / kWave; ++i) maxocc = max(maxocc, s_occ[i]);
-      const int steps = maxocc <= 1 ? 0 : 32 - __clz(maxocc - 1);
+      const int steps = stage_search<KT, kBlock>(keys, qs, qn, lowmask, fdir_i, q << d, ndir,
+                                                 kaddr, lkeys, lpair, s_occ, tid);
+      const uint32_t last = kaddr + (uint32_t)(qn - 1) * (uint32_t)sizeof(KT);
       if (!wave_active) continue;
       for (int g = g0; g < g1; ++g) {
         const int gt0 = grp_off[g], gt1 = grp_off[g + 1];
         if (gt1 <= gt0) continue;
         const KT xl = (KT)(xmask[g] & lowmask);
         KT tl[ROWS];
-        int base[ROWS];
+        uint32_t ba[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          base[r] = min((int)(lpair[(uint32_t)(tl[r] >> f)] & 0xFFFFu), qn - 1);
+          ba[r] = lpair[(uint32_t)(tl[r] >> f)];
         }
-        for (int st = steps - 1; st >= 0; --st) {
-#pragma unroll
-          for (int r = 0; r < ROWS; ++r) {
-            const int probe = min(base[r] + (1 << st), qn - 1);
-            base[r] = lkeys[probe] <= tl[r] ? probe : base[r];
-          }
-        }
+        bool hit[ROWS];
+        lift_search<KT, ROWS>(ba, tl, steps, last, hit);
         int idx[ROWS];
         bool any_hit = false;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-          idx[r] = (valid[r] && lkeys[base[r]] == tl[r]) ? base[r] : -1;
+          // only the chunk's rows are recorded
+          idx[r] = (valid[r] && hit[r]) ? (int)((ba[r] - kaddr) / (uint32_t)sizeof(KT)) : -1;
           any_hit |= idx[r] >= 0;
         }
         if (__any(any_hit)) record(idx, qs, gt0, gt1);
@@ -1335,7 +1383,7 @@ HitTables hit_tables_single(const sqdk_ham* h, const sqdk_space* sp, const void*
   t.max_bucket = sp->max_bucket;
   t.max_fine_bits = sp->fine_bits;
   t.dense_fine_bits = sp->fine_bits;
-  t.narrow = sp->bucket_bits <= 32;
+  t.narrow = sp->bucket_bits < 32;
   t.has_dense = sp->fbits != nullptr;
   t.has_search = sp->fbits == nullptr;
   return t;
@@ -1362,7 +1410,7 @@ HitTables hit_tables_batch(const sqdk_batch* b, const void* coef) {
   t.max_bucket = b->max_bucket;
   t.max_fine_bits = b->max_fine_bits;
   t.dense_fine_bits = b->dense_fine_bits;
-  t.narrow = b->max_bucket_bits <= 32;
+  t.narrow = b->max_bucket_bits < 32;
   t.has_dense = b->fbits != nullptr;
   t.has_search = true;   // a launch with no searched chunk returns at once
   return t;

@@ -347,7 +347,7 @@ class PauliOperator:
         off_idx = np.nonzero(~is_diag)[0]
         pruned = 0
         if prune and space.single_parity() and off_idx.size:
-            weights = np.array([bin(int(x)).count("1") for x in xm[off_idx]])
+            weights = np.bitwise_count(xm[off_idx])
             keep = (weights & 1) == 0
             pruned = int(np.count_nonzero(~keep))
             off_idx = off_idx[keep]
@@ -355,26 +355,16 @@ class PauliOperator:
         order = np.lexsort((xm[off_idx], hi))
         off_idx = off_idx[order]
         hi = hi[order]
-        zs, cs = [], []
-        diag_terms = 0
-        for g in diag_idx:
-            k = count[g]
-            nz = coef[g, :k] != 0
-            zs.append(zm[g, :k][nz])
-            cs.append(coef[g, :k][nz])
-            diag_terms += int(np.count_nonzero(nz))
-        # off-diagonal groups follow the X = 0 terms in the term arrays
-        base = diag_terms
-        offs = [base]
-        for g in off_idx:
-            k = count[g]
-            nz = coef[g, :k] != 0
-            zs.append(zm[g, :k][nz])
-            cs.append(coef[g, :k][nz])
-            base += int(np.count_nonzero(nz))
-            offs.append(base)
-        zmask = np.concatenate(zs) if zs else np.zeros(0, np.uint64)
-        coefs = np.concatenate(cs) if cs else np.zeros(0, np.complex128)
+        # the X = 0 terms first, then the off-diagonal groups in class order;
+        # within a group its nonzero terms in slot order
+        groups = np.concatenate([diag_idx, off_idx])
+        slots = np.arange(coef.shape[1])
+        valid = (slots[None, :] < count[groups][:, None]) & (coef[groups] != 0)
+        zmask = zm[groups][valid]
+        coefs = coef[groups][valid]
+        per_group = np.count_nonzero(valid, axis=1)
+        diag_terms = int(per_group[:diag_idx.size].sum())
+        offs = diag_terms + np.concatenate([[0], np.cumsum(per_group[diag_idx.size:])])
         if hi.size:
             starts = np.concatenate([[0], np.nonzero(hi[1:] != hi[:-1])[0] + 1])
             class_hi = hi[starts]

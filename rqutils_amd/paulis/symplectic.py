@@ -68,6 +68,50 @@ def _bits_to_u64(bits: np.ndarray) -> np.ndarray:
     return wide.view(">u8")[..., 0].astype(np.uint64)
 
 
+def _letter_codes(labels) -> np.ndarray:
+    """Upper-cased Pauli strings as an ``[n, L]`` array of character codes."""
+    width = len(labels[0])
+    if width == 0:
+        raise ValueError("empty Pauli strings")
+    text = np.array([p.upper() for p in labels], dtype=f"<U{width}")
+    return text.view(np.uint32).reshape(len(labels), width)
+
+
+def _unique_rows(rows: np.ndarray):
+    """``(first, inverse)`` of the distinct rows in ascending lexicographic
+    order (``np.unique(rows, axis=0)``'s order): the index of each distinct
+    row's first occurrence and, per row, the rank of its distinct row. Rows
+    are compared as big-endian byte strings (or as integers when they fit 64
+    bits), which orders them element by element."""
+    n, width = rows.shape
+    if rows.dtype == np.uint8 and width <= 64 and np.all(rows <= 1):
+        keys = _bits_to_u64(rows)
+    else:
+        big = np.ascontiguousarray(rows.astype(rows.dtype.newbyteorder(">")))
+        keys = big.view(np.dtype((np.void, big.itemsize * width))).ravel()
+    _, first, inverse = np.unique(keys, return_index=True, return_inverse=True)
+    return first, np.ravel(inverse)
+
+
+def _packed_to_u64(packed: np.ndarray, num_qubits: int, pad: bool) -> np.ndarray:
+    """Masks of rows packed by :func:`pack_rows` without unpacking them: the
+    packed bytes read as one big-endian integer hold the row's bits followed
+    by the zero fill of the last byte."""
+    packed = np.asarray(packed, dtype=np.uint8)
+    nbits = num_qubits + (1 if pad else 0)
+    nbytes = packed.shape[-1]
+    if num_qubits > 64:
+        raise ValueError(f"{num_qubits} qubits exceed the 64-bit key width")
+    if nbytes > 8:
+        return _bits_to_u64(unpack_rows(packed, num_qubits, pad))
+    wide = np.zeros(packed.shape[:-1] + (8,), dtype=np.uint8)
+    wide[..., 8 - nbytes:] = packed
+    value = wide.view(">u8")[..., 0].astype(np.uint64) >> np.uint64(8 * nbytes - nbits)
+    if num_qubits < 64:
+        value &= np.uint64((1 << num_qubits) - 1)
+    return value
+
+
 @dataclass(frozen=True)
 class PauliSumXZ:
     """Symplectic representation of a sum of Pauli strings.
@@ -103,15 +147,16 @@ class PauliSumXZ:
             if len(set(len(p) for p in labels)) != 1:
                 raise ValueError("Pauli strings have non-uniform lengths")
             coeffs = np.asarray(coeffs)
-            letters = np.array([list(p.upper()) for p in labels])
+            letters = _letter_codes(labels)
             keep = coeffs != 0
             letters, coeffs = letters[keep], coeffs[keep]
-            letters, where = np.unique(letters, axis=0, return_inverse=True)
+            first, where = _unique_rows(letters)
+            letters = letters[first]
             merged = np.zeros(letters.shape[0], dtype=np.result_type(coeffs.dtype, np.float64))
             np.add.at(merged, np.ravel(where), coeffs)
             coeffs = merged
-            xbits = (letters == "X") | (letters == "Y")
-            zbits = (letters == "Y") | (letters == "Z")
+            xbits = (letters == ord("X")) | (letters == ord("Y"))
+            zbits = (letters == ord("Y")) | (letters == ord("Z"))
             num_qubits = letters.shape[1]
         elif HAS_QISKIT and isinstance(paulisum, SparsePauliOp):  # pragma: no cover
             if not np.allclose(paulisum.coeffs.imag, 0.0):
@@ -130,20 +175,22 @@ class PauliSumXZ:
                 warnings.warn("Found nonzero imaginary part when force_real=True")
             coeffs = np.real(coeffs)
 
-        # Groups: distinct X signatures in ascending (lexicographic) order.
-        xsig, group_of, sizes = np.unique(xbits, axis=0, return_inverse=True,
-                                          return_counts=True)
-        group_of = np.ravel(group_of)
+        # Groups: distinct X signatures in ascending (lexicographic) order; a
+        # group's terms keep their order (lexicographic by letter).
+        xbits = np.asarray(xbits, dtype=np.uint8)
+        zbits = np.asarray(zbits, dtype=np.uint8)
+        first_x, group_of = _unique_rows(xbits)
+        xsig = xbits[first_x]
+        sizes = np.bincount(group_of, minlength=xsig.shape[0])
         n_groups, kmax = xsig.shape[0], int(np.max(sizes))
         zsig = np.zeros((n_groups, kmax, num_qubits), dtype=np.uint8)
         phc = np.zeros((n_groups, kmax), dtype=np.complex128)
         order = np.argsort(group_of, kind="stable")
         starts = np.concatenate([[0], np.cumsum(sizes)])
-        for g in range(n_groups):
-            members = order[starts[g]:starts[g + 1]]
-            zsig[g, : sizes[g]] = zbits[members]
-            phc[g, : sizes[g]] = coeffs[members] * _MINUS_I_POWERS[
-                phase_power(xsig[g][None, :], zbits[members])]
+        slot = np.empty(group_of.size, dtype=np.int64)
+        slot[order] = np.arange(group_of.size) - starts[group_of[order]]
+        zsig[group_of, slot] = zbits
+        phc[group_of, slot] = coeffs * _MINUS_I_POWERS[phase_power(xbits, zbits)]
         if np.all(phc.imag == 0.0):
             phc = phc.real
         return cls(pack_rows(xsig, add_padding), pack_rows(zsig, add_padding), phc,
@@ -162,10 +209,8 @@ class PauliSumXZ:
         """
         if padded is None:
             padded = self.padding
-        xb = unpack_rows(self.x, self.num_qubits, padded)
-        zb = unpack_rows(self.z, self.num_qubits, padded)
-        xm = _bits_to_u64(xb)
-        zm = _bits_to_u64(zb)
+        xm = _packed_to_u64(self.x, self.num_qubits, padded)
+        zm = _packed_to_u64(self.z, self.num_qubits, padded)
         coef = np.asarray(self.c, dtype=np.complex128)
         nz = coef != 0
         count = np.where(nz.any(axis=1),
