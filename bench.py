@@ -415,6 +415,9 @@ def build_problem(w, dev, tune=True, launch=None):
         log(f"[{w.name}] autotune -> cap={tuned.bucket_cap} dense_bits={tuned.dense_bits} "
             f"rows={tuned.rows_per_thread} split={tuned.group_split} compact={tuned.compact} "
             f"({time.time() - t:.1f} s)")
+    if tuned is None:
+        # first use of the kernels and the host code (the autotuner's runs do it otherwise)
+        SQDProblem(w.hamiltonian, keys, w.num_qubits, device=dev, **(launch or {}))
     sync()
     t0 = time.perf_counter()
     if launch:

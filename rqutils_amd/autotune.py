@@ -8,8 +8,10 @@ Three knobs decide the matvec's speed on a given problem:
   against idle lanes on small buckets;
 * the group split, which deals a chunk's groups to several workgroups so that
   small subspaces still fill the 256 CUs (partial sums are reduced after);
-* the lookup: a search in the staged keys, or, where its directory fits
-  (``dense_fits``), the dense bitmap lookup at a few bucket widths;
+* the lookup: a search in the staged keys (at a few sub-bucket sizes of its
+  directory: a coarser one stages fewer bytes per class, a finer one needs
+  fewer search steps), or, where its directory fits (``dense_fits``), the
+  dense bitmap lookup at a few bucket widths;
 * how hits are applied: by every lane for all its rows (register mode), or
   through a per-wave LDS queue, one lane per hit (compacted mode), which pays
   off once a wave has hits in most groups.
@@ -38,6 +40,7 @@ __all__ = ["TuneResult", "autotune", "signature", "DEFAULT_CAPS", "DEFAULT_ROWS"
 DEFAULT_CAPS = (512, 1024, 2048, 4096, 8192)
 DEFAULT_ROWS = (4, 8)
 DEFAULT_DENSE_BITS = (15, 16, 17)
+DEFAULT_FINE_SHIFTS = (-1, -2, -3)
 CU_COUNT = 256
 
 _MEMO: dict = {}
@@ -50,16 +53,18 @@ class TuneResult:
     group_split: int
     ms: float
     precision: str
-    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits, threads, compact)
+    table: list = field(default_factory=list)   # (cap, rows, split, ms, dense bits, threads, compact, fine shift)
     dense_bits: int = 0                         # bucket bits of the dense lookup, 0: search
     block_threads: int = 256
     compact: int = 0                            # hits through the per-wave LDS queue
+    fine_shift: int = -2                        # search directory (Subspace.fine_shift)
 
     def space_kw(self) -> dict:
         kw = {"rows_per_thread": self.rows_per_thread, "block_threads": self.block_threads}
         if self.dense_bits:
             return {**kw, "dense": True, "bucket_bits": self.dense_bits}
-        return {**kw, "bucket_cap": self.bucket_cap, "dense": False}
+        return {**kw, "bucket_cap": self.bucket_cap, "dense": False,
+                "fine_shift": self.fine_shift}
 
 
 def signature(hamiltonian: PauliSumXZ, keys: torch.Tensor, num_qubits: int, precision: str) -> str:
@@ -122,8 +127,8 @@ def _split_candidates(n_items: int):
 
 def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str = "c64",
              caps=DEFAULT_CAPS, rows=DEFAULT_ROWS, dense_bits=DEFAULT_DENSE_BITS,
-             threads=(256,), compact=(0, 1), reps: int = 3, use_cache: bool = True,
-             verbose: bool = False) -> TuneResult:
+             threads=(256,), compact=(0, 1), fine_shifts=DEFAULT_FINE_SHIFTS, reps: int = 3,
+             use_cache: bool = True, verbose: bool = False) -> TuneResult:
     """Fastest (lookup, bucket cap or width, rows per thread, group split) for
     this problem. ``threads=(256, 512)`` also times 512-thread workgroups
     (off by default: measured faster only for complex64 on the sparse 40-qubit
@@ -153,22 +158,23 @@ def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str 
     seen_m = {}
     # (cap, rows, dense bits, threads); 512-thread workgroups only at 4 rows
     shapes = [(r, 256) for r in rows] + ([(4, 512)] if 4 in rows and 512 in threads else [])
-    candidates = [(int(c), r, 0, t) for c in sorted(set(int(c) for c in caps)) for r, t in shapes]
+    candidates = [(int(c), r, 0, t, fs) for c in sorted(set(int(c) for c in caps))
+                  for r, t in shapes for fs in fine_shifts]
     if dense_fits(num_qubits, n):
         lo_m, hi_m = max(5, num_qubits - 26), min(num_qubits, 19)
-        candidates += [(0, r, int(m), t) for m in dense_bits if lo_m <= m <= hi_m
+        candidates += [(0, r, int(m), t, 0) for m in dense_bits if lo_m <= m <= hi_m
                        for r, t in shapes]
-    for cap, r, dm, nt in candidates:
+    for cap, r, dm, nt, fs in candidates:
         try:
             if dm:
                 sp = Subspace(kt, num_qubits, device=dev, dense=True, bucket_bits=dm,
                               rows_per_thread=r, block_threads=nt)
             else:
                 sp = Subspace(kt, num_qubits, device=dev, bucket_cap=cap, rows_per_thread=r,
-                              block_threads=nt)
+                              block_threads=nt, fine_shift=fs)
         except ValueError:
             continue            # keys too concentrated for this cap
-        key = (sp.bucket_bits, r, sp.dense, nt)
+        key = (sp.bucket_bits, sp.fine_bits, r, sp.dense, nt)
         if seen_m.get(key):
             continue            # same bucket level as a smaller cap: same kernel
         seen_m[key] = True
@@ -182,18 +188,18 @@ def autotune(hamiltonian, keys, num_qubits: int, *, device=None, precision: str 
                     ms = _time_matvec(op, v, reps)
                 except (ValueError, RuntimeError):
                     continue    # the compacted areas exceed the LDS for this shape
-                table.append((cap, r, split, round(ms, 4), dm, nt, cm))
+                table.append((cap, r, split, round(ms, 4), dm, nt, cm, fs))
                 if verbose:
                     print(f"[autotune] {'dense' if dm else 'cap=%d' % cap} m={sp.bucket_bits} "
-                          f"rows={r} threads={nt} split={split} compact={cm}: {ms:.3f} ms",
-                          flush=True)
+                          f"fine={sp.fine_bits} rows={r} threads={nt} split={split} "
+                          f"compact={cm}: {ms:.3f} ms", flush=True)
                 if best is None or ms < best[3]:
-                    best = (cap, r, split, ms, dm, nt, cm)
+                    best = (cap, r, split, ms, dm, nt, cm, fs)
         del op, sp
     if best is None:
         raise ValueError("no matvec configuration fits this subspace")
     res = TuneResult(best[0], best[1], best[2], best[3], precision, table, best[4], best[5],
-                     best[6])
+                     best[6], best[7])
     _MEMO[sig] = res
     _save_disk(sig, res)
     return res

@@ -27,6 +27,7 @@ __all__ = ["Subspace", "PauliOperator", "dense_fits", "states_to_keys", "keys_to
 
 DEFAULT_BUCKET_CAP = 8192       # keys per bucket staged in LDS (32 KiB as uint32)
 DEFAULT_ROWS_PER_THREAD = 4     # rows per thread; a chunk is 256 x this
+DEFAULT_FINE_SHIFT = -2         # sub-buckets of about 4 keys (log2 of mean bucket - 2 fine bits)
 MAX_DIR_BITS = 26               # at most 2^26 buckets
 DENSE_BUCKET_BITS = 16          # default bucket width of the dense lookup (16 KiB of LDS)
 
@@ -84,12 +85,16 @@ class Subspace:
 
     ``block_threads=512`` (with 4 rows per thread) makes 2,048-row chunks, so
     one staged target table serves twice the rows of a 256-thread chunk.
+    ``fine_shift`` sets the search's directory when ``fine_bits`` is not
+    given: ``round(log2(mean non-empty bucket)) + fine_shift`` fine bits
+    (default -2: about 4 keys per sub-bucket).
     """
 
     def __init__(self, keys, num_qubits: int, *, device=None, bucket_bits: int | None = None,
                  bucket_cap: int = DEFAULT_BUCKET_CAP,
                  rows_per_thread: int = DEFAULT_ROWS_PER_THREAD, fine_bits: int | None = None,
-                 dense: bool | None = False, block_threads: int = 256):
+                 dense: bool | None = False, block_threads: int = 256,
+                 fine_shift: int | None = None):
         L = _lib.lib()
         self.device = _device(device)
         if not 1 <= num_qubits <= 64:
@@ -105,6 +110,7 @@ class Subspace:
             raise ValueError(f"subspace size {n} outside 1..2^31-1")
         self.keys = kt
         self.n = int(n)
+        self.fine_shift = DEFAULT_FINE_SHIFT if fine_shift is None else int(fine_shift)
         stream = _lib.stream_ptr(self.device)
         with torch.cuda.device(self.device):
             self.pc_mask = torch.zeros(2, dtype=torch.int64, device=self.device)
@@ -192,11 +198,13 @@ class Subspace:
         if self.max_bucket >= 65536:
             raise ValueError(f"largest bucket holds {self.max_bucket} keys (limit 65535)")
         if fine_bits is None:
-            # Sized by the mean non-empty bucket: the directory slice staged per
-            # class then costs at most about as many bytes as the keys.
+            # Sized by the mean non-empty bucket: 2^-fine_shift of its keys per
+            # sub-bucket on average (4 by default). A coarser directory stages
+            # fewer bytes per class; the search then takes a few more LDS steps.
             sizes = self.bstart[1:] - self.bstart[:-1]
             mean = float(self.n) / max(1, int(torch.count_nonzero(sizes).item()))
-            fine_bits = min(self.bucket_bits, 13, max(0, int(round(np.log2(max(mean, 1.0))))))
+            fine_bits = min(self.bucket_bits, 13,
+                            max(0, int(round(np.log2(max(mean, 1.0)))) + self.fine_shift))
         self.fine_bits = int(fine_bits)
         self.fdir = torch.empty(self.n_buckets << self.fine_bits, dtype=torch.int16,
                                 device=self.device)
