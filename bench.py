@@ -97,6 +97,7 @@ def hit_table_block(prob, v, out_join, o128_join, hits, steps, dev):
     res = {}
     for prec, vec, ref in ((_lib.C64, v, out_join), (_lib.C128, v.to(torch.complex128), o128_join)):
         name = "c64" if prec == _lib.C64 else "c128"
+        op.drop_cache()             # each precision's build timed from scratch
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         op.build_cache(prec)
@@ -414,6 +415,7 @@ def build_problem(w, dev, tune=True, launch=None):
         tuned = SQDProblem(w.hamiltonian, keys, w.num_qubits, device=dev, tune="auto").tuned
         log(f"[{w.name}] autotune -> cap={tuned.bucket_cap} dense_bits={tuned.dense_bits} "
             f"rows={tuned.rows_per_thread} split={tuned.group_split} compact={tuned.compact} "
+            f"fine_shift={tuned.fine_shift} "
             f"({time.time() - t:.1f} s)")
     if tuned is None:
         # first use of the kernels and the host code (the autotuner's runs do it otherwise)
@@ -492,7 +494,7 @@ def matvec_block(prob, w, args, dev):
                       "rows_per_thread": sp.rows_per_thread, "group_split": op.tune.group_split,
                       "lookup": "dense bitmap" if sp.dense else "search",
                       "compact_hits": bool(op.tune.compact)},
-           "kernel": f"bucket_join_kernel<float2,{'uint32_t' if sp.bucket_bits <= 32 else 'uint64_t'},"
+           "kernel": f"bucket_join_kernel<float2,{'uint32_t' if sp.bucket_bits < 32 else 'uint64_t'},"
                      f"{sp.rows_per_thread},false,{str(sp.dense).lower()},"
                      f"{str(bool(op.tune.compact)).lower()}>"}
     return res, (v, out, o128, hits)
@@ -558,12 +560,18 @@ def time_to_solution(prob, w, dev, setup_s, runs=None, tol=1e-6):
     op = prob.op
     op.drop_cache()
     build = {}
-    for prec in sorted({TTS_RUNS[k][1] for k in runs if TTS_RUNS[k][3]}):
+    precs = sorted({TTS_RUNS[k][1] for k in runs if TTS_RUNS[k][3]})
+    for prec in precs:
+        # each precision's table timed on its own (a second precision reuses
+        # the first one's layout), then all of them built for the runs
+        op.drop_cache()
         sync()
         t0 = time.perf_counter()
         op.build_cache(_lib.C128 if prec == "c128" else _lib.C64)
         sync()
         build[prec] = time.perf_counter() - t0
+    for prec in precs:
+        op.build_cache(_lib.C128 if prec == "c128" else _lib.C64)
     res = {"setup_s": round(setup_s, 4), "table_build_s": {k: round(v, 4) for k, v in build.items()},
            "start": "seeded complex Gaussian (seed 77)", "tol": tol}
     for name in runs:
@@ -1046,6 +1054,8 @@ def main():
     ap.add_argument("--rows-per-thread", type=int, default=4, help="with a fixed launch")
     ap.add_argument("--group-split", type=int, default=1, help="with a fixed launch")
     ap.add_argument("--compact", type=int, default=0, help="with a fixed launch (0/1)")
+    ap.add_argument("--fine-shift", type=int, default=-2,
+                    help="with a fixed search launch: fine bits = log2(mean bucket) + this")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -1092,7 +1102,8 @@ def main():
         launch = {"rows_per_thread": args.rows_per_thread, "group_split": args.group_split,
                   "compact": args.compact}
         launch.update({"dense": True, "bucket_bits": args.dense_bits} if args.dense_bits
-                      else {"dense": False, "bucket_cap": args.bucket_cap})
+                      else {"dense": False, "bucket_cap": args.bucket_cap,
+                            "fine_shift": args.fine_shift})
     prob, setup_s = build_problem(w, dev, launch=launch)
     mv, (v, out, o128, hits) = matvec_block(prob, w, args, dev)
     record = {

@@ -1019,10 +1019,19 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       for (int e = tid; e < ndir; e += kBlock) ldir[e] = make_uint2(fbits_i[fbase + e], fdir_i[fbase + e]);
       __syncthreads();
       if (!wave_active) continue;
+      // the next group's X mask and term range are loaded one group ahead
+      uint64_t xm_n = xmask[g0];
+      int gt0_n = grp_off[g0], gt1_n = grp_off[g0 + 1];
       for (int g = g0; g < g1; ++g) {
-        const int gt0 = grp_off[g], gt1 = grp_off[g + 1];
+        const uint64_t xm = xm_n;
+        const int gt0 = gt0_n, gt1 = gt1_n;
+        if (g + 1 < g1) {
+          xm_n = xmask[g + 1];
+          gt0_n = gt1;
+          gt1_n = grp_off[g + 2];
+        }
         if (gt1 <= gt0) continue;
-        const uint32_t xl = (uint32_t)(xmask[g] & lowmask);
+        const uint32_t xl = (uint32_t)(xm & lowmask);
         int idx[ROWS];
         bool any_hit = false;
 #pragma unroll
@@ -1041,10 +1050,18 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
                                                  kaddr, lkeys, lpair, s_occ, tid);
       const uint32_t last = kaddr + (uint32_t)(qn - 1) * (uint32_t)sizeof(KT);
       if (!wave_active) continue;
+      uint64_t xm_n = xmask[g0];
+      int gt0_n = grp_off[g0], gt1_n = grp_off[g0 + 1];
       for (int g = g0; g < g1; ++g) {
-        const int gt0 = grp_off[g], gt1 = grp_off[g + 1];
+        const uint64_t xm = xm_n;
+        const int gt0 = gt0_n, gt1 = gt1_n;
+        if (g + 1 < g1) {
+          xm_n = xmask[g + 1];
+          gt0_n = gt1;
+          gt1_n = grp_off[g + 2];
+        }
         if (gt1 <= gt0) continue;
-        const KT xl = (KT)(xmask[g] & lowmask);
+        const KT xl = (KT)(xm & lowmask);
         KT tl[ROWS];
         uint32_t ba[ROWS];
 #pragma unroll

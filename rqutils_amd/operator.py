@@ -424,14 +424,21 @@ class PauliOperator:
         ns = C.c_int64(0)
         _lib.check(L.sqdk_cache_slices(C.byref(s), C.byref(ns)), "sqdk_cache_slices")
         ns = ns.value
-        slot_row = torch.empty(64 * ns, dtype=torch.int32, device=dev)
-        sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
-        rlen = torch.empty(self.space.n, dtype=torch.int32, device=dev)
         stream = _lib.stream_ptr(dev)
-        _lib.check(L.sqdk_cache_count(C.byref(h), C.byref(s), slot_row.data_ptr(),
-                                      sbase.data_ptr(), rlen.data_ptr(), stream),
-                   "sqdk_cache_count")
-        entries = int(sbase[-1].item())
+        if self._caches:
+            # the layout (slot map, slice bases, row lengths) does not depend on
+            # the precision: the other precision's count is reused
+            other = next(iter(self._caches.values()))
+            slot_row, sbase, rlen = other[1], other[2], other[6]
+            entries = next(iter(self.cache_info.values()))["entries"]
+        else:
+            slot_row = torch.empty(64 * ns, dtype=torch.int32, device=dev)
+            sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
+            rlen = torch.empty(self.space.n, dtype=torch.int32, device=dev)
+            _lib.check(L.sqdk_cache_count(C.byref(h), C.byref(s), slot_row.data_ptr(),
+                                          sbase.data_ptr(), rlen.data_ptr(), stream),
+                       "sqdk_cache_count")
+            entries = int(sbase[-1].item())
         cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
         esz = 16 if precision == _lib.C128 else 8
         nbytes = entries * (4 + esz) + self.space.n * (esz + 4) + ns * (8 + 64 * 4)
