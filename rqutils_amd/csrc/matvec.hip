@@ -474,13 +474,16 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
   const bool wave_active = __any(valid[0]);
   // Dense lookup: byte offset of each row's 32-value sub-bucket entry; the
   // target's entry is this XOR the group's (x >> 5) << 3.
+  // Search lookup: byte offset of each row's sub-bucket entry (the target's
+  // is this XOR the group's (x >> f) << 2, since (s ^ x) >> f = (s >> f) ^ (x >> f)).
   uint32_t sa[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    sa[r] = ((uint32_t)sl[r] >> 5) << 3;
+    sa[r] = DENSE ? ((uint32_t)sl[r] >> 5) << 3 : (uint32_t)(sl[r] >> f) << 2;
     // keep the precomputed offset: one XOR per lookup instead of a shift and mask
-    if constexpr (DENSE) __asm__ __volatile__("" : "+v"(sa[r]));
+    __asm__ __volatile__("" : "+v"(sa[r]));
   }
+  const uint32_t paddr = lds_addr(lpair);  // LDS address of the sub-bucket table
 
   // X = 0 group: the diagonal (first share only), one pass over its terms
   // for all rows of the thread.
@@ -604,12 +607,13 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
         }
         if (gt1 <= gt0) continue;  // a group whose terms all had zero coefficients
         const KT xl = (KT)(xm & lowmask);
+        const uint32_t xa = (uint32_t)(xl >> f) << 2;
         KT tl[ROWS];
         uint32_t ba[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          ba[r] = lpair[(uint32_t)(tl[r] >> f)];
+          ba[r] = lds_load<uint32_t>(paddr + (sa[r] ^ xa));
         }
         bool hit[ROWS];
         lift_search<KT, ROWS>(ba, tl, steps, last, hit);
@@ -938,6 +942,8 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   const uint64_t bucket_base = m >= 64 ? 0ull : (bucket << m);
   const bool narrow = ((bucket_base | lowmask) >> 32) == 0;
   const uint32_t row1 = min(bstart_i[bucket + 1], row0 + (uint32_t)(ROWS * kBlock));
+  uint32_t sa[ROWS];
+  const uint32_t paddr = lds_addr(lpair);  // LDS address of the sub-bucket table
   KT sl[ROWS];
   bool valid[ROWS];
   uint32_t cnt[ROWS];
@@ -958,6 +964,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
     const uint32_t i = row0 + wslot + r * kWave + lane;
     valid[r] = i < row1;
     sl[r] = valid[r] ? (KT)(keys[i] & lowmask) : (KT)0;
+    sa[r] = DENSE ? 0u : (uint32_t)(sl[r] >> f) << 2;  // the row's sub-bucket entry offset
     cnt[r] = 0;
     ebase[r] = 0;
     if constexpr (MODE == 1) {
@@ -1062,12 +1069,13 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
         }
         if (gt1 <= gt0) continue;
         const KT xl = (KT)(xm & lowmask);
+        const uint32_t xa = (uint32_t)(xl >> f) << 2;
         KT tl[ROWS];
         uint32_t ba[ROWS];
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          ba[r] = lpair[(uint32_t)(tl[r] >> f)];
+          ba[r] = lds_load<uint32_t>(paddr + (sa[r] ^ xa));
         }
         bool hit[ROWS];
         lift_search<KT, ROWS>(ba, tl, steps, last, hit);
