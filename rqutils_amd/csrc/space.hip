@@ -14,6 +14,13 @@ std::string& last_error_slot() {
 namespace {
 
 constexpr int kBlock = 256;
+// Directory entries a fine directory may hold (its kernels loop over them
+// with a grid of at most 2^20 workgroups).
+constexpr int64_t kMaxFineEntries = 1ll << 31;
+
+inline unsigned grid_stride_blocks(int64_t total) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(ceil_div(total, kBlock), 1ll << 20));
+}
 
 // Strict ascent, key width and popcount presence. One atomicOr per block.
 __global__ void __launch_bounds__(kBlock)
@@ -71,15 +78,16 @@ dir_kernel(const uint64_t* __restrict__ keys, int64_t n, int shift, int64_t np,
 __global__ void __launch_bounds__(kBlock)
 fine_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
             int m, int d, int64_t total, uint16_t* __restrict__ fdir) {
-  const int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  if (idx >= total) return;
-  const int64_t b = idx >> d;
-  const uint64_t e = (uint64_t)(idx & ((1ll << d) - 1));
-  const int64_t lo = bstart[b];
-  const int64_t hi = bstart[b + 1];
-  const uint64_t t = ((uint64_t)b << m) | (e << (m - d));
-  const int64_t j = lower_bound_range<uint64_t>(keys, lo, hi, t);
-  fdir[idx] = (uint16_t)(j - lo);
+  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    const int64_t b = idx >> d;
+    const uint64_t e = (uint64_t)(idx & ((1ll << d) - 1));
+    const int64_t lo = bstart[b];
+    const int64_t hi = bstart[b + 1];
+    const uint64_t t = ((uint64_t)b << m) | (e << (m - d));
+    const int64_t j = lower_bound_range<uint64_t>(keys, lo, hi, t);
+    fdir[idx] = (uint16_t)(j - lo);
+  }
 }
 
 // fbits[e] = OR of 1 << (key & 31) over the keys of 32-value sub-bucket e
@@ -87,16 +95,17 @@ fine_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bsta
 __global__ void __launch_bounds__(kBlock)
 bits_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
             const uint16_t* __restrict__ fdir, int d, int64_t total, uint32_t* __restrict__ fbits) {
-  const int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-  if (idx >= total) return;
-  const int64_t b = idx >> d;
-  const int64_t e = idx & ((1ll << d) - 1);
-  const int64_t base = bstart[b];
-  const int64_t lo = base + fdir[idx];
-  const int64_t hi = (e + 1 < (1ll << d)) ? base + fdir[idx + 1] : (int64_t)bstart[b + 1];
-  uint32_t bits = 0;
-  for (int64_t j = lo; j < hi; ++j) bits |= 1u << (uint32_t)(keys[j] & 31u);
-  fbits[idx] = bits;
+  for (int64_t idx = blockIdx.x * (int64_t)kBlock + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * kBlock) {
+    const int64_t b = idx >> d;
+    const int64_t e = idx & ((1ll << d) - 1);
+    const int64_t base = bstart[b];
+    const int64_t lo = base + fdir[idx];
+    const int64_t hi = (e + 1 < (1ll << d)) ? base + fdir[idx + 1] : (int64_t)bstart[b + 1];
+    uint32_t bits = 0;
+    for (int64_t j = lo; j < hi; ++j) bits |= 1u << (uint32_t)(keys[j] & 31u);
+    fbits[idx] = bits;
+  }
 }
 
 // Row chunks. Single block: serial scan over buckets in block-sized strides.
@@ -220,7 +229,9 @@ int sqdk_space_fine(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t
   SQDK_REQUIRE(fine_bits >= 0 && fine_bits <= bucket_bits && fine_bits <= 15,
                "fine_bits %d invalid", fine_bits);
   const int64_t total = (1ll << (num_qubits - bucket_bits)) << fine_bits;
-  fine_kernel<<<(unsigned)ceil_div(total, kBlock), kBlock, 0, as_stream(stream)>>>(
+  SQDK_REQUIRE(total <= kMaxFineEntries, "fine directory of %lld entries (limit 2^31)",
+               (long long)total);
+  fine_kernel<<<grid_stride_blocks(total), kBlock, 0, as_stream(stream)>>>(
       keys, bstart, bucket_bits, fine_bits, total, fdir);
   SQDK_LAUNCHED("fine_kernel");
   return SQDK_OK;
@@ -237,7 +248,9 @@ int sqdk_space_bits(const uint64_t* keys, int64_t n, int32_t num_qubits, int32_t
   SQDK_REQUIRE(fine_bits == bucket_bits - 5, "dense lookup needs fine_bits = bucket_bits - 5 "
                "(got %d, %d)", fine_bits, bucket_bits);
   const int64_t total = (1ll << (num_qubits - bucket_bits)) << fine_bits;
-  bits_kernel<<<(unsigned)ceil_div(total, kBlock), kBlock, 0, as_stream(stream)>>>(
+  SQDK_REQUIRE(total <= kMaxFineEntries, "fine directory of %lld entries (limit 2^31)",
+               (long long)total);
+  bits_kernel<<<grid_stride_blocks(total), kBlock, 0, as_stream(stream)>>>(
       keys, bstart, fdir, fine_bits, total, fbits);
   SQDK_LAUNCHED("bits_kernel");
   return SQDK_OK;

@@ -205,6 +205,11 @@ class Subspace:
             mean = float(self.n) / max(1, int(torch.count_nonzero(sizes).item()))
             fine_bits = min(self.bucket_bits, 13,
                             max(0, int(round(np.log2(max(mean, 1.0)))) + self.fine_shift))
+            # and at most max(2^22, 8 n) directory entries in all (few keys in
+            # many buckets would otherwise make it huge)
+            budget = max(1 << 22, 8 * self.n)
+            while fine_bits > 0 and (self.n_buckets << fine_bits) > budget:
+                fine_bits -= 1
         self.fine_bits = int(fine_bits)
         self.fdir = torch.empty(self.n_buckets << self.fine_bits, dtype=torch.int16,
                                 device=self.device)

@@ -108,6 +108,40 @@ def test_matvec_wide_keys_and_fine_bits(gpu):
         assert _rel(got, ref) < 1e-12, fine
 
 
+@pytest.mark.parametrize("bucket_bits", [14, 30, 31, 32, 33])
+def test_matvec_search_padding_and_clamps(gpu, bucket_bits):
+    """The search's padded key slice: 32-bit LDS keys below 32 bucket bits,
+    64-bit from 32 on (a target may then equal the 32-bit padding value);
+    directories from one sub-bucket per bucket (every lifting stride of 64
+    keys or more clamped) to the default, all giving the same products."""
+    from rqutils_amd.operator import PauliOperator, Subspace
+    rng = np.random.default_rng(bucket_bits)
+    n = 40
+    labels = random_labels(rng, n, 150, 1, 3)
+    coeffs = rng.normal(size=150)
+    if bucket_bits == 14:
+        base = rng.choice(2 ** 16, 3000, replace=False).astype(np.uint64)
+    else:
+        base = rng.integers(0, 2 ** n, size=1500, dtype=np.uint64)
+    x1 = np.uint64(int("".join(rng.choice(list("01"), n)), 2))
+    keys = np.unique(np.concatenate([base, base ^ np.uint64(0b101), base ^ x1]))
+    keys = keys[keys < np.uint64(2 ** n)]
+    p = OracleProblem(labels + ["".join("X" if (int(x1) >> (n - 1 - q)) & 1 else "I"
+                                        for q in range(n))],
+                      np.concatenate([coeffs, [0.7]]), keys, n)
+    v = _rand_vec(rng, keys.size)
+    ref = p.matvec(v)
+    outs = []
+    for fine in (0, 1, 3, None):
+        sp = Subspace(keys, n, bucket_bits=bucket_bits, fine_bits=fine)
+        op = PauliOperator((p.labels, p.coeffs), sp)
+        got = op.matvec(torch.from_numpy(v).cuda()).cpu().numpy()
+        assert _rel(got, ref) < 1e-12, (fine, sp.max_bucket)
+        outs.append(got)
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+
+
 def test_diagonal_group0(gpu):
     p = small_problem(4, n=10, terms=60, nstates=400)
     prob = _problem_on_gpu(p)
