@@ -897,8 +897,14 @@ constexpr int kHitSortLog2 = 8;
 // each slot into slot_row and each slice's entry count (64 x width) into
 // sbase[sid]. MODE 1 (sbase now holds each slice's first entry): the entries
 // at the slots slot_row gives, the padding and the diagonal.
+// Occupancy of the table's fill pass at 4 rows per thread: 6 waves per SIMD
+// for complex64 with 32-bit LDS keys (at most 80 VGPRs).
+template <typename VT, typename KT, int ROWS, int MODE>
+constexpr int kTableWaves = (ROWS == 4 && MODE == 1 && sizeof(VT) == 8 && sizeof(KT) == 4) ? 6 : 1;
+
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
 __global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(kTableWaves<VT, KT, ROWS, MODE>)))
 hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ bstart,
                  const uint16_t* __restrict__ fdir, const uint32_t* __restrict__ fbits,
                  const uint32_t* __restrict__ items, const uint64_t* __restrict__ xmask,
