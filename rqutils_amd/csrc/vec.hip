@@ -132,7 +132,8 @@ __device__ inline double precond_scale(double d, double theta, double2 st) {
 }
 
 // W (the search direction in the basis) may be R itself (no preconditioner,
-// diag == NULL) or a separate vector that receives diag-preconditioned R'.
+// diag == NULL) or a separate vector that receives diag-preconditioned R'
+// (R is then not stored).
 // With gpart, the pass also sums, per block, the next iteration's Gram sums
 // over the values it stores: every sum of gram_partial_kernel except R.AR
 // (the next product's, summed by gram_rar_kernel), with X.AR taken as
@@ -167,8 +168,12 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
     Vec<VT>::store(AX, i, axn);
     Vec<VT>::store(P, i, pn);
     Vec<VT>::store(AP, i, apn);
-    Vec<VT>::store(R, i, rn);
-    if (diag) Vec<VT>::store(W, i, wn);
+    // With the preconditioner the basis holds W; R itself is read again only
+    // after being rebuilt (refresh, verification), so only |R|^2 is kept.
+    if (diag)
+      Vec<VT>::store(W, i, wn);
+    else
+      Vec<VT>::store(R, i, rn);
     if (gpart) {
       const double2 xs = Vec<VT>::round(xn), ws = Vec<VT>::round(wn), ps = Vec<VT>::round(pn);
       const double2 axs = Vec<VT>::round(axn), aps = Vec<VT>::round(apn);
