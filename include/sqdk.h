@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define SQDK_ABI_VERSION 19
+#define SQDK_ABI_VERSION 20
 
 enum sqdk_status {
   SQDK_OK = 0,
@@ -323,27 +323,45 @@ typedef struct sqdk_cache {
 /* Slices of the subspace: n_items x chunk_rows / 64 (chunks of 1024, 2048 or
  * 4096 rows). */
 int sqdk_cache_slices(const sqdk_space* space, int64_t* n_slices);
-/* slot_row [64 n_slices], slice_base [n_slices + 1] and row_len [n] (device). */
+/* Bytes of the optional build workspace (device): n_items x (ceil(n_classes / 8)
+ * + 1) words of class masks, then 2 x 4 bytes per row. The count records in it,
+ * per chunk and class, which of the chunk's four waves had a hit, which waves
+ * hold a row of more than two hits, and every row's first two source rows; the
+ * fill then takes the entries of rows from waves without such a row from that
+ * record and joins only the (wave, class) pairs with hits of the others. */
+int sqdk_cache_workspace(const sqdk_ham* ham, const sqdk_space* space, size_t* bytes);
+/* slot_row [64 n_slices], slice_base [n_slices + 1] and row_len [n] (device);
+ * work: the build workspace (written here, read by sqdk_cache_fill) or NULL. */
 int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space, uint32_t* slot_row,
-                     uint64_t* slice_base, uint32_t* row_len, void* stream);
+                     uint64_t* slice_base, uint32_t* row_len, void* work, size_t work_bytes,
+                     void* stream);
 /* col / coef [slice_base[n_slices]], diag [n] (device; coef and diag complex of
- * precision) at the slots of sqdk_cache_count; row_len [n] is rewritten. */
+ * precision) at the slots of sqdk_cache_count; row_len [n] is rewritten.
+ * work: the workspace the same table's sqdk_cache_count wrote, or NULL (the
+ * fill then joins every class); the entries are the same either way. */
 int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                     const uint64_t* slice_base, const uint32_t* slot_row, uint32_t* col,
-                    void* coef, void* diag, uint32_t* row_len, void* stream);
+                    void* coef, void* diag, uint32_t* row_len, const void* work,
+                    size_t work_bytes, void* stream);
 /* out = H v through the table (v, out: n complex of the cache's precision);
  * sums in FP64, each row's in group order; a row reads its own hits only. */
 int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* out, void* stream);
 
 /* The same table for a whole batch (rows and slices global, one table): slices
  * = n_items x chunk_rows / 64; count and fill as above. The product is
- * sqdk_cache_matvec; sqdk_batch.cache hands it to the batched solve. */
+ * sqdk_cache_matvec; sqdk_batch.cache hands it to the batched solve. The build
+ * workspace (or NULL) is sized for max_classes (the largest n_classes of the
+ * instances), the same in workspace, count and fill; classes of an instance past
+ * max_classes are never skipped. */
 int sqdk_batch_cache_slices(const sqdk_batch* batch, int64_t* n_slices);
+int sqdk_batch_cache_workspace(const sqdk_batch* batch, int32_t max_classes, size_t* bytes);
 int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slot_row, uint64_t* slice_base,
-                           uint32_t* row_len, void* stream);
+                           uint32_t* row_len, int32_t max_classes, void* work,
+                           size_t work_bytes, void* stream);
 int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision, const uint64_t* slice_base,
                           const uint32_t* slot_row, uint32_t* col, void* coef, void* diag,
-                          uint32_t* row_len, void* stream);
+                          uint32_t* row_len, int32_t max_classes, const void* work,
+                          size_t work_bytes, void* stream);
 
 /* ---- operator functions on the reference's own arrays ------------------- *
  * For callers that hold the reference's packed arrays. A packed padded row of

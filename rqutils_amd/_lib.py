@@ -13,7 +13,7 @@ from pathlib import Path
 
 import torch  # noqa: F401  (binds the process to torch's HIP runtime first)
 
-ABI_VERSION = 19
+ABI_VERSION = 20
 
 C128, C64 = 0, 1
 LOCG_REFERENCE, LOCG_FUSED = 0, 1
@@ -169,18 +169,25 @@ SIGNATURES = {
     "sqdk_locg_workspace": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.POINTER(SqdkLocgOpts),
                                       C.POINTER(C.c_size_t)]),
     "sqdk_cache_slices": (C.c_int, [C.POINTER(SqdkSpace), C.POINTER(C.c_int64)]),
+    "sqdk_cache_workspace": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace),
+                                       C.POINTER(C.c_size_t)]),
     "sqdk_cache_count": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace), C.c_void_p,
-                                   C.c_void_p, C.c_void_p, C.c_void_p]),
+                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t,
+                                   C.c_void_p]),
     "sqdk_cache_fill": (C.c_int, [C.POINTER(SqdkHam), C.POINTER(SqdkSpace), C.c_int32,
                                   C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                  C.c_void_p, C.c_void_p]),
+                                  C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "sqdk_cache_matvec": (C.c_int, [C.POINTER(SqdkCache), C.c_void_p, C.c_void_p, C.c_void_p]),
     "sqdk_batch_cache_slices": (C.c_int, [C.POINTER(SqdkBatch), C.POINTER(C.c_int64)]),
+    "sqdk_batch_cache_workspace": (C.c_int, [C.POINTER(SqdkBatch), C.c_int32,
+                                             C.POINTER(C.c_size_t)]),
     "sqdk_batch_cache_count": (C.c_int, [C.POINTER(SqdkBatch), C.c_void_p, C.c_void_p,
-                                         C.c_void_p, C.c_void_p]),
+                                         C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t,
+                                         C.c_void_p]),
     "sqdk_batch_cache_fill": (C.c_int, [C.POINTER(SqdkBatch), C.c_int32, C.c_void_p,
                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                        C.c_void_p, C.c_void_p]),
+                                        C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t,
+                                        C.c_void_p]),
     "sqdk_locg_solve": (C.c_int, [C.POINTER(SqdkOp), C.c_int64, C.c_void_p,
                                   C.POINTER(SqdkLocgOpts), C.c_void_p, C.c_size_t,
                                   C.POINTER(SqdkLocgResult), C.c_void_p]),

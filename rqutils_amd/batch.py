@@ -222,9 +222,16 @@ class SQDBatch:
         slot_row = torch.empty(64 * ns, dtype=torch.int32, device=dev)
         sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
         rlen = torch.empty(self.n, dtype=torch.int32, device=dev)
+        # the count's record for the fill (class masks, rows' first hits)
+        max_classes = max((op.n_classes for op in self.ops), default=0)
+        nw = C.c_size_t(0)
+        _lib.check(L.sqdk_batch_cache_workspace(C.byref(self.cstruct), max_classes,
+                                                C.byref(nw)), "sqdk_batch_cache_workspace")
+        cmask = torch.empty(max(nw.value, 1), dtype=torch.uint8, device=dev)
         stream = _lib.stream_ptr(dev)
         _lib.check(L.sqdk_batch_cache_count(C.byref(self.cstruct), slot_row.data_ptr(),
-                                            sbase.data_ptr(), rlen.data_ptr(), stream),
+                                            sbase.data_ptr(), rlen.data_ptr(), max_classes,
+                                            cmask.data_ptr(), cmask.numel(), stream),
                    "sqdk_batch_cache_count")
         entries = int(sbase[-1].item())
         cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
@@ -233,12 +240,13 @@ class SQDBatch:
         diag = torch.empty(self.n, dtype=cdt, device=dev)
         _lib.check(L.sqdk_batch_cache_fill(C.byref(self.cstruct), precision, sbase.data_ptr(),
                                            slot_row.data_ptr(), col.data_ptr(), coef.data_ptr(),
-                                           diag.data_ptr(), rlen.data_ptr(), stream),
+                                           diag.data_ptr(), rlen.data_ptr(), max_classes,
+                                           cmask.data_ptr(), cmask.numel(), stream),
                    "sqdk_batch_cache_fill")
         spw = 4 if entries <= 8 * self.n else 1
         cs = _lib.SqdkCache(self.n, ns, slot_row.data_ptr(), sbase.data_ptr(), col.data_ptr(),
                             coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(), precision, spw)
-        self._caches[precision] = (cs, slot_row, sbase, col, coef, diag, rlen)
+        self._caches[precision] = (cs, slot_row, sbase, col, coef, diag, rlen, cmask)
         esz = 16 if precision == _lib.C128 else 8
         self.cache_info[precision] = {"slices": ns, "entries": entries, "slices_per_wave": spw,
                                       "bytes": entries * (4 + esz) + self.n * (esz + 4)

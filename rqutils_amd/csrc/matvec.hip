@@ -179,6 +179,29 @@ __device__ inline double2 gather0(__amdgpu_buffer_rsrc_t r, int idx, double2) {
                       __hiloint2double((int)x[3], (int)x[2]));
 }
 
+// The fields of an instance descriptor in scalar registers (the descriptor is
+// the same for the whole workgroup; without this the compiler may keep it in
+// vector registers and turn every loop over its ranges into a divergent one).
+__device__ inline int64_t uniform_i64(int64_t x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+__device__ inline sqdk_inst uniform_inst(const sqdk_inst& d) {
+  sqdk_inst u;
+  u.bstart_base = uniform_i64(d.bstart_base);
+  u.fdir_base = uniform_i64(d.fdir_base);
+  u.class_base = __builtin_amdgcn_readfirstlane(d.class_base);
+  u.n_classes = __builtin_amdgcn_readfirstlane(d.n_classes);
+  u.diag_t0 = __builtin_amdgcn_readfirstlane(d.diag_t0);
+  u.diag_t1 = __builtin_amdgcn_readfirstlane(d.diag_t1);
+  u.bucket_bits = __builtin_amdgcn_readfirstlane(d.bucket_bits);
+  u.fine_bits = __builtin_amdgcn_readfirstlane(d.fine_bits);
+  u.fbits_base = uniform_i64(d.fbits_base);
+  return u;
+}
+
 // sum_k c_k (-1)^{popcount(z_k & s_r)} for every row r of a thread, terms in
 // index order (the order of group_coef, so results are identical: adding
 // (+-1) c with one FMA rounds exactly like adding +-c). The term tables are
@@ -361,20 +384,74 @@ __device__ inline void lift_search(uint32_t (&ba)[ROWS], const KT (&tl)[ROWS], i
 // padding, and per sub-bucket the LDS byte address of its first key; returns
 // the lifting steps, ceil(log2(largest sub-bucket)). Every thread of the
 // workgroup calls it (two barriers).
+// Dense lookup: stages the target bucket's (occupancy bitmap, first-key
+// offset) pair of every 32-value sub-bucket, kStageDense pairs per thread per
+// round. No barriers: the caller places them.
+constexpr int kStageDense = 4;
+
+template <int NT>
+__device__ inline void stage_dense(const uint32_t* __restrict__ fb,
+                                   const uint16_t* __restrict__ fdr, uint64_t fbase, int ndir,
+                                   uint2* ldir, int tid) {
+  for (int e0 = tid; e0 < ndir; e0 += kStageDense * NT) {
+    uint32_t bits[kStageDense], off[kStageDense];
+#pragma unroll
+    for (int u = 0; u < kStageDense; ++u) {
+      const int e = e0 + u * NT;
+      bits[u] = e < ndir ? fb[fbase + e] : 0u;
+      off[u] = e < ndir ? (uint32_t)fdr[fbase + e] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kStageDense; ++u) {
+      const int e = e0 + u * NT;
+      if (e < ndir) ldir[e] = make_uint2(bits[u], off[u]);
+    }
+  }
+}
+
+// The staging loops keep several loads per thread in flight: all of a round's
+// loads are issued before the first LDS store, so a round costs one memory
+// latency instead of one per element.
+constexpr int kStageKeys = 8;  // keys per thread per round
+constexpr int kStageDir = 2;   // directory entries per thread per round
+
 template <typename KT, int NT>
 __device__ inline int stage_search(const uint64_t* __restrict__ keys, uint64_t qs, int qn,
                                    uint64_t lowmask, const uint16_t* __restrict__ fdr,
                                    uint64_t fbase, int ndir, uint32_t kaddr, KT* lkeys,
                                    uint32_t* lpair, int* s_occ, int tid) {
   __syncthreads();  // the previous class is done with the LDS tables
-  for (int i = tid; i < qn; i += NT) lkeys[i] = (KT)(keys[qs + i] & lowmask);
+  for (int i0 = tid; i0 < qn; i0 += kStageKeys * NT) {
+    KT kv[kStageKeys];
+#pragma unroll
+    for (int u = 0; u < kStageKeys; ++u) {
+      const int i = i0 + u * NT;
+      kv[u] = i < qn ? (KT)keys[qs + i] : (KT)0;
+    }
+#pragma unroll
+    for (int u = 0; u < kStageKeys; ++u) {
+      const int i = i0 + u * NT;
+      if (i < qn) lkeys[i] = (KT)(kv[u] & (KT)lowmask);
+    }
+  }
   for (int i = tid; i < kKeyPad; i += NT) lkeys[qn + i] = (KT)~(KT)0;
   int occ = 0;
-  for (int e = tid; e < ndir; e += NT) {
-    const uint32_t lo = fdr[fbase + e];
-    const uint32_t hi = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
-    lpair[e] = kaddr + lo * (uint32_t)sizeof(KT);
-    occ = max(occ, (int)(hi - lo));
+  for (int e0 = tid; e0 < ndir; e0 += kStageDir * NT) {
+    uint32_t lo[kStageDir], hi[kStageDir];
+#pragma unroll
+    for (int u = 0; u < kStageDir; ++u) {
+      const int e = e0 + u * NT;
+      lo[u] = e < ndir ? (uint32_t)fdr[fbase + e] : 0u;
+      hi[u] = (e + 1 < ndir) ? (uint32_t)fdr[fbase + e + 1] : (uint32_t)qn;
+    }
+#pragma unroll
+    for (int u = 0; u < kStageDir; ++u) {
+      const int e = e0 + u * NT;
+      if (e < ndir) {
+        lpair[e] = kaddr + lo[u] * (uint32_t)sizeof(KT);
+        occ = max(occ, (int)(hi[u] - lo[u]));
+      }
+    }
   }
 #pragma unroll
   for (int sh = kWave / 2; sh > 0; sh >>= 1) occ = max(occ, __shfl_xor(occ, sh));
@@ -521,8 +598,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
     const int qn = (int)(qe - qs);
     if constexpr (DENSE) {
       __syncthreads();  // the previous class is done with the LDS table
-      const uint64_t fbase = q << d;
-      for (int e = tid; e < ndir; e += NT) ldir[e] = make_uint2(fb[fbase + e], fdr[fbase + e]);
+      stage_dense<NT>(fb, fdr, q << d, ndir, ldir, tid);
       __syncthreads();
       if (!wave_active) continue;  // the barriers above are still reached by every wave
       const VT* __restrict__ vq = v + qs;
@@ -876,6 +952,21 @@ int launch_batch(const sqdk_batch* b, const void* coef, const void* v, void* out
 // a row are stored in class and group order. The X = 0 group is kept as a
 // separate diagonal vector.
 //
+// Build workspace (optional). The counting pass also records
+//  * per chunk and class, which of the workgroup's four waves had a hit (4
+//    bits per class, 8 classes per 32-bit word, cstride words per chunk),
+//    followed by one word whose bit w says that wave w has a row with more
+//    than kShortHits hits;
+//  * per row, the source rows of its first kShortHits hits.
+// The fill pass then takes the entries of a wave whose rows all have at most
+// kShortHits hits from that record (the group of a hit follows from the X mask
+// s ^ t, the groups being sorted by X mask) and searches only in the other
+// waves, and there only the classes they hit. The lookups are deterministic,
+// so the skipped work would have found nothing and the entries are the same.
+// Where hits are rare (0.1 per row on the 40-qubit sum: about 1 in 10^4 rows
+// with more than two, 20% of the (chunk, class) pairs with any) this removes
+// nearly all of the fill pass's join.
+//
 // (A layout with one step per group and a lane mask, which keeps the join's
 // gather locality, measured slower: 7.6 hits per step on the 28-qubit chain,
 // 1.1 on the 40-qubit sum, and each step costs a wave iteration.)
@@ -892,6 +983,9 @@ constexpr uint32_t kHitPad = 0xFFFFFFFFu;
 // the padding itself, until rows from far apart in the chunk share a slice
 // and the gathers lose their locality.
 constexpr int kHitSortLog2 = 8;
+
+// Hits per row that the counting pass records for the fill (build workspace).
+constexpr int kShortHits = 2;
 
 // MODE 0: every row's hit count into rlen, then the chunk's sort: the row of
 // each slot into slot_row and each slice's entry count (64 x width) into
@@ -915,18 +1009,20 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
                  sqdk_inst single, int dmax, int max_bucket, int sort_log2,
                  uint32_t* __restrict__ slot_row, uint64_t* __restrict__ sbase,
                  uint32_t* __restrict__ col, typename CoefT<VT>::type* __restrict__ hcoef,
-                 VT* __restrict__ diag, uint32_t* __restrict__ rlen) {
+                 VT* __restrict__ diag, uint32_t* __restrict__ rlen,
+                 uint32_t* __restrict__ cmask, int cstride, uint32_t* __restrict__ rowhits) {
   using CT = typename CoefT<VT>::type;
   using RT = typename CoefT<VT>::real;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
   const int lane = tid & (kWave - 1);
-  const int wave = tid / kWave;
+  // (the wave index held in a scalar register: branches on it stay uniform)
+  const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
   const int wslot = wave * (kWave * ROWS);
   const int w = blockIdx.x;
   // The chunk's instance (a batch) or the single one; a mixed batch runs one
   // launch per lookup, each leaving the other's chunks alone.
-  const sqdk_inst D = descs ? descs[item_inst[w]] : single;
+  const sqdk_inst D = uniform_inst(descs ? descs[item_inst[w]] : single);
   if (descs && (D.fbits_base >= 0) != DENSE) return;
   const uint32_t* __restrict__ bstart_i = bstart + D.bstart_base;
   const uint16_t* __restrict__ fdir_i = fdir + D.fdir_base;
@@ -956,6 +1052,20 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   uint64_t ebase[ROWS];
   constexpr int P = ROWS * kBlock;                    // slots of the chunk
   const int64_t slot0 = (int64_t)w * P;               // its first slot (64 per slice)
+  // this chunk's class-mask words and wave flags (counting pass: cleared here;
+  // the class loop's first barrier orders the clearing before any atomicOr)
+  uint32_t* __restrict__ cm = cmask ? cmask + (int64_t)w * (cstride + 1) : nullptr;
+  const int ncm = cmask ? 8 * cstride : 0;            // classes the mask covers
+  if constexpr (MODE == 0) {
+    if (cm)
+      for (int j = tid; j <= cstride; j += kBlock) cm[j] = 0u;
+  }
+  // counting pass: the source rows of each row's first kShortHits hits
+  uint32_t hsrc[ROWS][kShortHits];
+#pragma unroll
+  for (int r = 0; r < ROWS; ++r)
+#pragma unroll
+    for (int j = 0; j < kShortHits; ++j) hsrc[r][j] = 0u;
   if constexpr (MODE == 1) {
     // the sorted slot of each of the chunk's rows (LDS is free before the classes)
     uint16_t* inv = reinterpret_cast<uint16_t*>(smem);
@@ -995,11 +1105,58 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       if (valid[r]) diag[row0 + wslot + r * kWave + lane] = dv;
     }
   }
+  // (class range and class tables read into scalar registers: the loop and its
+  // branches stay uniform)
+  const int cbase = __builtin_amdgcn_readfirstlane(D.class_base);
+  const int cend = cbase + __builtin_amdgcn_readfirstlane(D.n_classes);
+  // fill pass: waves whose rows all have at most kShortHits hits (flag bit
+  // clear) take their entries from the counting pass's record
+  const uint32_t ovf = (MODE == 1 && cm) ? cm[cstride] : 0xFu;
+  if (MODE == 1 && wave_active && ((ovf >> wave) & 1u) == 0u) {
+    // the instance's off-diagonal groups, sorted by X mask
+    const int gb = class_off[cbase], ge = class_off[cend];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+      if (!valid[r]) continue;
+      const uint32_t row = row0 + wslot + r * kWave + lane;
+      const uint32_t nh = rlen[row];
+      const uint64_t s_key = bucket_base | (uint64_t)sl[r];
+#pragma unroll
+      for (int j = 0; j < kShortHits; ++j) {
+        if ((uint32_t)j >= nh) break;
+        const uint32_t src = rowhits[(uint64_t)kShortHits * row + j];
+        const uint64_t x = s_key ^ keys[src];
+        int lo = gb, hi = ge;  // first group with X mask >= x: the hit's group
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (xmask[mid] < x) lo = mid + 1;
+          else hi = mid;
+        }
+        RT cr1[1], ci1[1];
+        const KT s1[1] = {sl[r]};
+        rows_coef<CT, KT, 1>(zmask, coef, grp_off[lo], grp_off[lo + 1], bucket_base, narrow, s1,
+                             cr1, ci1);
+        const uint64_t e = ebase[r] + (uint64_t)j * kWave;
+        col[e] = src;
+        CT c;
+        c.x = cr1[0];
+        c.y = ci1[0];
+        hcoef[e] = c;
+      }
+      cnt[r] = nh;
+    }
+  }
   // record the hits of one group (idx[r] = index in the target slice or -1)
   auto record = [&](const int (&idx)[ROWS], uint32_t qs, int gt0, int gt1) {
     if constexpr (MODE == 0) {
 #pragma unroll
-      for (int r = 0; r < ROWS; ++r) cnt[r] += idx[r] >= 0 ? 1u : 0u;
+      for (int r = 0; r < ROWS; ++r) {
+        const uint32_t src = qs + (uint32_t)idx[r];
+#pragma unroll
+        for (int j = 0; j < kShortHits; ++j)
+          hsrc[r][j] = (idx[r] >= 0 && cnt[r] == (uint32_t)j) ? src : hsrc[r][j];
+        cnt[r] += idx[r] >= 0 ? 1u : 0u;
+      }
     } else {
       RT cr[ROWS], ci[ROWS];
       rows_coef<CT, KT, ROWS>(zmask, coef, gt0, gt1, bucket_base, narrow, sl, cr, ci);
@@ -1018,20 +1175,47 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
       }
     }
   };
-  for (int c = D.class_base; c < D.class_base + D.n_classes; ++c) {
-    const int g0 = class_off[c], g1 = class_off[c + 1];
+  if constexpr (MODE == 0) {
+    if (cm) __syncthreads();  // the mask words are clear before any class records a hit
+  }
+  // fill pass: the chunk's first 64 mask words sit one per lane, so a class's
+  // word is a register read (the rest, if any, are loaded when reached)
+  uint32_t mreg = 0xFFFFFFFFu;
+  if constexpr (MODE == 1) {
+    if (cm && lane < cstride) mreg = cm[lane];
+  }
+  for (int c = cbase; c < cend; ++c) {
+    const int lc = c - cbase;  // the class within the instance
+    // fill pass: the waves of this chunk that had hits in this class (uniform
+    // over the workgroup; all four where the mask does not cover the class),
+    // tested before the class's tables are read
+    uint32_t cwaves = 0xFu;
+    if constexpr (MODE == 1) {
+      if (lc < ncm) {
+        const int wi = lc >> 3;
+        const uint32_t word = wi < kWave ? (uint32_t)__builtin_amdgcn_readlane((int)mreg, wi)
+                                         : cm[wi];
+        cwaves = (word >> (4 * (lc & 7))) & 0xFu;
+      }
+      cwaves &= ovf;               // only waves with a long row search
+      if (cwaves == 0u) continue;  // no such wave of the chunk has a hit here
+    }
+    const int g0 = __builtin_amdgcn_readfirstlane(class_off[c]);
+    const int g1 = __builtin_amdgcn_readfirstlane(class_off[c + 1]);
     if (g0 >= g1) continue;
-    const uint64_t q = bucket ^ (uint64_t)class_hi[c];
-    const uint32_t qs = bstart_i[q];
-    const uint32_t qe = bstart_i[q + 1];
+    const uint64_t q = bucket ^ (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane(
+                                    (int)class_hi[c]);
+    const uint32_t qs = (uint32_t)__builtin_amdgcn_readfirstlane((int)bstart_i[q]);
+    const uint32_t qe = (uint32_t)__builtin_amdgcn_readfirstlane((int)bstart_i[q + 1]);
     if (qs == qe) continue;
     const int qn = (int)(qe - qs);
+    // counting pass: whether this wave had a hit in the class
+    bool class_any = false;
     if constexpr (DENSE) {
       __syncthreads();
-      const uint64_t fbase = q << d;
-      for (int e = tid; e < ndir; e += kBlock) ldir[e] = make_uint2(fbits_i[fbase + e], fdir_i[fbase + e]);
+      stage_dense<kBlock>(fbits_i, fdir_i, q << d, ndir, ldir, tid);
       __syncthreads();
-      if (!wave_active) continue;
+      if (!wave_active || ((cwaves >> wave) & 1u) == 0u) continue;
       // the next group's X mask and term range are loaded one group ahead
       uint64_t xm_n = xmask[g0];
       int gt0_n = grp_off[g0], gt1_n = grp_off[g0 + 1];
@@ -1056,13 +1240,16 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
           idx[r] = hit ? (int)rank : -1;
           any_hit |= hit;
         }
-        if (__any(any_hit)) record(idx, qs, gt0, gt1);
+        if (__any(any_hit)) {
+          record(idx, qs, gt0, gt1);
+          class_any = true;
+        }
       }
     } else {
       const int steps = stage_search<KT, kBlock>(keys, qs, qn, lowmask, fdir_i, q << d, ndir,
                                                  kaddr, lkeys, lpair, s_occ, tid);
       const uint32_t last = kaddr + (uint32_t)(qn - 1) * (uint32_t)sizeof(KT);
-      if (!wave_active) continue;
+      if (!wave_active || ((cwaves >> wave) & 1u) == 0u) continue;
       uint64_t xm_n = xmask[g0];
       int gt0_n = grp_off[g0], gt1_n = grp_off[g0 + 1];
       for (int g = g0; g < g1; ++g) {
@@ -1093,8 +1280,15 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
           idx[r] = (valid[r] && hit[r]) ? (int)((ba[r] - kaddr) / (uint32_t)sizeof(KT)) : -1;
           any_hit |= idx[r] >= 0;
         }
-        if (__any(any_hit)) record(idx, qs, gt0, gt1);
+        if (__any(any_hit)) {
+          record(idx, qs, gt0, gt1);
+          class_any = true;
+        }
       }
+    }
+    if constexpr (MODE == 0) {
+      if (class_any && lane == 0 && lc < ncm)
+        atomicOr(&cm[lc >> 3], (1u << wave) << (4 * (lc & 7)));
     }
   }
   if constexpr (MODE == 0) {
@@ -1107,6 +1301,20 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
 #pragma unroll
     for (int r = 0; r < ROWS; ++r)
       if (valid[r]) rlen[row0 + wslot + r * kWave + lane] = cnt[r];
+    if (cm) {
+      bool longrow = false;
+#pragma unroll
+      for (int r = 0; r < ROWS; ++r) {
+        const uint32_t row = row0 + wslot + r * kWave + lane;
+        if (valid[r]) {
+#pragma unroll
+          for (int j = 0; j < kShortHits; ++j)
+            rowhits[(uint64_t)kShortHits * row + j] = hsrc[r][j];
+        }
+        longrow |= valid[r] && cnt[r] > (uint32_t)kShortHits;
+      }
+      if (__any(longrow) && lane == 0) atomicOr(&cm[cstride], 1u << wave);
+    }
     __syncthreads();  // the class loop is done with the lookup tables
     uint64_t* sk = reinterpret_cast<uint64_t*>(smem);
 #pragma unroll
@@ -1310,6 +1518,11 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     if (valid[u]) out[row[u]] = from_d2<VT>(make_double2(sr[u], si[u]));
 }
 
+// 32-bit words of class mask per chunk: 4 bits (one per wave) per class.
+static_assert(kBlock / kWave == 4, "the class mask holds four waves per class");
+inline int class_mask_stride(int n_classes) { return (n_classes + 7) / 8; }
+
+
 // Tables of one hit-table pass (a single instance or a batch).
 struct HitTables {
   const uint64_t* keys;
@@ -1330,7 +1543,39 @@ struct HitTables {
   int dense_fine_bits;         // dense-lookup chunks: their largest fine_bits
   bool has_dense, has_search;
   int sort_log2;               // rows of a sort window: 2^sort_log2 (6 .. log2 chunk rows)
+  uint32_t* cmask;             // class mask (cstride + 1 words per chunk), or NULL
+  int cstride;
+  uint32_t* rowhits;           // kShortHits source rows per row (with cmask)
 };
+
+// Build workspace: the class masks and wave flags (n_items x (stride + 1)
+// words), then, 256-byte aligned, kShortHits source rows per row.
+struct BuildWork {
+  size_t mask_bytes, bytes;
+};
+
+inline BuildWork build_work_layout(int64_t n_items, int n_classes, int64_t n) {
+  BuildWork b;
+  b.mask_bytes = (((size_t)n_items * (class_mask_stride(n_classes) + 1) * 4 + 255) / 256) * 256;
+  b.bytes = b.mask_bytes + (size_t)n * kShortHits * 4;
+  return b;
+}
+
+// Points the tables at a caller's build workspace (or at none).
+inline int attach_build_work(HitTables& t, int n_classes, int64_t n, const void* work,
+                             size_t work_bytes) {
+  t.cstride = class_mask_stride(n_classes);
+  t.cmask = nullptr;
+  t.rowhits = nullptr;
+  if (!work) return SQDK_OK;
+  const BuildWork b = build_work_layout(t.n_items, n_classes, n);
+  if (work_bytes < b.bytes)
+    return fail(SQDK_ENOSPC, "build workspace of %zu bytes, %zu needed", work_bytes, b.bytes);
+  t.cmask = reinterpret_cast<uint32_t*>(const_cast<void*>(work));
+  t.rowhits = reinterpret_cast<uint32_t*>(static_cast<char*>(const_cast<void*>(work)) +
+                                          b.mask_bytes);
+  return SQDK_OK;
+}
 
 template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
 int launch_hit_table(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
@@ -1349,7 +1594,8 @@ int launch_hit_table(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, ui
       t.keys, t.bstart, t.fdir, t.fbits, t.items, t.xmask, t.grp_off, t.zmask,
       reinterpret_cast<const typename CoefT<VT>::type*>(t.coef), t.class_hi, t.class_off,
       t.descs, t.item_inst, t.single, dmax, t.max_bucket, t.sort_log2, slot_row, sbase, col,
-      reinterpret_cast<typename CoefT<VT>::type*>(hcoef), reinterpret_cast<VT*>(diag), rlen);
+      reinterpret_cast<typename CoefT<VT>::type*>(hcoef), reinterpret_cast<VT*>(diag), rlen,
+      t.cmask, t.cstride, t.rowhits);
   SQDK_LAUNCHED("hit_table_kernel");
   return SQDK_OK;
 }
@@ -1651,9 +1897,19 @@ extern "C" int sqdk_cache_slices(const sqdk_space* space, int64_t* n_slices) {
   return SQDK_OK;
 }
 
+extern "C" int sqdk_cache_workspace(const sqdk_ham* ham, const sqdk_space* space,
+                                    size_t* bytes) {
+  using namespace sqdk;
+  SQDK_REQUIRE(ham && space && bytes, "NULL pointer argument");
+  SQDK_REQUIRE(space->n_items >= 0 && space->n >= 0 && ham->n_classes >= 0,
+               "negative size in ham or space");
+  *bytes = build_work_layout(space->n_items, ham->n_classes, space->n).bytes;
+  return SQDK_OK;
+}
+
 extern "C" int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space,
                                 uint32_t* slot_row, uint64_t* slice_base, uint32_t* row_len,
-                                void* stream) {
+                                void* work, size_t work_bytes, void* stream) {
   using namespace sqdk;
   SQDK_TRY(check_hit_space(ham, space));
   SQDK_REQUIRE(slot_row && slice_base && row_len, "NULL pointer argument");
@@ -1661,8 +1917,9 @@ extern "C" int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space,
   hipStream_t s = as_stream(stream);
   const int64_t ns = (int64_t)space->n_items * (space->chunk_rows / kWave);
   SQDK_HIP(hipMemsetAsync(slice_base + ns, 0, sizeof(uint64_t), s));
-  SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_single(ham, space, ham->coef32),
-                                          slot_row, slice_base, nullptr, nullptr, nullptr,
+  HitTables t = hit_tables_single(ham, space, ham->coef32);
+  SQDK_TRY(attach_build_work(t, ham->n_classes, space->n, work, work_bytes));
+  SQDK_TRY((hit_table_dispatch<float2, 0>(t, slot_row, slice_base, nullptr, nullptr, nullptr,
                                           row_len, s)));
   scan_u64_kernel<<<1, 1024, 0, s>>>(slice_base, ns + 1);
   SQDK_LAUNCHED("scan_u64_kernel");
@@ -1671,7 +1928,8 @@ extern "C" int sqdk_cache_count(const sqdk_ham* ham, const sqdk_space* space,
 
 extern "C" int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
                                const uint64_t* slice_base, const uint32_t* slot_row, uint32_t* col,
-                               void* coef, void* diag, uint32_t* row_len, void* stream) {
+                               void* coef, void* diag, uint32_t* row_len, const void* work,
+                               size_t work_bytes, void* stream) {
   using namespace sqdk;
   SQDK_TRY(check_hit_space(ham, space));
   SQDK_REQUIRE(slice_base && slot_row && col && coef && diag && row_len,
@@ -1679,17 +1937,16 @@ extern "C" int sqdk_cache_fill(const sqdk_ham* ham, const sqdk_space* space, int
   hipStream_t s = as_stream(stream);
   uint64_t* sb = const_cast<uint64_t*>(slice_base);
   uint32_t* sr = const_cast<uint32_t*>(slot_row);
-  if (precision == SQDK_C128) {
-    SQDK_REQUIRE(ham->n_terms == 0 || ham->coef, "ham has no complex128 coefficients");
-    return hit_table_dispatch<double2, 1>(hit_tables_single(ham, space, ham->coef), sr, sb,
-                                          col, coef, diag, row_len, s);
-  }
-  if (precision == SQDK_C64) {
-    SQDK_REQUIRE(ham->n_terms == 0 || ham->coef32, "ham has no complex64 coefficients");
-    return hit_table_dispatch<float2, 1>(hit_tables_single(ham, space, ham->coef32), sr, sb,
-                                         col, coef, diag, row_len, s);
-  }
-  return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  if (precision != SQDK_C128 && precision != SQDK_C64)
+    return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  const bool wide = precision == SQDK_C128;
+  SQDK_REQUIRE(ham->n_terms == 0 || (wide ? (const void*)ham->coef : (const void*)ham->coef32),
+               "ham has no %s coefficients", wide ? "complex128" : "complex64");
+  HitTables t = hit_tables_single(ham, space, wide ? (const void*)ham->coef
+                                                   : (const void*)ham->coef32);
+  SQDK_TRY(attach_build_work(t, ham->n_classes, space->n, work, work_bytes));
+  return wide ? hit_table_dispatch<double2, 1>(t, sr, sb, col, coef, diag, row_len, s)
+              : hit_table_dispatch<float2, 1>(t, sr, sb, col, coef, diag, row_len, s);
 }
 
 extern "C" int sqdk_cache_matvec(const sqdk_cache* cache, const void* v, void* out,
@@ -1708,19 +1965,34 @@ extern "C" int sqdk_batch_cache_slices(const sqdk_batch* batch, int64_t* n_slice
   return SQDK_OK;
 }
 
+extern "C" int sqdk_batch_cache_workspace(const sqdk_batch* batch, int32_t max_classes,
+                                          size_t* bytes) {
+  using namespace sqdk;
+  SQDK_REQUIRE(batch && bytes, "NULL pointer argument");
+  SQDK_REQUIRE(batch->n_items >= 0 && batch->n >= 0 && max_classes >= 0,
+               "negative size in batch or max_classes");
+  *bytes = build_work_layout(batch->n_items, max_classes, batch->n).bytes;
+  return SQDK_OK;
+}
+
 extern "C" int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slot_row,
-                                      uint64_t* slice_base, uint32_t* row_len, void* stream) {
+                                      uint64_t* slice_base, uint32_t* row_len,
+                                      int32_t max_classes, void* work, size_t work_bytes,
+                                      void* stream) {
   using namespace sqdk;
   SQDK_REQUIRE(batch && slot_row && slice_base && row_len, "NULL pointer argument");
   SQDK_REQUIRE(batch->keys && batch->bstart && batch->fdir && batch->items && batch->item_inst &&
                    batch->inst, "batch has NULL subspace tables");
   SQDK_REQUIRE(batch->n_items >= 1, "empty batch");
+  SQDK_REQUIRE(max_classes >= 0, "negative max_classes %d", max_classes);
   int64_t ns = 0;
   SQDK_TRY(sqdk_batch_cache_slices(batch, &ns));
   hipStream_t s = as_stream(stream);
   SQDK_HIP(hipMemsetAsync(slice_base + ns, 0, sizeof(uint64_t), s));
-  SQDK_TRY((hit_table_dispatch<float2, 0>(hit_tables_batch(batch, batch->coef32), slot_row,
-                                          slice_base, nullptr, nullptr, nullptr, row_len, s)));
+  HitTables t = hit_tables_batch(batch, batch->coef32);
+  SQDK_TRY(attach_build_work(t, max_classes, batch->n, work, work_bytes));
+  SQDK_TRY((hit_table_dispatch<float2, 0>(t, slot_row, slice_base, nullptr, nullptr, nullptr,
+                                          row_len, s)));
   scan_u64_kernel<<<1, 1024, 0, s>>>(slice_base, ns + 1);
   SQDK_LAUNCHED("scan_u64_kernel");
   return SQDK_OK;
@@ -1729,18 +2001,21 @@ extern "C" int sqdk_batch_cache_count(const sqdk_batch* batch, uint32_t* slot_ro
 extern "C" int sqdk_batch_cache_fill(const sqdk_batch* batch, int32_t precision,
                                      const uint64_t* slice_base, const uint32_t* slot_row,
                                      uint32_t* col, void* coef, void* diag, uint32_t* row_len,
+                                     int32_t max_classes, const void* work, size_t work_bytes,
                                      void* stream) {
   using namespace sqdk;
   SQDK_REQUIRE(batch && slice_base && slot_row && col && coef && diag && row_len,
                "NULL pointer argument");
+  SQDK_REQUIRE(max_classes >= 0, "negative max_classes %d", max_classes);
   hipStream_t s = as_stream(stream);
   uint64_t* sb = const_cast<uint64_t*>(slice_base);
   uint32_t* sr = const_cast<uint32_t*>(slot_row);
-  if (precision == SQDK_C128)
-    return hit_table_dispatch<double2, 1>(hit_tables_batch(batch, batch->coef), sr, sb, col,
-                                          coef, diag, row_len, s);
-  if (precision == SQDK_C64)
-    return hit_table_dispatch<float2, 1>(hit_tables_batch(batch, batch->coef32), sr, sb,
-                                         col, coef, diag, row_len, s);
-  return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  if (precision != SQDK_C128 && precision != SQDK_C64)
+    return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  const bool wide = precision == SQDK_C128;
+  HitTables t = hit_tables_batch(batch, wide ? (const void*)batch->coef
+                                             : (const void*)batch->coef32);
+  SQDK_TRY(attach_build_work(t, max_classes, batch->n, work, work_bytes));
+  return wide ? hit_table_dispatch<double2, 1>(t, sr, sb, col, coef, diag, row_len, s)
+              : hit_table_dispatch<float2, 1>(t, sr, sb, col, coef, diag, row_len, s);
 }

@@ -431,18 +431,23 @@ class PauliOperator:
         ns = ns.value
         stream = _lib.stream_ptr(dev)
         if self._caches:
-            # the layout (slot map, slice bases, row lengths) does not depend on
-            # the precision: the other precision's count is reused
+            # the layout (slot map, slice bases, row lengths, class mask) does
+            # not depend on the precision: the other precision's count is reused
             other = next(iter(self._caches.values()))
-            slot_row, sbase, rlen = other[1], other[2], other[6]
+            slot_row, sbase, rlen, cmask = other[1], other[2], other[6], other[7]
             entries = next(iter(self.cache_info.values()))["entries"]
         else:
             slot_row = torch.empty(64 * ns, dtype=torch.int32, device=dev)
             sbase = torch.empty(ns + 1, dtype=torch.int64, device=dev)
             rlen = torch.empty(self.space.n, dtype=torch.int32, device=dev)
+            # the count's record for the fill (class masks, rows' first hits)
+            nw = C.c_size_t(0)
+            _lib.check(L.sqdk_cache_workspace(C.byref(h), C.byref(s), C.byref(nw)),
+                       "sqdk_cache_workspace")
+            cmask = torch.empty(max(nw.value, 1), dtype=torch.uint8, device=dev)
             _lib.check(L.sqdk_cache_count(C.byref(h), C.byref(s), slot_row.data_ptr(),
-                                          sbase.data_ptr(), rlen.data_ptr(), stream),
-                       "sqdk_cache_count")
+                                          sbase.data_ptr(), rlen.data_ptr(), cmask.data_ptr(),
+                                          cmask.numel(), stream), "sqdk_cache_count")
             entries = int(sbase[-1].item())
         cdt = torch.complex128 if precision == _lib.C128 else torch.complex64
         esz = 16 if precision == _lib.C128 else 8
@@ -454,13 +459,14 @@ class PauliOperator:
         diag = torch.empty(self.space.n, dtype=cdt, device=dev)
         _lib.check(L.sqdk_cache_fill(C.byref(h), C.byref(s), precision, sbase.data_ptr(),
                                      slot_row.data_ptr(), col.data_ptr(), coef.data_ptr(),
-                                     diag.data_ptr(), rlen.data_ptr(), stream), "sqdk_cache_fill")
+                                     diag.data_ptr(), rlen.data_ptr(), cmask.data_ptr(),
+                                     cmask.numel(), stream), "sqdk_cache_fill")
         # few entries per row: the product is latency-bound, a wave takes 4 slices
         spw = 4 if entries <= 8 * self.space.n else 1
         cs = _lib.SqdkCache(self.space.n, ns, slot_row.data_ptr(), sbase.data_ptr(),
                             col.data_ptr(), coef.data_ptr(), diag.data_ptr(), rlen.data_ptr(),
                             precision, spw)
-        self._caches[precision] = (cs, slot_row, sbase, col, coef, diag, rlen)
+        self._caches[precision] = (cs, slot_row, sbase, col, coef, diag, rlen, cmask)
         self.cache_info[precision] = {"slices": ns, "entries": entries, "bytes": nbytes,
                                       "slices_per_wave": spw}
         return True

@@ -291,7 +291,7 @@ def hproj(hamiltonian, states, unique_states: bool = False) -> csr_array:
     # hit table (one bucket-join pass), unpacked from its 64-slot slices: entry
     # k of the row in slot t sits at slice_base[t // 64] + 64 k + t % 64.
     op.build_cache(_lib.C128)
-    _, slot_row, sbase, col, coef, diag, rlen = op._caches[_lib.C128]
+    _, slot_row, sbase, col, coef, diag, rlen, _ = op._caches[_lib.C128]
     dev = space.device
     sr = slot_row.view(-1).to(torch.int64) & 0xFFFFFFFF
     slot = torch.nonzero(sr != 0xFFFFFFFF).view(-1)

@@ -31,7 +31,7 @@ def _table_to_csr(op, prec, n):
     only after a row's hits, slots without a row all padding, slice widths
     equal to their largest count, row lengths."""
     import scipy.sparse as sps
-    cs, slot_row, sbase, col, coef, diag, rlen = op._caches[prec]
+    cs, slot_row, sbase, col, coef, diag, rlen, _ = op._caches[prec]
     slot_row = slot_row.cpu().numpy().view(np.uint32).reshape(-1, 64)
     sbase = sbase.cpu().numpy()
     col = col.cpu().numpy().view(np.uint32)
@@ -237,6 +237,17 @@ def test_batch_cache_matches_join(gpu, dense, rows):
         want = p.matvec(v.cpu().numpy()[lo:hi])
         assert _rel(cached[lo:hi], want) < 1e-13, i
         assert _rel(cached[lo:hi], join[lo:hi]) < 1e-13, i
+    # the build workspace changes no entry: a fill without it writes the same bits
+    import ctypes as C
+    cs, slot_row, sbase, col, coef, diag, rlen, _ = b._caches[_lib.C128]
+    ref = (torch.empty_like(col), torch.empty_like(coef), torch.empty_like(diag),
+           torch.empty_like(rlen))
+    _lib.check(_lib.lib().sqdk_batch_cache_fill(
+        C.byref(b.cstruct), _lib.C128, sbase.data_ptr(), slot_row.data_ptr(), ref[0].data_ptr(),
+        ref[1].data_ptr(), ref[2].data_ptr(), ref[3].data_ptr(), 0, None, 0,
+        _lib.stream_ptr(col.device)), "sqdk_batch_cache_fill")
+    for got, want in zip((col, coef, diag, rlen), ref):
+        assert torch.equal(got.view(torch.uint8), want.view(torch.uint8))
 
 
 @pytest.mark.parametrize("use_graph", [False, True])
@@ -255,3 +266,115 @@ def test_batch_cache_solve(gpu, use_graph):
         ev = p.ground()
         assert abs(c.eigval - ev) / abs(ev) < 1e-10
         assert abs(a.eigval - c.eigval) / abs(ev) < 1e-11
+
+
+def _fill_without_work(op, prec):
+    """The fill pass of op's table again, into fresh buffers, without the build
+    workspace (every class joined for every row); returns (col, coef, diag,
+    row_len)."""
+    import ctypes as C
+    cs, slot_row, sbase, col, coef, diag, rlen, _ = op._caches[prec]
+    h, s, _ = op.structs()
+    out = (torch.empty_like(col), torch.empty_like(coef), torch.empty_like(diag),
+           torch.empty_like(rlen))
+    _lib.check(_lib.lib().sqdk_cache_fill(C.byref(h), C.byref(s), prec, sbase.data_ptr(),
+                                          slot_row.data_ptr(), out[0].data_ptr(),
+                                          out[1].data_ptr(), out[2].data_ptr(),
+                                          out[3].data_ptr(), None, 0,
+                                          _lib.stream_ptr(col.device)), "sqdk_cache_fill")
+    return out
+
+
+@pytest.mark.parametrize("prec", [_lib.C128, _lib.C64])
+@pytest.mark.parametrize("kind", ["sparse32", "mixed28", "dense", "small"])
+def test_build_workspace_gives_the_same_table(gpu, prec, kind):
+    """The counting pass's record (class masks, wave flags, each row's first two
+    source rows) lets the fill take short rows' entries from the record and
+    skip (chunk, class) pairs and waves without hits: the table must be bit for
+    bit the one a fill without the workspace writes. The record itself is
+    checked against the table: a wave's class bit is set iff one of its rows
+    has an entry whose source lies in the class's target bucket, its flag iff
+    one of its rows has more than two entries, and a row's recorded sources are
+    its first entries."""
+    from rqutils_amd import workloads as W
+    from rqutils_amd.sqd import SQDProblem
+    if kind in ("sparse32", "mixed28"):
+        # 2^15 states of the 16-ones (14-ones) sector: 0.01 (0.13) hits per row,
+        # no row (a dozen rows) with more than two
+        nq = 32 if kind == "sparse32" else 28
+        w = W.random40(seed=3, n_states=1 << 15, n_terms=3000, pool=800, num_qubits=nq)
+        prob = SQDProblem(w.hamiltonian, w.keys, w.num_qubits, cache=True, bucket_cap=512)
+        n = w.n
+    elif kind == "dense":
+        p = small_problem(21, n=12, terms=120, nstates=2000)
+        prob = _problem(p, cache=True, dense=True, bucket_bits=8)
+        n = p.keys.size
+    else:
+        p = small_problem(22, n=12, terms=90, nstates=1500)
+        prob = _problem(p, cache=True, rows_per_thread=8)
+        n = p.keys.size
+    op = prob.op
+    assert op.build_cache(prec)
+    cs, slot_row, sbase, col, coef, diag, rlen, work = op._caches[prec]
+    ref = _fill_without_work(op, prec)
+    for got, want in zip((col, coef, diag, rlen), ref):
+        assert torch.equal(got.view(torch.uint8), want.view(torch.uint8))
+    # the record against the table: wave k of chunk i covers the rows
+    # items[i] + 64 R k .. +64 R (R rows per thread); a class's hits land in
+    # its target bucket, bucket(row) ^ class_hi
+    sp = op.space
+    nc = op.n_classes
+    stride = (nc + 7) // 8
+    items = sp.items[: sp.n_items].cpu().numpy().astype(np.int64)
+    keys = sp.keys.cpu().numpy().view(np.uint64)
+    m = sp.bucket_bits
+    chi = op.class_hi[:nc].cpu().numpy().astype(np.uint64)
+    raw = work.cpu().numpy()
+    nmask = sp.n_items * (stride + 1)
+    mask_bytes = -(-4 * nmask // 256) * 256
+    words = raw[: 4 * nmask].view(np.uint32).reshape(sp.n_items, stride + 1)
+    rowhits = raw[mask_bytes: mask_bytes + 8 * n].view(np.uint32).reshape(n, 2)
+    tab, _, _ = _table_to_csr(op, prec, n)
+    tab = tab.tocsr()
+    hcount = rlen.cpu().numpy().astype(np.int64)
+    R = sp.chunk_rows // 256
+    bucket_of = keys >> np.uint64(m)
+    skipped = longwaves = shortwaves = 0
+    for i, r0 in enumerate(items):
+        r1 = int(items[i + 1]) if i + 1 < sp.n_items else n
+        r1 = min(r1, r0 + sp.chunk_rows)
+        b = bucket_of[r0]
+        flags = 0
+        for k in range(4):
+            lo, hi = r0 + 64 * R * k, min(r0 + 64 * R * (k + 1), r1)
+            if lo < hi and np.any(hcount[lo:hi] > 2):
+                flags |= 1 << k
+            longwaves += lo < hi and bool(flags >> k & 1)
+            shortwaves += lo < hi and not flags >> k & 1
+        assert int(words[i, stride]) == flags, (i, int(words[i, stride]), flags)
+        for c in range(nc):
+            tgt = b ^ chi[c]
+            nib = (int(words[i, c // 8]) >> (4 * (c % 8))) & 0xF
+            want = 0
+            for k in range(4):
+                lo, hi = r0 + 64 * R * k, min(r0 + 64 * R * (k + 1), r1)
+                if lo >= hi:
+                    continue
+                srcs = tab[lo:hi].indices
+                if srcs.size and np.any(bucket_of[srcs] == tgt):
+                    want |= 1 << k
+            assert nib == want, (i, c, nib, want)
+            skipped += nib == 0
+    # the recorded sources of every row with at most two hits, in table order
+    cols = col.cpu().numpy().view(np.uint32)
+    sr = slot_row.cpu().numpy().view(np.uint32)
+    sb = sbase.cpu().numpy()
+    for slot in np.nonzero(sr != 0xFFFFFFFF)[0]:
+        row = int(sr[slot])
+        e0 = int(sb[slot // 64]) + slot % 64
+        for j in range(min(2, int(hcount[row]))):
+            assert rowhits[row, j] == cols[e0 + 64 * j], (row, j)
+    if kind == "sparse32":
+        assert skipped > 0.2 * sp.n_items * nc and longwaves == 0
+    if kind == "mixed28":
+        assert longwaves > 0 and shortwaves > longwaves
