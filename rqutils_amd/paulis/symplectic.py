@@ -14,7 +14,7 @@ qubit n-1, the Qiskit order).
 from __future__ import annotations
 
 import warnings
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Any
 
 import numpy as np
@@ -66,6 +66,55 @@ def _bits_to_u64(bits: np.ndarray) -> np.ndarray:
     widths = [(0, 0)] * (packed.ndim - 1) + [(8 - nbytes, 0)]
     wide = np.ascontiguousarray(np.pad(packed, widths))
     return wide.view(">u8")[..., 0].astype(np.uint64)
+
+
+# 2-bit codes of the Pauli letters (either case) in their lexicographic order;
+# 255 marks any other byte.
+_PAULI_CODE = np.full(256, 255, dtype=np.uint8)
+for _code, _letter in enumerate("IXYZ"):
+    _PAULI_CODE[ord(_letter)] = _code
+    _PAULI_CODE[ord(_letter.lower())] = _code
+
+
+def _pauli_codes(labels, width: int):
+    """``[n, width]`` letter codes (I, X, Y, Z -> 0..3, either case) of ASCII
+    Pauli strings, or None if a string holds any other character."""
+    try:
+        raw = "".join(labels).encode("ascii")
+    except (UnicodeEncodeError, TypeError):
+        return None
+    if len(raw) != width * len(labels):
+        return None
+    codes = _PAULI_CODE[np.frombuffer(raw, dtype=np.uint8)]
+    if np.any(codes == 255):
+        return None
+    return codes.reshape(len(labels), width)
+
+
+def _codes_unique(codes: np.ndarray):
+    """``(first, inverse)`` of the distinct rows of 2-bit letter codes in
+    lexicographic order: four codes per byte, big-endian, compared as byte
+    strings (code order is letter order, and every row is padded alike)."""
+    n, width = codes.shape
+    cols = -(-width // 4) * 4
+    c = np.zeros((n, cols), dtype=np.uint8)
+    c[:, :width] = codes
+    c = c.reshape(n, cols // 4, 4)
+    packed = np.ascontiguousarray((c[..., 0] << 6) | (c[..., 1] << 4) | (c[..., 2] << 2) | c[..., 3])
+    keys = packed.view(np.dtype((np.void, cols // 4))).ravel()
+    _, first, inverse = np.unique(keys, return_index=True, return_inverse=True)
+    return first, np.ravel(inverse)
+
+
+def _masks_to_packed(masks: np.ndarray, num_qubits: int, pad: bool) -> np.ndarray:
+    """The rows :func:`pack_rows` makes of the bit rows whose integer values
+    are ``masks`` (column 0 the most significant bit), for rows of at most 64
+    bits with the pad bit."""
+    nbits = num_qubits + (1 if pad else 0)
+    nbytes = -(-nbits // 8)
+    value = np.asarray(masks, dtype=np.uint64) << np.uint64(8 * nbytes - nbits)
+    be = np.ascontiguousarray(value.astype(">u8")).view(np.uint8)
+    return np.ascontiguousarray(be.reshape(value.shape + (8,))[..., 8 - nbytes:])
 
 
 def _letter_codes(labels) -> np.ndarray:
@@ -127,6 +176,8 @@ class PauliSumXZ:
     c: np.ndarray
     num_qubits: int
     padding: bool = False  # packed signatures carry the leading pad bit
+    # (xmask u64[J], zmask u64[J, Kmax]) when the constructor already had them
+    _mask_cache: Any = field(default=None, repr=False, compare=False)
 
     @classmethod
     def from_paulisum(cls, paulisum: Any, force_real: bool = False,
@@ -144,9 +195,13 @@ class PauliSumXZ:
             labels, coeffs = paulisum
             if len(labels) != len(coeffs):
                 raise ValueError("Lengths of Pauli and coeff lists do not match")
-            if len(set(len(p) for p in labels)) != 1:
+            if len(set(map(len, labels))) != 1:
                 raise ValueError("Pauli strings have non-uniform lengths")
             coeffs = np.asarray(coeffs)
+            width = len(labels[0])
+            codes = _pauli_codes(labels, width) if 0 < width <= 63 else None
+            if codes is not None:
+                return cls._from_codes(codes, coeffs, force_real, add_padding)
             letters = _letter_codes(labels)
             keep = coeffs != 0
             letters, coeffs = letters[keep], coeffs[keep]
@@ -196,6 +251,47 @@ class PauliSumXZ:
         return cls(pack_rows(xsig, add_padding), pack_rows(zsig, add_padding), phc,
                    int(num_qubits), bool(add_padding))
 
+    @classmethod
+    def _from_codes(cls, codes: np.ndarray, coeffs: np.ndarray, force_real: bool,
+                    add_padding: bool) -> "PauliSumXZ":
+        """The tuple path of :meth:`from_paulisum` on integer masks, for strings
+        of I, X, Y, Z of at most 63 letters: the same arrays, without building
+        the [J, Kmax, n] bit array."""
+        keep = coeffs != 0
+        codes, coeffs = codes[keep], coeffs[keep]
+        first, where = _codes_unique(codes)
+        codes = codes[first]
+        merged = np.zeros(codes.shape[0], dtype=np.result_type(coeffs.dtype, np.float64))
+        np.add.at(merged, where, coeffs)
+        coeffs = merged
+        num_qubits = codes.shape[1]
+        xm = _bits_to_u64((codes == 1) | (codes == 2))
+        zm = _bits_to_u64((codes == 2) | (codes == 3))
+        if force_real:
+            if np.any(np.imag(coeffs) != 0.0):
+                warnings.warn("Found nonzero imaginary part when force_real=True")
+            coeffs = np.real(coeffs)
+        # groups: distinct X masks ascending (the lexicographic order of the
+        # signatures); a group's terms keep their order
+        xsig, first_x, group_of = np.unique(xm, return_index=True, return_inverse=True)
+        group_of = np.ravel(group_of)
+        sizes = np.bincount(group_of, minlength=xsig.shape[0])
+        n_groups, kmax = xsig.shape[0], int(np.max(sizes))
+        order = np.argsort(group_of, kind="stable")
+        starts = np.concatenate([[0], np.cumsum(sizes)])
+        slot = np.empty(group_of.size, dtype=np.int64)
+        slot[order] = np.arange(group_of.size) - starts[group_of[order]]
+        zsig = np.zeros((n_groups, kmax), dtype=np.uint64)
+        zsig[group_of, slot] = zm
+        phc = np.zeros((n_groups, kmax), dtype=np.complex128)
+        phase = np.bitwise_count(xm & zm).astype(np.int64) & 3
+        phc[group_of, slot] = coeffs * _MINUS_I_POWERS[phase]
+        if np.all(phc.imag == 0.0):
+            phc = phc.real
+        return cls(_masks_to_packed(xsig, num_qubits, add_padding),
+                   _masks_to_packed(zsig, num_qubits, add_padding), phc, int(num_qubits),
+                   bool(add_padding), (xsig.astype(np.uint64), zsig))
+
     @property
     def arrays(self):
         return self.x, self.z, self.c
@@ -209,8 +305,11 @@ class PauliSumXZ:
         """
         if padded is None:
             padded = self.padding
-        xm = _packed_to_u64(self.x, self.num_qubits, padded)
-        zm = _packed_to_u64(self.z, self.num_qubits, padded)
+        if self._mask_cache is not None and padded == self.padding:
+            xm, zm = self._mask_cache
+        else:
+            xm = _packed_to_u64(self.x, self.num_qubits, padded)
+            zm = _packed_to_u64(self.z, self.num_qubits, padded)
         coef = np.asarray(self.c, dtype=np.complex128)
         nz = coef != 0
         count = np.where(nz.any(axis=1),

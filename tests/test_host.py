@@ -62,6 +62,44 @@ def test_paulisumxz_matches_oracle_packing():
             assert h.num_qubits == nq
 
 
+def test_paulisumxz_mask_path_matches_bit_path(monkeypatch):
+    """Strings of I, X, Y, Z (either case, at most 63 letters) take the
+    integer-mask construction; it must give the bit-array construction's
+    arrays byte for byte, and the same masks, on inputs with repeated strings,
+    zero and complex coefficients, lower case and one-letter strings."""
+    import rqutils_amd.paulis.symplectic as sym
+    from rqutils_amd.paulis import PauliSumXZ
+    rng = np.random.default_rng(11)
+    cases = []
+    for n in (1, 3, 7, 8, 9, 31, 40, 63):
+        labels = random_labels(rng, n, 60, 1, min(n, 4))
+        labels = labels + labels[:15]                   # repeated strings are merged
+        coeffs = rng.normal(size=len(labels))
+        coeffs[::7] = 0.0                               # zero coefficients are dropped
+        cases.append((labels, coeffs, True))
+        cases.append(([l.lower() if i % 3 == 0 else l for i, l in enumerate(labels)],
+                      coeffs + 1j * rng.normal(size=len(labels)), False))
+    for labels, coeffs, force_real in cases:
+        for pad in (False, True):
+            fast = PauliSumXZ.from_paulisum((labels, coeffs), force_real=force_real,
+                                            add_padding=pad)
+            assert fast._mask_cache is not None
+            with monkeypatch.context() as m:
+                m.setattr(sym, "_pauli_codes", lambda labels, width: None)
+                slow = PauliSumXZ.from_paulisum((labels, coeffs), force_real=force_real,
+                                                add_padding=pad)
+            assert slow._mask_cache is None
+            assert fast.x.dtype == slow.x.dtype and np.array_equal(fast.x, slow.x)
+            assert np.array_equal(fast.z, slow.z) and fast.z.shape == slow.z.shape
+            assert fast.c.dtype == slow.c.dtype and np.array_equal(fast.c, slow.c)
+            for a, b in zip(fast.masks(), slow.masks()):
+                assert np.array_equal(a, b)
+    # other letters, and strings of 64 letters, keep the bit-array construction
+    assert sym._pauli_codes(["XQZ"], 3) is None
+    long = PauliSumXZ.from_paulisum((["X" * 64, "Z" * 64], [1.0, 2.0]), add_padding=True)
+    assert long._mask_cache is None and long.x.shape == (2, 9)
+
+
 def test_paulisumxz_errors_and_masks():
     from rqutils_amd.paulis import PauliSumXZ
     with pytest.raises(ValueError):
