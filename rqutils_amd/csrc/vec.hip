@@ -14,34 +14,68 @@ namespace {
 
 constexpr int kBlock = 256;
 
-template <typename VT>
+// NP pairs exactly (1..4: the solvers' calls), or NP = kMaxPairs for any
+// count up to it. The grid is small and fixed (kDotBlocks, for reproducible
+// partial sums), so each thread keeps U elements' loads in flight per step,
+// all issued before the first is added; they are added in index order, as
+// one at a time.
+template <typename VT, int NP>
 __global__ void __launch_bounds__(kBlock)
 dots_partial_kernel(int64_t n, int npairs, PairList pl, double* __restrict__ partials,
                     const int32_t* __restrict__ skip) {
   if (skip && skip[0] != 0) return;
-  __shared__ double red[2 * kMaxPairs * (kBlock / kWave)];
-  double acc[2 * kMaxPairs];
+  constexpr int U = NP <= 2 ? 4 : 2;
+  __shared__ double red[2 * NP * (kBlock / kWave)];
+  double acc[2 * NP];
 #pragma unroll
-  for (int k = 0; k < 2 * kMaxPairs; ++k) acc[k] = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
+  for (int k = 0; k < 2 * NP; ++k) acc[k] = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  if constexpr (NP == kMaxPairs) {
+    // any count: one element per step, each pair's loads just before its use
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += stride) {
 #pragma unroll
-    for (int k = 0; k < kMaxPairs; ++k) {
-      if (k < npairs) {
-        const double2 x = Vec<VT>::load(reinterpret_cast<const VT*>(pl.a[k]), i);
-        const double2 y = Vec<VT>::load(reinterpret_cast<const VT*>(pl.b[k]), i);
-        const double2 p = cmulc(x, y);
-        acc[2 * k] += p.x;
-        acc[2 * k + 1] += p.y;
+      for (int k = 0; k < NP; ++k) {
+        if (k < npairs) {
+          const double2 x = Vec<VT>::load(reinterpret_cast<const VT*>(pl.a[k]), i);
+          const double2 y = Vec<VT>::load(reinterpret_cast<const VT*>(pl.b[k]), i);
+          const double2 p = cmulc(x, y);
+          acc[2 * k] += p.x;
+          acc[2 * k + 1] += p.y;
+        }
+      }
+    }
+  } else {
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += U * stride) {
+      double2 x[NP][U], y[NP][U];
+#pragma unroll
+      for (int k = 0; k < NP; ++k) {
+        const VT* a = reinterpret_cast<const VT*>(pl.a[k]);
+        const VT* b = reinterpret_cast<const VT*>(pl.b[k]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t j = i + u * stride < n ? i + u * stride : i;
+          x[k][u] = Vec<VT>::load(a, j);
+          y[k][u] = Vec<VT>::load(b, j);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (i + u * stride >= n) break;
+#pragma unroll
+        for (int k = 0; k < NP; ++k) {
+          const double2 p = cmulc(x[k][u], y[k][u]);
+          acc[2 * k] += p.x;
+          acc[2 * k + 1] += p.y;
+        }
       }
     }
   }
-  block_sum<2 * kMaxPairs>(acc, red);
+  block_sum<2 * NP>(acc, red);
   if (threadIdx.x == 0) {
     // a constant bound: indexed by the runtime pair count, acc would live in
     // scratch memory, and every accumulation above with it
 #pragma unroll
-    for (int k = 0; k < 2 * kMaxPairs; ++k)
+    for (int k = 0; k < 2 * NP; ++k)
       if (k < 2 * npairs) partials[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
   }
 }
@@ -53,7 +87,20 @@ dots_final_kernel(int nblocks, const double* __restrict__ partials, double* __re
   if (skip && skip[0] != 0) return;
   __shared__ double red[kBlock / kWave];
   double v[1] = {0.0};
-  for (int i = threadIdx.x; i < nblocks; i += kBlock) v[0] += partials[(int64_t)blockIdx.x * nblocks + i];
+  // eight loads in flight per thread, added in the same order as one at a time
+  const double* __restrict__ p = partials + (int64_t)blockIdx.x * nblocks;
+  constexpr int U = 8;
+  for (int i0 = threadIdx.x; i0 < nblocks; i0 += U * kBlock) {
+    double t[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = i0 + u * kBlock;
+      t[u] = i < nblocks ? p[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (i0 + u * kBlock < nblocks) v[0] += t[u];
+  }
   block_sum<1>(v, red);
   if (threadIdx.x == 0) out[blockIdx.x] = v[0];
 }
@@ -76,18 +123,36 @@ gram_partial_kernel(int64_t n, const VT* __restrict__ X, const VT* __restrict__ 
   double acc[2 * kGramSums];
 #pragma unroll
   for (int k = 0; k < 2 * kGramSums; ++k) acc[k] = 0.0;
-  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * kBlock) {
-    const double2 x = Vec<VT>::load(X, i), r = Vec<VT>::load(R, i), p = Vec<VT>::load(P, i);
-    const double2 ax = Vec<VT>::load(AX, i), ar = Vec<VT>::load(AR, i), ap = Vec<VT>::load(AP, i);
-    const double2 res = RES ? Vec<VT>::load(RES, i) : r;
-    const double2 prods[kGramSums] = {cmulc(x, x),  cmulc(r, r),  cmulc(p, p),   cmulc(x, r),  cmulc(x, p),
-                                      cmulc(r, p),  cmulc(x, ax), cmulc(r, ar),  cmulc(p, ap), cmulc(x, ar),
-                                      cmulc(x, ap), cmulc(r, ap), cmulc(ax, ax), cmulc(res, res)};
+  // (the grid is small and fixed: two elements' loads per thread in flight,
+  // added in index order)
+  constexpr int U = 2;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += U * stride) {
+    double2 x[U], r[U], p[U], ax[U], ar[U], ap[U], res[U];
 #pragma unroll
-    for (int k = 0; k < kGramSums; ++k) {
-      acc[2 * k] += prods[k].x;
-      acc[2 * k + 1] += prods[k].y;
+    for (int u = 0; u < U; ++u) {
+      const int64_t j = i + u * stride < n ? i + u * stride : i;
+      x[u] = Vec<VT>::load(X, j);
+      r[u] = Vec<VT>::load(R, j);
+      p[u] = Vec<VT>::load(P, j);
+      ax[u] = Vec<VT>::load(AX, j);
+      ar[u] = Vec<VT>::load(AR, j);
+      ap[u] = Vec<VT>::load(AP, j);
+      res[u] = RES ? Vec<VT>::load(RES, j) : r[u];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (i + u * stride >= n) break;
+      const double2 prods[kGramSums] = {
+          cmulc(x[u], x[u]),   cmulc(r[u], r[u]),   cmulc(p[u], p[u]),     cmulc(x[u], r[u]),
+          cmulc(x[u], p[u]),   cmulc(r[u], p[u]),   cmulc(x[u], ax[u]),    cmulc(r[u], ar[u]),
+          cmulc(p[u], ap[u]),  cmulc(x[u], ar[u]),  cmulc(x[u], ap[u]),    cmulc(r[u], ap[u]),
+          cmulc(ax[u], ax[u]), cmulc(res[u], res[u])};
+#pragma unroll
+      for (int k = 0; k < kGramSums; ++k) {
+        acc[2 * k] += prods[k].x;
+        acc[2 * k + 1] += prods[k].y;
+      }
     }
   }
   block_sum<2 * kGramSums>(acc, red);
@@ -317,12 +382,21 @@ int dots_impl(int64_t n, int precision, int npairs, const PairList& pl, double* 
   SQDK_REQUIRE(npairs >= 1 && npairs <= kMaxPairs, "npairs %d out of range", npairs);
   SQDK_REQUIRE(n >= 1, "empty vectors");
   const unsigned g = dot_grid(n);
-  if (precision == SQDK_C128)
-    dots_partial_kernel<double2><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip);
-  else if (precision == SQDK_C64)
-    dots_partial_kernel<float2><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip);
-  else
+  if (precision != SQDK_C128 && precision != SQDK_C64)
     return fail(SQDK_EINVAL, "unknown precision %d", precision);
+  auto launch = [&](auto vt) {
+    using VT = decltype(vt);
+    switch (npairs) {
+      case 1: dots_partial_kernel<VT, 1><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip); break;
+      case 2: dots_partial_kernel<VT, 2><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip); break;
+      case 3: dots_partial_kernel<VT, 3><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip); break;
+      case 4: dots_partial_kernel<VT, 4><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip); break;
+      default:
+        dots_partial_kernel<VT, kMaxPairs><<<g, kBlock, 0, s>>>(n, npairs, pl, partials, skip);
+    }
+  };
+  if (precision == SQDK_C128) launch(double2{});
+  else launch(float2{});
   SQDK_LAUNCHED("dots_partial_kernel");
   dots_final_kernel<<<2 * npairs, kBlock, 0, s>>>((int)g, partials, out, skip);
   SQDK_LAUNCHED("dots_final_kernel");
