@@ -40,6 +40,8 @@
 #include "internal.hpp"
 
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 namespace sqdk {
 
@@ -549,18 +551,21 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
 
   // Waves of a short chunk that hold no row skip the per-group work.
   const bool wave_active = __any(valid[0]);
-  // Dense lookup: byte offset of each row's 32-value sub-bucket entry; the
+  // Dense lookup: LDS address of each row's 32-value sub-bucket entry; the
   // target's entry is this XOR the group's (x >> 5) << 3.
-  // Search lookup: byte offset of each row's sub-bucket entry (the target's
+  // Search lookup: LDS address of each row's sub-bucket entry (the target's
   // is this XOR the group's (x >> f) << 2, since (s ^ x) >> f = (s >> f) ^ (x >> f)).
+  // Both tables start at the dynamic LDS base, which is address 0 (the kernel
+  // declares no static LDS; launch_rows checks), so base + (off ^ y) equals
+  // (base + off) ^ y and a lookup's address is one XOR.
+  const uint32_t tbase = lds_addr(smem);
   uint32_t sa[ROWS];
 #pragma unroll
   for (int r = 0; r < ROWS; ++r) {
-    sa[r] = DENSE ? ((uint32_t)sl[r] >> 5) << 3 : (uint32_t)(sl[r] >> f) << 2;
-    // keep the precomputed offset: one XOR per lookup instead of a shift and mask
+    sa[r] = tbase + (DENSE ? ((uint32_t)sl[r] >> 5) << 3 : (uint32_t)(sl[r] >> f) << 2);
+    // keep the precomputed address: one XOR per lookup instead of a shift and mask
     __asm__ __volatile__("" : "+v"(sa[r]));
   }
-  const uint32_t paddr = lds_addr(lpair);  // LDS address of the sub-bucket table
 
   // X = 0 group: the diagonal (first share only), one pass over its terms
   // for all rows of the thread.
@@ -626,7 +631,8 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
           // (bit-field extracts use the low 5 bits of t as offset and width);
           // the rank is formed for every row and masked to -1 on a miss
           const uint32_t t = (uint32_t)sl[r] ^ xl;
-          const uint2 pr = *reinterpret_cast<const uint2*>(smem + (sa[r] ^ xa));
+          const uint64_t pw = lds_load<uint64_t>(sa[r] ^ xa);
+          const uint2 pr = make_uint2((uint32_t)pw, (uint32_t)(pw >> 32));
           const uint32_t rank = pr.y + (uint32_t)__popc(__builtin_amdgcn_ubfe(pr.x, 0u, t));
           const bool hit = valid[r] && __builtin_amdgcn_ubfe(pr.x, t, 1u) != 0u;
           idx[r] = (int)(rank | (hit ? 0u : 0xFFFFFFFFu));
@@ -689,7 +695,7 @@ bucket_join_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          ba[r] = lds_load<uint32_t>(paddr + (sa[r] ^ xa));
+          ba[r] = lds_load<uint32_t>(sa[r] ^ xa);
         }
         bool hit[ROWS];
         lift_search<KT, ROWS>(ba, tl, steps, last, hit);
@@ -754,10 +760,29 @@ split_reduce_kernel(int64_t n, int nparts, const VT* __restrict__ partial, VT* _
   }
 }
 
+// The lookups fold the LDS base of the tables into each row's address, which
+// needs the dynamic LDS to start at address 0: the kernel must declare no
+// static LDS. Checked once per kernel.
+inline int check_lds_base(const void* kern, const char* name) {
+  static std::mutex mu;
+  static std::vector<const void*> ok;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const void* k : ok)
+    if (k == kern) return SQDK_OK;
+  hipFuncAttributes attr{};
+  SQDK_HIP(hipFuncGetAttributes(&attr, kern));
+  if (attr.sharedSizeBytes != 0)
+    return fail(SQDK_EHIP, "%s declares %zu bytes of static LDS; its lookups need none", name,
+                attr.sharedSizeBytes);
+  ok.push_back(kern);
+  return SQDK_OK;
+}
+
 template <typename VT, typename KT, int ROWS, bool SV, bool DENSE, bool COMPACT, int NT = kBlock>
 int launch_rows(const MatvecTables<VT>& t, const MatvecParams& args, size_t lds, hipStream_t s) {
   // (t.descs / t.item_inst are NULL for a single instance)
   auto kern = bucket_join_kernel<VT, KT, ROWS, SV, DENSE, COMPACT, NT>;
+  SQDK_TRY(check_lds_base(reinterpret_cast<const void*>(kern), "bucket_join_kernel"));
   if (lds > 65536) {
     SQDK_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -1050,7 +1075,9 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
   const bool narrow = ((bucket_base | lowmask) >> 32) == 0;
   const uint32_t row1 = min(bstart_i[bucket + 1], row0 + (uint32_t)(ROWS * kBlock));
   uint32_t sa[ROWS];
-  const uint32_t paddr = lds_addr(lpair);  // LDS address of the sub-bucket table
+  // (the sub-bucket table starts at the dynamic LDS base, address 0: see the
+  // join kernel)
+  const uint32_t tbase = lds_addr(smem);
   KT sl[ROWS];
   bool valid[ROWS];
   uint32_t cnt[ROWS];
@@ -1085,7 +1112,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
     const uint32_t i = row0 + wslot + r * kWave + lane;
     valid[r] = i < row1;
     sl[r] = valid[r] ? (KT)(keys[i] & lowmask) : (KT)0;
-    sa[r] = DENSE ? 0u : (uint32_t)(sl[r] >> f) << 2;  // the row's sub-bucket entry offset
+    sa[r] = DENSE ? 0u : tbase + ((uint32_t)(sl[r] >> f) << 2);  // the row's sub-bucket entry
     cnt[r] = 0;
     ebase[r] = 0;
     if constexpr (MODE == 1) {
@@ -1273,7 +1300,7 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
           tl[r] = sl[r] ^ xl;
-          ba[r] = lds_load<uint32_t>(paddr + (sa[r] ^ xa));
+          ba[r] = lds_load<uint32_t>(sa[r] ^ xa);
         }
         bool hit[ROWS];
         lift_search<KT, ROWS>(ba, tl, steps, last, hit);
@@ -1586,6 +1613,7 @@ template <typename VT, typename KT, int ROWS, bool DENSE, int MODE>
 int launch_hit_table(const HitTables& t, uint32_t* slot_row, uint64_t* sbase, uint32_t* col,
                      void* hcoef, void* diag, uint32_t* rlen, hipStream_t s) {
   auto kern = hit_table_kernel<VT, KT, ROWS, DENSE, MODE>;
+  SQDK_TRY(check_lds_base(reinterpret_cast<const void*>(kern), "hit_table_kernel"));
   const int dmax = DENSE ? t.dense_fine_bits : t.max_fine_bits;
   // (the chunk's sort keys, or its slot map, reuse the LDS after the classes)
   const size_t lds = std::max<size_t>(
