@@ -118,9 +118,10 @@ class Subspace:
             _lib.check(L.sqdk_space_scan(kt.data_ptr(), n, self.num_qubits,
                                          self.pc_mask.data_ptr(), err.data_ptr(), stream),
                        "sqdk_space_scan")
-            if int(err.item()) != 0:
+            n_err = int(err.item())
+            if n_err != 0:
                 raise ValueError(f"keys must be strictly ascending and below 2^{num_qubits} "
-                                 f"({int(err.item())} violations)")
+                                 f"({n_err} violations)")
             self.rows_per_thread = int(rows_per_thread)
             if block_threads not in (256, 512) or (block_threads == 512 and rows_per_thread != 4):
                 raise ValueError("block_threads must be 256, or 512 with 4 rows per thread")
@@ -146,9 +147,23 @@ class Subspace:
                    "sqdk_space_dir")
         return d
 
-    @staticmethod
-    def _max_bucket(d: torch.Tensor) -> int:
-        return int((d[1:] - d[:-1]).max().item())
+    def _widest_bits(self, cap: int) -> int:
+        """Largest m for which no bucket ``key >> m`` holds more than ``cap`` keys.
+
+        The keys are strictly ascending, so a bucket holds more than ``cap`` of
+        them exactly when some window ``keys[i] .. keys[i + cap]`` lies in it,
+        i.e. when the highest set bit of ``keys[i] ^ keys[i + cap]`` is below m.
+        The answer is the smallest such highest bit over all windows (64 when
+        there is no window): one pass and one host read, where a search over m
+        would build a directory per probe.
+        """
+        if self.n <= cap:
+            return 64
+        x = torch.bitwise_xor(self.keys[cap:], self.keys[:-cap])
+        # (x >> 1) as an unsigned value, so the minimum is the unsigned one;
+        # x > 0, and its highest bit is the bit length of x >> 1.
+        v = int(((x >> 1) & 0x7FFFFFFFFFFFFFFF).min().item())
+        return v.bit_length()
 
     def _plan_dense(self, bucket_bits):
         nq = self.num_qubits
@@ -160,8 +175,7 @@ class Subspace:
         if not lo_m <= m <= min(nq, 19):
             raise ValueError(f"bucket_bits={m} outside {lo_m}..{min(nq, 19)} for the dense lookup")
         # Buckets must hold fewer than 2^16 keys (16-bit offsets).
-        while m > lo_m and self._max_bucket(self._dir(m)) >= 65536:
-            m -= 1
+        m = max(lo_m, min(m, self._widest_bits(65535)))
         self._plan(m, None, m - 5)
         L = _lib.lib()
         self.fbits = torch.empty(self.n_buckets << self.fine_bits, dtype=torch.int32,
@@ -178,31 +192,26 @@ class Subspace:
         hi_m = min(nq, 63)
         if bucket_bits is None:
             # Largest m with max bucket <= cap (the max bucket shrinks with m).
-            if self._max_bucket(self._dir(lo_m)) > cap:
+            widest = self._widest_bits(cap)
+            if widest < lo_m:
                 raise ValueError(f"keys too concentrated: a bucket at the finest level 2^{lo_m} "
                                  f"holds more than {cap} keys")
-            a, b = lo_m, hi_m  # invariant: max_bucket(a) <= cap
-            while a < b:
-                mid = (a + b + 1) // 2
-                if self._max_bucket(self._dir(mid)) <= cap:
-                    a = mid
-                else:
-                    b = mid - 1
-            bucket_bits = a
+            bucket_bits = min(widest, hi_m)
         if not lo_m <= bucket_bits <= hi_m:
             raise ValueError(f"bucket_bits={bucket_bits} outside {lo_m}..{hi_m}")
         self.bucket_bits = int(bucket_bits)
         self.bstart = self._dir(self.bucket_bits)
         self.n_buckets = 1 << (nq - self.bucket_bits)
-        self.max_bucket = self._max_bucket(self.bstart)
+        sizes = self.bstart[1:] - self.bstart[:-1]
+        max_bucket, nonempty = torch.stack([sizes.max(), torch.count_nonzero(sizes)]).tolist()
+        self.max_bucket = int(max_bucket)
         if self.max_bucket >= 65536:
             raise ValueError(f"largest bucket holds {self.max_bucket} keys (limit 65535)")
         if fine_bits is None:
             # Sized by the mean non-empty bucket: 2^-fine_shift of its keys per
             # sub-bucket on average (4 by default). A coarser directory stages
             # fewer bytes per class; the search then takes a few more LDS steps.
-            sizes = self.bstart[1:] - self.bstart[:-1]
-            mean = float(self.n) / max(1, int(torch.count_nonzero(sizes).item()))
+            mean = float(self.n) / max(1, int(nonempty))
             fine_bits = min(self.bucket_bits, 13,
                             max(0, int(round(np.log2(max(mean, 1.0)))) + self.fine_shift))
             # and at most max(2^22, 8 n) directory entries in all (few keys in
@@ -223,9 +232,10 @@ class Subspace:
         _lib.check(L.sqdk_space_items(self.bstart.data_ptr(), self.n_buckets, self.chunk_rows,
                                       self.items.data_ptr(), max_items, n_items.data_ptr(),
                                       stream), "sqdk_space_items")
-        self.n_items = int(n_items.item())
-        popcounts = int(self.pc_mask[0].item()) & ((1 << 64) - 1)
-        popcounts |= (int(self.pc_mask[1].item()) & 1) << 64
+        n_items_h, pc0, pc1 = torch.cat([n_items.to(torch.int64), self.pc_mask]).tolist()
+        self.n_items = int(n_items_h)
+        popcounts = int(pc0) & ((1 << 64) - 1)
+        popcounts |= (int(pc1) & 1) << 64
         self.popcounts = popcounts
 
     # -- C view ----------------------------------------------------------------

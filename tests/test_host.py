@@ -130,6 +130,44 @@ def test_keys_round_trip_and_order():
             assert np.array_equal(keys, so.packed_to_keys(so.pack_states(states), n))
 
 
+def _max_bucket_brute(keys, m):
+    hi = [int(k) >> m for k in keys]
+    best, run = 1, 1
+    for a, b in zip(hi, hi[1:]):
+        run = run + 1 if a == b else 1
+        best = max(best, run)
+    return best
+
+
+def test_widest_bucket_bits_matches_brute_force():
+    import torch
+    from rqutils_amd.operator import Subspace
+    rng = np.random.default_rng(3)
+    cases = []
+    for nq in (6, 12, 20, 40, 63, 64):
+        for n in (1, 5, 40, 300):
+            if n > 2 ** min(nq, 20):
+                continue
+            keys = np.unique(rng.integers(0, 2 ** min(nq, 62), 4 * n, dtype=np.uint64)
+                             << np.uint64(max(0, nq - 62)))[:n]
+            cases.append((nq, keys))
+    # clustered keys, one near 2^64 (the unsigned comparison), consecutive runs
+    cases.append((64, np.array([1, 2, 3, 2 ** 63, 2 ** 63 + 5, 2 ** 64 - 1], dtype=np.uint64)))
+    cases.append((16, np.arange(1000, 1300, dtype=np.uint64)))
+    for nq, keys in cases:
+        sp = Subspace.__new__(Subspace)
+        sp.keys = torch.from_numpy(keys.view(np.int64).copy())
+        sp.n = keys.size
+        for cap in (1, 2, 3, 7, 64, 65535):
+            got = sp._widest_bits(cap)
+            want = 0
+            while want < 64 and _max_bucket_brute(keys, want + 1) <= cap:
+                want += 1
+            if keys.size <= cap:
+                want = 64
+            assert got == want, (nq, keys.size, cap, got, want)
+
+
 class _FakeSpace:
     def __init__(self, m, parity_single):
         self.bucket_bits = m
