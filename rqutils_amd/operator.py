@@ -363,7 +363,7 @@ class PauliOperator:
 
     @staticmethod
     def _tables(ham: PauliSumXZ, space: Subspace, prune: bool) -> _HostTables:
-        xm, zm, coef, count = ham.masks()
+        xm, toff, tz, tc = ham.ragged()
         m = space.bucket_bits
         is_diag = xm == 0
         diag_idx = np.nonzero(is_diag)[0]
@@ -381,11 +381,13 @@ class PauliOperator:
         # the X = 0 terms first, then the off-diagonal groups in class order;
         # within a group its nonzero terms in slot order
         groups = np.concatenate([diag_idx, off_idx])
-        slots = np.arange(coef.shape[1])
-        valid = (slots[None, :] < count[groups][:, None]) & (coef[groups] != 0)
-        zmask = zm[groups][valid]
-        coefs = coef[groups][valid]
-        per_group = np.count_nonzero(valid, axis=1)
+        per_group = toff[groups + 1] - toff[groups]
+        # the groups' term ranges laid end to end
+        ends = np.cumsum(per_group)
+        take = np.arange(int(ends[-1]) if ends.size else 0, dtype=np.int64)
+        take += np.repeat(toff[groups] - (ends - per_group), per_group)
+        zmask = tz[take]
+        coefs = tc[take]
         diag_terms = int(per_group[:diag_idx.size].sum())
         offs = diag_terms + np.concatenate([[0], np.cumsum(per_group[diag_idx.size:])])
         if hi.size:

@@ -96,6 +96,13 @@ def _codes_unique(codes: np.ndarray):
     lexicographic order: four codes per byte, big-endian, compared as byte
     strings (code order is letter order, and every row is padded alike)."""
     n, width = codes.shape
+    if width <= 32:
+        # two bits per letter, the first letter highest: one integer per row
+        # orders the rows like the byte strings below
+        weights = np.uint64(1) << (np.uint64(2) * np.arange(width - 1, -1, -1, dtype=np.uint64))
+        keys = (codes.astype(np.uint64) * weights).sum(axis=1, dtype=np.uint64)
+        _, first, inverse = np.unique(keys, return_index=True, return_inverse=True)
+        return first, np.ravel(inverse)
     cols = -(-width // 4) * 4
     c = np.zeros((n, cols), dtype=np.uint8)
     c[:, :width] = codes
@@ -178,6 +185,8 @@ class PauliSumXZ:
     padding: bool = False  # packed signatures carry the leading pad bit
     # (xmask u64[J], zmask u64[J, Kmax]) when the constructor already had them
     _mask_cache: Any = field(default=None, repr=False, compare=False)
+    # the nonzero terms grouped (see ``ragged``) when the constructor had them
+    _ragged_cache: Any = field(default=None, repr=False, compare=False)
 
     @classmethod
     def from_paulisum(cls, paulisum: Any, force_real: bool = False,
@@ -281,16 +290,26 @@ class PauliSumXZ:
         starts = np.concatenate([[0], np.cumsum(sizes)])
         slot = np.empty(group_of.size, dtype=np.int64)
         slot[order] = np.arange(group_of.size) - starts[group_of[order]]
+        # The padded [J, Kmax] arrays are mostly zero padding where one group
+        # (the X = 0 terms, say) is large: they are scattered from the T
+        # terms, and the ragged form is kept for the operator tables.
         zsig = np.zeros((n_groups, kmax), dtype=np.uint64)
         zsig[group_of, slot] = zm
-        phc = np.zeros((n_groups, kmax), dtype=np.complex128)
         phase = np.bitwise_count(xm & zm).astype(np.int64) & 3
-        phc[group_of, slot] = coeffs * _MINUS_I_POWERS[phase]
-        if np.all(phc.imag == 0.0):
-            phc = phc.real
-        return cls(_masks_to_packed(xsig, num_qubits, add_padding),
-                   _masks_to_packed(zsig, num_qubits, add_padding), phc, int(num_qubits),
-                   bool(add_padding), (xsig.astype(np.uint64), zsig))
+        phased = coeffs * _MINUS_I_POWERS[phase]
+        real = bool(np.all(phased.imag == 0.0))
+        phc = np.zeros((n_groups, kmax), dtype=np.float64 if real else np.complex128)
+        phc[group_of, slot] = phased.real if real else phased
+        packed_z = np.zeros((n_groups, kmax, -(-(num_qubits + add_padding) // 8)), np.uint8)
+        packed_z[group_of, slot] = _masks_to_packed(zm, num_qubits, add_padding)
+        tz, tc = zm[order], phased[order].astype(np.complex128)
+        nonzero = tc != 0
+        counts = np.bincount(group_of[order][nonzero], minlength=n_groups)
+        offs = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        xsig = xsig.astype(np.uint64)
+        return cls(_masks_to_packed(xsig, num_qubits, add_padding), packed_z, phc,
+                   int(num_qubits), bool(add_padding), (xsig, zsig),
+                   (xsig, offs, tz[nonzero], tc[nonzero]))
 
     @property
     def arrays(self):
@@ -315,6 +334,18 @@ class PauliSumXZ:
         count = np.where(nz.any(axis=1),
                          coef.shape[1] - np.argmax(nz[:, ::-1], axis=1), 0).astype(np.int32)
         return xm, zm, coef, count
+
+    def ragged(self):
+        """The nonzero terms grouped: ``(xmask u64[J], offs i64[J+1], zmask
+        u64[T], coef c128[T])``; group g's terms are ``offs[g]:offs[g + 1]``, in
+        slot order (the terms ``masks()`` counts, zero coefficients left out)."""
+        if self._ragged_cache is not None:
+            return self._ragged_cache
+        xm, zm, coef, count = self.masks()
+        slots = np.arange(coef.shape[1])
+        valid = (slots[None, :] < count[:, None]) & (coef != 0)
+        offs = np.concatenate([[0], np.cumsum(np.count_nonzero(valid, axis=1))]).astype(np.int64)
+        return xm, offs, zm[valid], coef[valid]
 
     def matmul(self, rhs):
         """Full-space product ``H @ rhs`` over all ``2**num_qubits`` basis states.

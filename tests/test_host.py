@@ -79,6 +79,8 @@ def test_paulisumxz_mask_path_matches_bit_path(monkeypatch):
         cases.append((labels, coeffs, True))
         cases.append(([l.lower() if i % 3 == 0 else l for i, l in enumerate(labels)],
                       coeffs + 1j * rng.normal(size=len(labels)), False))
+    # merged coefficients that cancel leave a zero term in its group
+    cases.append((["XZ", "ZX", "XZ", "YY", "XX"], np.array([1.0, 2.0, -1.0, 0.5, 0.25]), True))
     for labels, coeffs, force_real in cases:
         for pad in (False, True):
             fast = PauliSumXZ.from_paulisum((labels, coeffs), force_real=force_real,
@@ -94,6 +96,11 @@ def test_paulisumxz_mask_path_matches_bit_path(monkeypatch):
             assert fast.c.dtype == slow.c.dtype and np.array_equal(fast.c, slow.c)
             for a, b in zip(fast.masks(), slow.masks()):
                 assert np.array_equal(a, b)
+            # the grouped nonzero terms, kept by the mask construction and
+            # derived from masks() otherwise
+            assert fast._ragged_cache is not None and slow._ragged_cache is None
+            for a, b in zip(fast.ragged(), slow.ragged()):
+                assert a.dtype == b.dtype and np.array_equal(a, b)
     # other letters, and strings of 64 letters, keep the bit-array construction
     assert sym._pauli_codes(["XQZ"], 3) is None
     long = PauliSumXZ.from_paulisum((["X" * 64, "Z" * 64], [1.0, 2.0]), add_padding=True)
