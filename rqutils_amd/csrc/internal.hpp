@@ -62,12 +62,19 @@ int gram_impl(int64_t n, int precision, const void* const* v, const void* const*
 // With gpart (kGramPartialDoubles doubles), the pass also leaves the next
 // iteration's Gram sums, except R.AR, as per-block partials there;
 // gram_rar_impl adds R.AR after the next product and reduces all 14.
+// cdiag (the hit table's diagonal, cache precision; with diag only) replaces
+// diag in the preconditioner; cdiag_mode bit 0: AW is the product without its
+// diagonal, which the pass adds (cdiag W); bit 1: the next product will be
+// without it too, so the pass leaves sum cdiag |W'|^2 as R.AR's partial for
+// gram_rar_impl to add to (accumulate).
 int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, void* W,
                       const void* AW, void* P, void* AP, const double* diag,
                       const double* slots, int c_base, int stat_slot, double* gpart,
-                      hipStream_t s, const int32_t* skip = nullptr);
+                      hipStream_t s, const int32_t* skip = nullptr,
+                      const void* cdiag = nullptr, int cdiag_mode = 0);
 int gram_rar_impl(int64_t n, int precision, const void* W, const void* AW, double* gpart,
-                  double* out, hipStream_t s, const int32_t* skip = nullptr);
+                  double* out, hipStream_t s, const int32_t* skip = nullptr,
+                  bool accumulate = false);
 // Partial sums of the split Gram path: 28 doubles per block of a streaming pass.
 constexpr int64_t kGramPartialDoubles = (int64_t)28 * 8192;
 
@@ -81,13 +88,14 @@ int precond_impl(int64_t n, int precision, const void* R, const double* diag,
 int diag_stats_impl(int64_t n, const double* diag, double* partials, int max_blocks,
                     double* out, hipStream_t s);
 
-// out = the cached hit table's product (precision must be the cache's).
-// In a batch, item_inst / inst_skip / slices_per_item leave converged
-// instances' slices alone.
+// out = the cached hit table's product (precision must be the cache's), or
+// with offdiag its part off the diagonal (the table's diagonal and v's own
+// entries are not read). In a batch, item_inst / inst_skip / slices_per_item
+// leave converged instances' slices alone.
 int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void* out,
                     hipStream_t s, const int32_t* skip = nullptr, int32_t* counter = nullptr,
                     const uint16_t* item_inst = nullptr, const int32_t* inst_skip = nullptr,
-                    int slices_per_item = 0);
+                    int slices_per_item = 0, bool offdiag = false);
 
 int convert_impl(int64_t n, int from_p, const void* in, int to_p, void* out, hipStream_t s);
 

@@ -406,6 +406,11 @@ struct Solver {
     return matvec_impl(op->ham, op->space, prec, in, out, op->tune, mv_work, mv_bytes, s,
                        skip ? F + kConv : nullptr, F + kMatvecs);
   }
+  // The hit table's product without its diagonal (the fused update adds it).
+  int apply_offdiag(const void* in, void* out, bool skip) {
+    return hit_matvec_impl(op->cache, prec, in, out, s, skip ? F + kConv : nullptr,
+                           F + kMatvecs, nullptr, nullptr, 0, true);
+  }
   int dots(std::initializer_list<std::pair<const void*, const void*>> pairs, bool skip,
            int slot = kD) {
     PairList pl{};
@@ -605,18 +610,35 @@ int run_fused(Solver& sv, void* x, const sqdk_locg_opts& o, int32_t* flags_host)
   // and A W are read after the product.
   bool full_gram = true;
   double* gram_out = reinterpret_cast<double*>(sv.S + kGram);
+  // With the hit table and the diagonal preconditioner, the products inside a
+  // refresh period skip the table's diagonal: the update pass, which reads W
+  // and a diagonal anyway, adds it (and its share of R.AR), so a product reads
+  // 2 vector elements less per row. Products whose result the full Gram pass
+  // reads (a period's first) and the refreshes stay whole.
+  const bool split_diag = sv.diag && sv.op->cache && !sv.op->fn;
+  const void* cdiag = split_diag ? sv.op->cache->diag : nullptr;
   auto iterate = [&](int count) -> int {
     for (int it = 0; it < count; ++it) {
-      SQDK_TRY(sv.apply(W, AW, true));
+      const bool off = split_diag && !full_gram;
+      if (off)
+        SQDK_TRY(sv.apply_offdiag(W, AW, true));
+      else
+        SQDK_TRY(sv.apply(W, AW, true));
       if (full_gram)
         SQDK_TRY(gram_impl(sv.n, sv.prec, V, AV, res, gram_out, sv.partials, sv.s,
                            sv.F + kConv));
       else
-        SQDK_TRY(gram_rar_impl(sv.n, sv.prec, W, AW, sv.gpart, gram_out, sv.s, sv.F + kConv));
+        SQDK_TRY(gram_rar_impl(sv.n, sv.prec, W, AW, sv.gpart, gram_out, sv.s, sv.F + kConv,
+                               off));
       SQDK_TRY(sv.scalar(kStepFusedRR, true));
+      // the next iteration's product skips the diagonal unless a refresh
+      // (after this update) makes it a period's first
+      const bool refresh_next = o.refresh_every > 0 && since_refresh + 1 >= o.refresh_every;
+      const bool off_next = split_diag && !refresh_next;
       SQDK_TRY(fused_update_impl(sv.n, sv.prec, X, AX, R, W, AW, P, AP, sv.diag,
                                  reinterpret_cast<const double*>(sv.S), kUpd, kDiagStat,
-                                 sv.gpart, sv.s, sv.F + kConv));
+                                 sv.gpart, sv.s, sv.F + kConv, cdiag,
+                                 (off ? 1 : 0) | (off_next ? 2 : 0)));
       full_gram = false;
       if (o.refresh_every > 0 && ++since_refresh >= o.refresh_every) {
         since_refresh = 0;

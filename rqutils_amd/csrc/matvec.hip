@@ -1472,9 +1472,12 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     const int64_t row = slot;
     const uint64_t b0 = sbase[sid0] + lane;
     const int len = (int)rlen[row];
-    const double2 dd = to_d2(diag[row]), xd = to_d2(v[row]);
-    double sr = dd.x * xd.x - dd.y * xd.y;
-    double si = dd.x * xd.y + dd.y * xd.x;
+    double sr = 0.0, si = 0.0;
+    if (diag) {
+      const double2 dd = to_d2(diag[row]), xd = to_d2(v[row]);
+      sr = dd.x * xd.x - dd.y * xd.y;
+      si = dd.x * xd.y + dd.y * xd.x;
+    }
     // hits per step: more gathers in flight for complex64 (1.75 vs 1.79 ms on the
     // 28-qubit chain); complex128 is faster at 4 (2.66 vs 2.74 ms)
     constexpr int U = sizeof(VT) == 8 ? 8 : 4;
@@ -1506,7 +1509,7 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     out[row] = from_d2<VT>(make_double2(sr, si));
     return;
   }
-  bool valid[S];
+  bool valid[S], wide[S];
   int64_t row[S];
   uint64_t b0[S];
   int len[S];
@@ -1518,15 +1521,20 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     valid[u] = sid0 + u < n_slices && slot != kHitPad;
     row[u] = valid[u] ? (int64_t)slot : 0;
     b0[u] = sbase[sid] + lane;
+    // a slice of rows without hits has no entries: its row lengths are not read
+    wide[u] = sbase[sid + 1] > sbase[sid];
   }
   int kmax = 0;
 #pragma unroll
   for (int u = 0; u < S; ++u) {
-    len[u] = valid[u] ? (int)rlen[row[u]] : 0;
+    len[u] = (valid[u] && wide[u]) ? (int)rlen[row[u]] : 0;
     kmax = max(kmax, len[u]);
-    const double2 dd = to_d2(diag[row[u]]), xd = to_d2(v[row[u]]);
-    sr[u] = valid[u] ? dd.x * xd.x - dd.y * xd.y : 0.0;
-    si[u] = valid[u] ? dd.x * xd.y + dd.y * xd.x : 0.0;
+    sr[u] = si[u] = 0.0;
+    if (diag) {
+      const double2 dd = to_d2(diag[row[u]]), xd = to_d2(v[row[u]]);
+      sr[u] = valid[u] ? dd.x * xd.x - dd.y * xd.y : 0.0;
+      si[u] = valid[u] ? dd.x * xd.y + dd.y * xd.x : 0.0;
+    }
   }
   for (int k = 0; k < kmax; ++k) {
     uint32_t j[S];
@@ -1807,7 +1815,8 @@ int matvec_impl(const sqdk_ham* ham, const sqdk_space* space, int32_t precision,
 
 int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void* out,
                     hipStream_t s, const int32_t* skip, int32_t* counter,
-                    const uint16_t* item_inst, const int32_t* inst_skip, int slices_per_item) {
+                    const uint16_t* item_inst, const int32_t* inst_skip, int slices_per_item,
+                    bool offdiag) {
   SQDK_REQUIRE(c != nullptr, "cache is NULL");
   SQDK_REQUIRE(v && out, "NULL vector");
   SQDK_REQUIRE(c->precision == precision, "cache precision %d, vectors %d", c->precision,
@@ -1827,7 +1836,8 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
     kern<<<(unsigned)grid, kBlock, 0, s>>>(
         c->n_slices, c->slot_row, c->slice_base, c->col,
         reinterpret_cast<const typename CoefT<VT>::type*>(c->coef),
-        reinterpret_cast<const VT*>(c->diag), c->row_len, reinterpret_cast<const VT*>(v),
+        offdiag ? nullptr : reinterpret_cast<const VT*>(c->diag), c->row_len,
+        reinterpret_cast<const VT*>(v),
         reinterpret_cast<VT*>(out), skip, counter, item_inst, inst_skip,
         slices_per_item > 0 ? slices_per_item : 1);
   };

@@ -203,13 +203,20 @@ __device__ inline double precond_scale(double d, double theta, double2 st) {
 // over the values it stores: every sum of gram_partial_kernel except R.AR
 // (the next product's, summed by gram_rar_kernel), with X.AR taken as
 // AX.R (A Hermitian).
-template <typename VT>
+// With the hit table's diagonal cdiag (the preconditioner then reads its real
+// part instead of diag): OFF_IN, AW is the product without the diagonal, and
+// A W = AW + cdiag W is formed here from the W it reads anyway; OFF_NEXT, the
+// next product is without it too, and the diagonal's share of R.AR, sum
+// cdiag |W'|^2, becomes R.AR's partial (gram_rar_kernel adds the rest). The
+// product then reads neither the diagonal nor v's own entries, 2 s_v bytes
+// per row less, and this pass reads as many bytes as with diag.
+template <typename VT, bool OFF_IN, bool OFF_NEXT>
 __global__ void __launch_bounds__(kBlock)
 fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, VT* W,
                     const VT* __restrict__ AW, VT* __restrict__ P, VT* __restrict__ AP,
-                    const double* __restrict__ diag, const double2* __restrict__ c,
-                    const double2* __restrict__ dstat, double* __restrict__ gpart,
-                    const int32_t* __restrict__ skip) {
+                    const double* __restrict__ diag, const VT* __restrict__ cdiag,
+                    const double2* __restrict__ c, const double2* __restrict__ dstat,
+                    double* __restrict__ gpart, const int32_t* __restrict__ skip) {
   if (skip && skip[0] != 0) return;
   __shared__ double red[2 * kGramSums * (kBlock / kWave)];
   // c[0..2] = c0, c1, c2; c[3..4] = d1, d2; c[5].x = theta
@@ -222,13 +229,23 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
   for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * kBlock) {
     const double2 x = Vec<VT>::load(X, i), w = Vec<VT>::load(W, i), p = Vec<VT>::load(P, i);
-    const double2 ax = Vec<VT>::load(AX, i), aw = Vec<VT>::load(AW, i), ap = Vec<VT>::load(AP, i);
+    const double2 ax = Vec<VT>::load(AX, i), ap = Vec<VT>::load(AP, i);
+    double2 aw = Vec<VT>::load(AW, i);
+    double dreal = 0.0;
+    double2 dc = make_double2(0.0, 0.0);
+    if (cdiag) {
+      dc = Vec<VT>::load(cdiag, i);
+      dreal = dc.x;
+    } else if (diag) {
+      dreal = diag[i];
+    }
+    if constexpr (OFF_IN) aw = cadd(aw, cmul(dc, w));
     const double2 xn = cadd(cadd(cmul(c0, x), cmul(c1, w)), cmul(c2, p));
     const double2 axn = cadd(cadd(cmul(c0, ax), cmul(c1, aw)), cmul(c2, ap));
     const double2 pn = cadd(cmul(d1, w), cmul(d2, p));
     const double2 apn = cadd(cmul(d1, aw), cmul(d2, ap));
     const double2 rn = csub(axn, cscale(xn, theta));
-    const double2 wn = diag ? cscale(rn, precond_scale(diag[i], theta, st)) : rn;
+    const double2 wn = diag ? cscale(rn, precond_scale(dreal, theta, st)) : rn;
     Vec<VT>::store(X, i, xn);
     Vec<VT>::store(AX, i, axn);
     Vec<VT>::store(P, i, pn);
@@ -245,15 +262,15 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
       const double2 rs = Vec<VT>::round(rn);
       // squared norms (sums 0, 1, 2, 12, 13) are real: only their real parts
       auto nrm = [](double2 a) { return make_double2(a.x * a.x + a.y * a.y, 0.0); };
+      const double ww = ws.x * ws.x + ws.y * ws.y;
       const double2 prods[kGramSums] = {
           nrm(xs), nrm(ws), nrm(ps), cmulc(xs, ws), cmulc(xs, ps),
-          cmulc(ws, ps), cmulc(xs, axs), make_double2(0.0, 0.0), cmulc(ps, aps),
+          cmulc(ws, ps), cmulc(xs, axs), make_double2(dc.x * ww, dc.y * ww), cmulc(ps, aps),
           cmulc(axs, ws), cmulc(xs, aps), cmulc(ws, aps), nrm(axs), nrm(rs)};
-      constexpr unsigned kReal = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 7) | (1u << 12) |
-                                 (1u << 13);
+      constexpr unsigned kReal = (1u << 0) | (1u << 1) | (1u << 2) | (1u << 12) | (1u << 13);
 #pragma unroll
       for (int k = 0; k < kGramSums; ++k) {
-        if (k == 7) continue;
+        if (k == 7 && !OFF_NEXT) continue;
         acc[2 * k] += prods[k].x;
         if (!((kReal >> k) & 1u)) acc[2 * k + 1] += prods[k].y;
       }
@@ -263,7 +280,7 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
   block_sum<2 * kGramSums>(acc, red);
   if (threadIdx.x == 0) {
     for (int k = 0; k < 2 * kGramSums; ++k) {
-      if (k == 14 || k == 15) continue;   // R.AR: gram_rar_kernel's
+      if (!OFF_NEXT && (k == 14 || k == 15)) continue;   // R.AR: gram_rar_kernel's
       gpart[(int64_t)k * gridDim.x + blockIdx.x] = acc[k];
     }
   }
@@ -274,7 +291,7 @@ fused_update_kernel(int64_t n, VT* __restrict__ X, VT* __restrict__ AX, VT* R, V
 template <typename VT>
 __global__ void __launch_bounds__(kBlock)
 gram_rar_kernel(int64_t n, const VT* __restrict__ W, const VT* __restrict__ AW,
-                double* __restrict__ gpart, const int32_t* __restrict__ skip) {
+                double* __restrict__ gpart, const int32_t* __restrict__ skip, int accumulate) {
   if (skip && skip[0] != 0) return;
   __shared__ double red[2 * (kBlock / kWave)];
   double acc[2] = {0.0, 0.0};
@@ -286,8 +303,11 @@ gram_rar_kernel(int64_t n, const VT* __restrict__ W, const VT* __restrict__ AW,
   }
   block_sum<2>(acc, red);
   if (threadIdx.x == 0) {
-    gpart[(int64_t)14 * gridDim.x + blockIdx.x] = acc[0];
-    gpart[(int64_t)15 * gridDim.x + blockIdx.x] = acc[1];
+    // (accumulate: the update pass left the diagonal's share there)
+    double* g0 = gpart + (int64_t)14 * gridDim.x + blockIdx.x;
+    double* g1 = gpart + (int64_t)15 * gridDim.x + blockIdx.x;
+    *g0 = accumulate ? *g0 + acc[0] : acc[0];
+    *g1 = accumulate ? *g1 + acc[1] : acc[1];
   }
 }
 
@@ -441,36 +461,54 @@ int lincomb_impl(int64_t n, int precision, void* out, int k, const void* const* 
   return SQDK_OK;
 }
 
+template <typename VT>
+void launch_fused_update(unsigned g, hipStream_t s, int64_t n, void* X, void* AX, void* R,
+                         void* W, const void* AW, void* P, void* AP, const double* diag,
+                         const void* cdiag, int mode, const double2* c, const double2* st,
+                         double* gpart, const int32_t* skip) {
+  auto run = [&](auto kern) {
+    kern<<<g, kBlock, 0, s>>>(n, (VT*)X, (VT*)AX, (VT*)R, (VT*)W, (const VT*)AW, (VT*)P, (VT*)AP,
+                              diag, (const VT*)cdiag, c, st, gpart, skip);
+  };
+  switch (mode) {
+    case 1: run(fused_update_kernel<VT, true, false>); break;
+    case 2: run(fused_update_kernel<VT, false, true>); break;
+    case 3: run(fused_update_kernel<VT, true, true>); break;
+    default: run(fused_update_kernel<VT, false, false>); break;
+  }
+}
+
 int fused_update_impl(int64_t n, int precision, void* X, void* AX, void* R, void* W,
                       const void* AW, void* P, void* AP, const double* diag,
                       const double* slots, int c_base, int stat_slot, double* gpart,
-                      hipStream_t s, const int32_t* skip) {
+                      hipStream_t s, const int32_t* skip, const void* cdiag, int cdiag_mode) {
+  SQDK_REQUIRE(cdiag_mode >= 0 && cdiag_mode <= 3, "cdiag_mode %d", cdiag_mode);
+  SQDK_REQUIRE(!cdiag_mode || (cdiag && diag), "the diagonal split needs the preconditioner's "
+               "diagonal and the hit table's");
+  SQDK_REQUIRE(!(cdiag_mode & 2) || gpart, "the diagonal's R.AR share needs gpart");
   const unsigned g = stream_grid(n);
   static_assert(kGramPartialDoubles >= (int64_t)2 * kGramSums * 8192, "split Gram partials");
   const double2* c = reinterpret_cast<const double2*>(slots) + c_base;
   const double2* st = reinterpret_cast<const double2*>(slots) + stat_slot;
   if (precision == SQDK_C128)
-    fused_update_kernel<double2><<<g, kBlock, 0, s>>>(n, (double2*)X, (double2*)AX, (double2*)R,
-                                                      (double2*)W, (const double2*)AW,
-                                                      (double2*)P, (double2*)AP, diag, c, st,
-                                                      gpart, skip);
+    launch_fused_update<double2>(g, s, n, X, AX, R, W, AW, P, AP, diag, cdiag, cdiag_mode, c, st,
+                                 gpart, skip);
   else
-    fused_update_kernel<float2><<<g, kBlock, 0, s>>>(n, (float2*)X, (float2*)AX, (float2*)R,
-                                                     (float2*)W, (const float2*)AW, (float2*)P,
-                                                     (float2*)AP, diag, c, st, gpart, skip);
+    launch_fused_update<float2>(g, s, n, X, AX, R, W, AW, P, AP, diag, cdiag, cdiag_mode, c, st,
+                                gpart, skip);
   SQDK_LAUNCHED("fused_update_kernel");
   return SQDK_OK;
 }
 
 int gram_rar_impl(int64_t n, int precision, const void* W, const void* AW, double* gpart,
-                  double* out, hipStream_t s, const int32_t* skip) {
+                  double* out, hipStream_t s, const int32_t* skip, bool accumulate) {
   const unsigned g = stream_grid(n);   // the fused update's grid: its partials share gpart
   if (precision == SQDK_C128)
     gram_rar_kernel<double2><<<g, kBlock, 0, s>>>(n, (const double2*)W, (const double2*)AW,
-                                                  gpart, skip);
+                                                  gpart, skip, accumulate ? 1 : 0);
   else
     gram_rar_kernel<float2><<<g, kBlock, 0, s>>>(n, (const float2*)W, (const float2*)AW, gpart,
-                                                 skip);
+                                                 skip, accumulate ? 1 : 0);
   SQDK_LAUNCHED("gram_rar_kernel");
   dots_final_kernel<<<2 * kGramSums, kBlock, 0, s>>>((int)g, gpart, out, skip);
   SQDK_LAUNCHED("dots_final_kernel");
