@@ -1450,7 +1450,9 @@ template <> __device__ inline double2 from_d2<double2>(double2 a) { return a; }
 // row's in group order.
 // In a batch, the slices of an instance whose flag inst_skip is set are left
 // alone (slices_per_item slices per chunk, a multiple of S).
-template <typename VT, int S>
+// DIAG: the rows' diagonal products enter (diag and v's own entries read);
+// without it the product off the diagonal.
+template <typename VT, int S, bool DIAG>
 __global__ void __launch_bounds__(kBlock)
 hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
                   const uint64_t* __restrict__ sbase, const uint32_t* __restrict__ col,
@@ -1473,7 +1475,7 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     const uint64_t b0 = sbase[sid0] + lane;
     const int len = (int)rlen[row];
     double sr = 0.0, si = 0.0;
-    if (diag) {
+    if constexpr (DIAG) {
       const double2 dd = to_d2(diag[row]), xd = to_d2(v[row]);
       sr = dd.x * xd.x - dd.y * xd.y;
       si = dd.x * xd.y + dd.y * xd.x;
@@ -1509,7 +1511,7 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     out[row] = from_d2<VT>(make_double2(sr, si));
     return;
   }
-  bool valid[S], wide[S];
+  bool valid[S];
   int64_t row[S];
   uint64_t b0[S];
   int len[S];
@@ -1521,16 +1523,14 @@ hit_matvec_kernel(int64_t n_slices, const uint32_t* __restrict__ slot_row,
     valid[u] = sid0 + u < n_slices && slot != kHitPad;
     row[u] = valid[u] ? (int64_t)slot : 0;
     b0[u] = sbase[sid] + lane;
-    // a slice of rows without hits has no entries: its row lengths are not read
-    wide[u] = sbase[sid + 1] > sbase[sid];
   }
   int kmax = 0;
 #pragma unroll
   for (int u = 0; u < S; ++u) {
-    len[u] = (valid[u] && wide[u]) ? (int)rlen[row[u]] : 0;
+    len[u] = valid[u] ? (int)rlen[row[u]] : 0;
     kmax = max(kmax, len[u]);
     sr[u] = si[u] = 0.0;
-    if (diag) {
+    if constexpr (DIAG) {
       const double2 dd = to_d2(diag[row[u]]), xd = to_d2(v[row[u]]);
       sr[u] = valid[u] ? dd.x * xd.x - dd.y * xd.y : 0.0;
       si[u] = valid[u] ? dd.x * xd.y + dd.y * xd.x : 0.0;
@@ -1842,11 +1842,19 @@ int hit_matvec_impl(const sqdk_cache* c, int32_t precision, const void* v, void*
         slices_per_item > 0 ? slices_per_item : 1);
   };
   if (precision == SQDK_C128) {
-    if (spw == 4) launch(hit_matvec_kernel<double2, 4>, double2{});
-    else launch(hit_matvec_kernel<double2, 1>, double2{});
+    if (spw == 4)
+      offdiag ? launch(hit_matvec_kernel<double2, 4, false>, double2{})
+              : launch(hit_matvec_kernel<double2, 4, true>, double2{});
+    else
+      offdiag ? launch(hit_matvec_kernel<double2, 1, false>, double2{})
+              : launch(hit_matvec_kernel<double2, 1, true>, double2{});
   } else if (precision == SQDK_C64) {
-    if (spw == 4) launch(hit_matvec_kernel<float2, 4>, float2{});
-    else launch(hit_matvec_kernel<float2, 1>, float2{});
+    if (spw == 4)
+      offdiag ? launch(hit_matvec_kernel<float2, 4, false>, float2{})
+              : launch(hit_matvec_kernel<float2, 4, true>, float2{});
+    else
+      offdiag ? launch(hit_matvec_kernel<float2, 1, false>, float2{})
+              : launch(hit_matvec_kernel<float2, 1, true>, float2{});
   } else {
     return fail(SQDK_EINVAL, "unknown precision %d", precision);
   }
