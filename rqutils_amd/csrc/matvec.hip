@@ -1304,15 +1304,19 @@ hit_table_kernel(const uint64_t* __restrict__ keys, const uint32_t* __restrict__
         }
         bool hit[ROWS];
         lift_search<KT, ROWS>(ba, tl, steps, last, hit);
-        int idx[ROWS];
+        // only the chunk's rows are recorded; the indices are formed in the
+        // rare case of a hit in the wave (the test itself is on lane masks)
         bool any_hit = false;
 #pragma unroll
         for (int r = 0; r < ROWS; ++r) {
-          // only the chunk's rows are recorded
-          idx[r] = (valid[r] && hit[r]) ? (int)((ba[r] - kaddr) / (uint32_t)sizeof(KT)) : -1;
-          any_hit |= idx[r] >= 0;
+          hit[r] = hit[r] && valid[r];
+          any_hit |= hit[r];
         }
         if (__any(any_hit)) {
+          int idx[ROWS];
+#pragma unroll
+          for (int r = 0; r < ROWS; ++r)
+            idx[r] = hit[r] ? (int)((ba[r] - kaddr) / (uint32_t)sizeof(KT)) : -1;
           record(idx, qs, gt0, gt1);
           class_any = true;
         }
